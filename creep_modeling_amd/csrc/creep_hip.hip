@@ -1,0 +1,909 @@
+// creep_hip.hip -- C-ABI of libcreep_hip.so (include/creep_hip.h): context
+// management, host-side macroscopic updates and the per-increment / per-pass
+// orchestration of the HIP kernels on one stream.
+//
+// Host-side restatements here: calc_velgradavg_corr + state_6x6_evpsc
+// (various_functions.f90:1526-1705), the dL_iter / mixing rule and error
+// normalisation (LS-EVPFFT.f90:313-343), the macro averages
+// (LS-EVPFFT.f90:460-523) and the macro tangents of calc_c0 (:2633-2647).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/creep_hip.h"
+#include "internal.h"
+#include "tensor.h"
+
+using namespace cg;
+
+namespace {
+
+struct HostPhase {
+    int igas = 0, nsyst = 0, nmodes = 0;
+    int nsm[MAXMODES] = {0};
+    std::vector<double> dnca, dbca, tau, thet, nrs, gamd0, hpfac, hard;
+    double C[81] = {0};
+    bool set = false;
+};
+
+// b-basis expansion of a 6x6 into a 4th-order tensor: C4 = sum_ab c_ab b_a (x) b_b
+void to4(const double* c66, double* c4) {
+    double B[6][9];
+    for (int a = 0; a < 6; a++) basis(a, B[a]);
+    for (int p = 0; p < 9; p++)
+        for (int q = 0; q < 9; q++) {
+            double s = 0.0;
+            for (int a = 0; a < 6; a++) {
+                if (B[a][p] == 0.0) continue;
+                for (int b = 0; b < 6; b++) s += c66[6 * a + b] * B[a][p] * B[b][q];
+            }
+            c4[9 * p + q] = s;
+        }
+}
+// b-basis components of a 4th-order tensor: c_ab = b_a : C4 : b_b
+void from4(const double* c4, double* c66) {
+    double B[6][9];
+    for (int a = 0; a < 6; a++) basis(a, B[a]);
+    for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) {
+            double s = 0.0;
+            for (int p = 0; p < 9; p++) {
+                if (B[a][p] == 0.0) continue;
+                for (int q = 0; q < 9; q++) s += B[a][p] * c4[9 * p + q] * B[b][q];
+            }
+            c66[6 * a + b] = s;
+        }
+}
+const int IJV[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+
+// state_6x6_evpsc (various_functions.f90:1638-1705)
+bool state6x6(const int* idsim, const int* iscau, const double* mev66, const double* devp0, double* scauchy,
+              double* dsim) {
+    double profac[6], dv[6], sv[6], m4[81], mv[36], d33[9], devv[6], a6[6], aa[6], A[36];
+    for (int i = 0; i < 6; i++) profac[i] = (i >= 3) ? 2.0 : 1.0;
+    for (int i = 0; i < 6; i++) {
+        dv[i] = dsim[3 * IJV[i][0] + IJV[i][1]];
+        sv[i] = scauchy[3 * IJV[i][0] + IJV[i][1]];
+    }
+    to4(mev66, m4);
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) mv[6 * i + j] = m4[27 * IJV[i][0] + 9 * IJV[i][1] + 3 * IJV[j][0] + IJV[j][1]];
+    from6(devp0, d33);
+    for (int i = 0; i < 6; i++) devv[i] = d33[3 * IJV[i][0] + IJV[i][1]];
+    for (int i = 0; i < 6; i++) a6[i] = dv[i] - devv[i];
+    for (int i = 0; i < 6; i++) {
+        double r = -1.0 * idsim[i] * a6[i];
+        for (int j = 0; j < 6; j++) {
+            r += mv[6 * i + j] * iscau[j] * sv[j] * profac[j];
+            A[6 * i + j] = iscau[j] * (i == j ? 1.0 : 0.0) - idsim[j] * mv[6 * i + j] * profac[j];
+        }
+        aa[i] = r;
+    }
+    bool ok = solve6(A, aa);
+    for (int i = 0; i < 6; i++) {
+        a6[i] = idsim[i] * a6[i] + iscau[i] * aa[i];
+        sv[i] = iscau[i] * sv[i] + idsim[i] * aa[i];
+    }
+    for (int i = 0; i < 6; i++) dv[i] = a6[i] + devv[i];
+    for (int i = 0; i < 6; i++) {
+        dsim[3 * IJV[i][0] + IJV[i][1]] = dv[i];
+        dsim[3 * IJV[i][1] + IJV[i][0]] = dv[i];
+        scauchy[3 * IJV[i][0] + IJV[i][1]] = sv[i];
+        scauchy[3 * IJV[i][1] + IJV[i][0]] = sv[i];
+    }
+    return ok;
+}
+
+void sym3(const double* a, double* s) {
+    double t[9];
+    t[0] = a[0]; t[4] = a[4]; t[8] = a[8];
+    t[1] = t[3] = 0.5 * (a[1] + a[3]);
+    t[2] = t[6] = 0.5 * (a[2] + a[6]);
+    t[5] = t[7] = 0.5 * (a[5] + a[7]);
+    for (int i = 0; i < 9; i++) s[i] = t[i];
+}
+void skw3(const double* a, double* s) {
+    double t[9];
+    t[0] = t[4] = t[8] = 0.0;
+    t[1] = 0.5 * (a[1] - a[3]); t[3] = -t[1];
+    t[2] = 0.5 * (a[2] - a[6]); t[6] = -t[2];
+    t[5] = 0.5 * (a[5] - a[7]); t[7] = -t[5];
+    for (int i = 0; i < 9; i++) s[i] = t[i];
+}
+
+}  // namespace
+
+struct cg_ctx {
+    int n1, n2, n3, nph, modal, do_mix, mixed;
+    long V;
+    double complgas;
+    char err[512];
+    HostPhase ph[MAXPH];
+    int ns = 0;
+    std::vector<double> euler;
+    std::vector<int> grain, phase;
+    bool voxels_set = false, initialised = false;
+    Fields f;
+    PhaseDev* d_ph = nullptr;
+    double* d_tab = nullptr;
+    FftCtx* fft = nullptr;
+    hipStream_t s = nullptr;
+    double* h_red = nullptr;   // pinned
+    double* d_sgsum = nullptr;
+    double* d_fluct = nullptr;
+    size_t part_cap = 0;
+    int sgsum_valid = 0, lold_same = 0, int_pow = 0, ngas = 0;
+    long last_solid = -1;
+    double nsys_solid = 0.0;
+    // macroscopic state (micro_type, types.f90:55-94)
+    double c066[36], c0[81], s0[81], de_ds[81], goperr0[81];
+    double wphsol, wphgas, wphcsol, wphcgas;
+    double defgradavg[9], defgradinvavgc[9], defgradinvavgcs[9], defgradinvavgcg[9];
+    double dvelgradavg[9], velgradavg[9], velgradavgsol[9], velgradavggas[9];
+    double sgavg[9], sgavgsol[9], sgsolavg[9];
+    double svmavg, dvmavg, lvmavg, evmavg, edotvmmx;
+    double epavg[9], eavg[9], eelavg[9], edotpavg[9], evmpavg, dvmpavg;
+    double dsim_snap[9], scauchy_snap[9];
+    // timing
+    int timing = 0;
+    hipEvent_t ev[8];
+    cg_stats st;
+};
+
+static int fail(cg_ctx* c, int code, const char* fmt, ...) {
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof c->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+#define HIPCHK(c, x)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (x);                                                                  \
+        if (e_ != hipSuccess) return fail((c), -2, "%s: %s", #x, hipGetErrorString(e_));      \
+    } while (0)
+
+static int dalloc(cg_ctx* c, double** p, size_t n) {
+    if (n == 0) { *p = nullptr; return 0; }
+    hipError_t e = hipMalloc(p, n * sizeof(double));
+    if (e != hipSuccess) return fail(c, -3, "hipMalloc(%zu doubles): %s", n, hipGetErrorString(e));
+    return 0;
+}
+
+static int ensure_part(cg_ctx* c, size_t n) {
+    if (n <= c->part_cap) return 0;
+    if (c->f.part) hipFree(c->f.part);
+    c->f.part = nullptr;
+    if (dalloc(c, &c->f.part, n)) return -3;
+    c->part_cap = n;
+    return 0;
+}
+
+// reduce nvals partial columns into h_red[0..nvals) (synchronous)
+static int reduce_to_host(cg_ctx* c, int nblocks, int nvals, double* host_out) {
+    launch_reduce_final(c->f.part, nblocks, nvals, c->f.red, c->s);
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->f.red, nvals * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    memcpy(host_out, c->h_red, nvals * sizeof(double));
+    return 0;
+}
+
+extern "C" {
+
+int cg_create(int n1, int n2, int n3, int nph, double complgas, int modal, int do_mix, int precision,
+              cg_ctx** out) {
+    if (!out) return -1;
+    *out = nullptr;
+    if (n1 < 2 || n2 < 1 || n3 < 1 || nph < 1 || nph > MAXPH) return -1;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return -4;
+    cg_ctx* c = new cg_ctx();
+    memset(c->err, 0, sizeof c->err);
+    c->n1 = n1; c->n2 = n2; c->n3 = n3; c->nph = nph;
+    c->V = (long)n1 * n2 * n3;
+    c->complgas = complgas; c->modal = modal; c->do_mix = do_mix; c->mixed = precision ? 1 : 0;
+    memset(&c->f, 0, sizeof c->f);
+    memset(&c->st, 0, sizeof c->st);
+    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return -2; }
+    for (int i = 0; i < 8; i++) hipEventCreate(&c->ev[i]);
+    *out = c;
+    return 0;
+}
+
+static void free_fields(cg_ctx* c) {
+    Fields& f = c->f;
+    void* ptrs[] = {f.phase, f.gas, f.w, f.sg, f.sgt, f.L, f.Lold, f.dLref, f.F, f.Fi, f.J, f.Fe, f.Fp, f.ag,
+                    f.sg66, f.c66m, f.cgm, f.cgas, f.sch, f.crss, f.ttau, f.gdot, f.gacc, f.edotp, f.ept, f.dgs,
+                    f.epvm, f.Finc, f.Feinc, f.s_L, f.s_Lold, f.s_sg, f.s_edotp, f.s_gdot, f.s_ttau, f.spec,
+                    f.specf, f.xnode, f.part, f.red};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    memset(&c->f, 0, sizeof c->f);
+}
+
+int cg_destroy(cg_ctx* c) {
+    if (!c) return 0;
+    if (c->s) hipStreamSynchronize(c->s);
+    free_fields(c);
+    if (c->d_ph) hipFree(c->d_ph);
+    if (c->d_tab) hipFree(c->d_tab);
+    if (c->d_sgsum) hipFree(c->d_sgsum);
+    if (c->d_fluct) hipFree(c->d_fluct);
+    if (c->h_red) hipHostFree(c->h_red);
+    if (c->fft) fft_destroy(c->fft);
+    for (int i = 0; i < 8; i++) hipEventDestroy(c->ev[i]);
+    if (c->s) hipStreamDestroy(c->s);
+    delete c;
+    return 0;
+}
+
+const char* cg_last_error(cg_ctx* c) { return c ? c->err : "null context"; }
+
+int cg_set_phase(cg_ctx* c, int iph, int igas, int nsyst, int nmodes, const int* nsm, const double* dnca,
+                 const double* dbca, const double* tau, const double* thet, const double* nrs,
+                 const double* gamd0, const double* hpfac, const double* hard, const double* C) {
+    if (!c) return -1;
+    if (iph < 1 || iph > c->nph) return fail(c, -1, "phase index %d out of range", iph);
+    HostPhase& P = c->ph[iph - 1];
+    P.igas = igas;
+    P.set = true;
+    if (igas) return 0;
+    if (nsyst < 1 || nmodes < 1 || nmodes > MAXMODES) return fail(c, -1, "bad phase sizes");
+    int tot = 0;
+    for (int m = 0; m < nmodes; m++) { P.nsm[m] = nsm[m]; tot += nsm[m]; }
+    if (tot != nsyst) return fail(c, -1, "sum(nsm) != nsyst");
+    P.nsyst = nsyst;
+    P.nmodes = nmodes;
+    P.dnca.assign(dnca, dnca + 3 * nsyst);
+    P.dbca.assign(dbca, dbca + 3 * nsyst);
+    P.tau.assign(tau, tau + 3 * nsyst);
+    P.thet.assign(thet, thet + 2 * nsyst);
+    P.nrs.assign(nrs, nrs + nsyst);
+    P.gamd0.assign(gamd0, gamd0 + nsyst);
+    P.hpfac.assign(hpfac, hpfac + nsyst);
+    P.hard.assign(hard, hard + (size_t)nsyst * nsyst);
+    memcpy(P.C, C, sizeof P.C);
+    c->ns = std::max(c->ns, nsyst);
+    return 0;
+}
+
+int cg_set_voxels(cg_ctx* c, const double* euler, const int* grain, const int* phase) {
+    if (!c) return -1;
+    c->euler.assign(euler, euler + 3 * c->V);
+    c->grain.assign(grain, grain + c->V);
+    c->phase.assign(phase, phase + c->V);
+    for (long v = 0; v < c->V; v++)
+        if (c->phase[v] < 1 || c->phase[v] > c->nph) return fail(c, -1, "phase id out of range at voxel %ld", v);
+    c->voxels_set = true;
+    return 0;
+}
+
+int cg_init(cg_ctx* c, const cg_process* p1) {
+    if (!c || !p1) return -1;
+    if (!c->voxels_set) return fail(c, -1, "cg_set_voxels not called");
+    for (int i = 0; i < c->nph; i++)
+        if (!c->ph[i].set) return fail(c, -1, "phase %d not set", i + 1);
+    const long V = c->V;
+    Fields& f = c->f;
+    f.n1 = c->n1; f.n2 = c->n2; f.n3 = c->n3; f.h1 = c->n1 / 2 + 1; f.V = V;
+    f.ns = std::max(c->ns, 1);
+    const size_t ns = f.ns;
+    // phase tables
+    PhaseDev hp[MAXPH];
+    memset(hp, 0, sizeof hp);
+    std::vector<double> tab;
+    int all_int = 1;
+    for (int i = 0; i < c->nph; i++) {
+        HostPhase& P = c->ph[i];
+        PhaseDev& D = hp[i];
+        D.igas = P.igas;
+        if (P.igas) continue;
+        D.nsyst = P.nsyst;
+        D.nmodes = P.nmodes;
+        int last = 0;
+        for (int m = 0; m < P.nmodes; m++) {
+            D.nsm[m] = P.nsm[m];
+            last += P.nsm[m];
+            // harden reads the mode parameters from the mode's last system (various_functions.f90:1466-1470)
+            D.mode_tau1[m] = P.tau[3 * (last - 1) + 2];
+            D.mode_thet0[m] = P.thet[2 * (last - 1)];
+            D.mode_thet1[m] = P.thet[2 * (last - 1) + 1];
+        }
+        from4(P.C, D.c66cr);
+        auto put = [&](const std::vector<double>& v) { int off = (int)tab.size(); tab.insert(tab.end(), v.begin(), v.end()); return off; };
+        D.off_dnca = put(P.dnca);
+        D.off_dbca = put(P.dbca);
+        D.off_nrs = put(P.nrs);
+        D.off_gamd0 = put(P.gamd0);
+        D.off_hard = put(P.hard);
+        D.off_tau = put(P.tau);
+        D.off_hpfac = put(P.hpfac);
+        for (double n : P.nrs)
+            if (!(n == floor(n) && n >= 1.0 && n <= 64.0)) all_int = 0;
+    }
+    c->int_pow = all_int;
+    HIPCHK(c, hipMalloc(&c->d_ph, sizeof hp));
+    HIPCHK(c, hipMemcpy(c->d_ph, hp, sizeof hp, hipMemcpyHostToDevice));
+    if (dalloc(c, &c->d_tab, std::max<size_t>(tab.size(), 1))) return -3;
+    if (!tab.empty()) HIPCHK(c, hipMemcpy(c->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+
+    // per-voxel ids
+    std::vector<uint8_t> hph(V), hgas(V);
+    c->ngas = 0;
+    c->last_solid = -1;
+    c->nsys_solid = 0.0;
+    for (long v = 0; v < V; v++) {
+        hph[v] = (uint8_t)(c->phase[v] - 1);
+        hgas[v] = (uint8_t)(c->ph[c->phase[v] - 1].igas ? 1 : 0);
+        if (hgas[v]) c->ngas++;
+        else { c->last_solid = v; c->nsys_solid += c->ph[c->phase[v] - 1].nsyst; }
+    }
+    HIPCHK(c, hipMalloc(&f.phase, V));
+    HIPCHK(c, hipMalloc(&f.gas, V));
+    HIPCHK(c, hipMemcpy(f.phase, hph.data(), V, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(f.gas, hgas.data(), V, hipMemcpyHostToDevice));
+    int rc = 0;
+    rc |= dalloc(c, &f.w, V); rc |= dalloc(c, &f.sg, 6 * V); rc |= dalloc(c, &f.sgt, 6 * V);
+    rc |= dalloc(c, &f.L, 9 * V); rc |= dalloc(c, &f.Lold, 9 * V); rc |= dalloc(c, &f.dLref, 9 * V);
+    rc |= dalloc(c, &f.F, 9 * V); rc |= dalloc(c, &f.Fi, 9 * V); rc |= dalloc(c, &f.J, V);
+    rc |= dalloc(c, &f.Fe, 9 * V); rc |= dalloc(c, &f.Fp, 9 * V); rc |= dalloc(c, &f.ag, 9 * V);
+    rc |= dalloc(c, &f.sg66, 36 * V); rc |= dalloc(c, &f.c66m, 36 * V);
+    if (c->modal) rc |= dalloc(c, &f.cgm, 36 * V);
+    if (c->ngas) rc |= dalloc(c, &f.cgas, 36 * V);
+    rc |= dalloc(c, &f.sch, 5 * ns * V); rc |= dalloc(c, &f.crss, 2 * ns * V); rc |= dalloc(c, &f.ttau, 2 * ns * V);
+    rc |= dalloc(c, &f.gdot, ns * V); rc |= dalloc(c, &f.gacc, V);
+    rc |= dalloc(c, &f.edotp, 6 * V); rc |= dalloc(c, &f.ept, 6 * V); rc |= dalloc(c, &f.dgs, 6 * V);
+    rc |= dalloc(c, &f.epvm, V); rc |= dalloc(c, &f.Finc, 9 * V); rc |= dalloc(c, &f.Feinc, 9 * V);
+    rc |= dalloc(c, &f.s_L, 9 * V); rc |= dalloc(c, &f.s_Lold, 9 * V); rc |= dalloc(c, &f.s_sg, 6 * V);
+    rc |= dalloc(c, &f.s_edotp, 6 * V); rc |= dalloc(c, &f.s_gdot, ns * V); rc |= dalloc(c, &f.s_ttau, 2 * ns * V);
+    const size_t nspec = (size_t)f.h1 * f.n2 * f.n3 * 9;
+    if (c->mixed) HIPCHK(c, hipMalloc(&f.specf, nspec * sizeof(float2)));
+    else HIPCHK(c, hipMalloc(&f.spec, nspec * sizeof(double2)));
+    rc |= dalloc(c, &f.red, 256);
+    if (rc) return rc;
+    if (ensure_part(c, (size_t)((V + 255) / 256 + 64) * 81)) return -3;
+    HIPCHK(c, hipHostMalloc(&c->h_red, 256 * sizeof(double)));
+    if (dalloc(c, &c->d_sgsum, 16) || dalloc(c, &c->d_fluct, 16)) return -3;
+    c->fft = fft_create(f, c->err, sizeof c->err);
+    if (!c->fft) return -5;
+
+    // voxel initialisation
+    int gmax = 0;
+    for (long v = 0; v < V; v++) gmax = std::max(gmax, c->grain[v]);
+    std::vector<long> nvox((size_t)(gmax + 1) * c->nph, 0);
+    for (long v = 0; v < V; v++)
+        if (!hgas[v]) {
+            if (c->grain[v] < 1) return fail(c, -1, "grain id out of bounds for Hall-Petch init");
+            nvox[(size_t)c->grain[v] * c->nph + c->phase[v] - 1]++;
+        }
+    // Hall-Petch shift per grain id: 252.8/sqrt(max((64 N)^(1/3), 12)) (IO_functions.f90:107-169);
+    // a grain spread over several phases uses its per-phase size, so tabulate per (grain, phase)
+    if (c->nph > 1) {
+        for (int g = 1; g <= gmax; g++) {
+            int nonzero = 0;
+            for (int p = 0; p < c->nph; p++) nonzero += nvox[(size_t)g * c->nph + p] > 0;
+            if (nonzero > 1) return fail(c, -1, "grain %d spans several phases (unsupported)", g);
+        }
+    }
+    std::vector<double> hpg(gmax + 1, 0.0);
+    const double hpk = 632.0 / 2.5, vvol = 64.0, dmin = 3.0 * pow(vvol, 1.0 / 3.0);
+    for (int g = 1; g <= gmax; g++) {
+        long n = 0;
+        for (int p = 0; p < c->nph; p++) n += nvox[(size_t)g * c->nph + p];
+        if (n > 0) hpg[g] = hpk / sqrt(std::max(pow((double)n * vvol, 1.0 / 3.0), dmin));
+    }
+    double *d_eul = nullptr, *d_hp = nullptr;
+    int* d_gr = nullptr;
+    if (dalloc(c, &d_eul, 3 * V) || dalloc(c, &d_hp, gmax + 1)) return -3;
+    HIPCHK(c, hipMalloc(&d_gr, V * sizeof(int)));
+    HIPCHK(c, hipMemcpy(d_eul, c->euler.data(), 3 * V * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d_hp, hpg.data(), (gmax + 1) * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d_gr, c->grain.data(), V * sizeof(int), hipMemcpyHostToDevice));
+    PhaseTables pt{c->d_ph, c->d_tab};
+    int nb;
+    launch_init_voxels(f, pt, d_eul, d_gr, d_hp, p1->udot, p1->tdot, c->s, &nb);
+    double sums[36];
+    if (reduce_to_host(c, nb, 36, sums)) return -2;
+    hipFree(d_eul); hipFree(d_hp); hipFree(d_gr);
+    // wphsol / wphgas (IO_functions.f90:347-363) and micro-level defaults (types.f90:338-341)
+    c->wphsol = 0.0; c->wphgas = 0.0;
+    const double wgt = 1.0 / (double)V;
+    for (long v = 0; v < V; v++) { if (hgas[v]) c->wphgas += wgt; else c->wphsol += wgt; }
+    c->wphcsol = c->wphsol; c->wphcgas = c->wphgas;
+    const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    memcpy(c->defgradavg, I3, sizeof I3); memcpy(c->defgradinvavgc, I3, sizeof I3);
+    memcpy(c->defgradinvavgcs, I3, sizeof I3); memcpy(c->defgradinvavgcg, I3, sizeof I3);
+    memset(c->dvelgradavg, 0, sizeof c->dvelgradavg);
+    memset(c->velgradavg, 0, sizeof c->velgradavg);
+    memset(c->sgavg, 0, sizeof c->sgavg);
+    memset(c->sgsolavg, 0, sizeof c->sgsolavg);
+    memset(c->goperr0, 0, sizeof c->goperr0);
+    memset(c->epavg, 0, sizeof c->epavg); memset(c->eavg, 0, sizeof c->eavg);
+    memset(c->eelavg, 0, sizeof c->eelavg); memset(c->edotpavg, 0, sizeof c->edotpavg);
+    c->svmavg = 0.0; c->evmavg = 0.0; c->edotvmmx = 0.0; c->evmpavg = 0.0; c->dvmpavg = 0.0;
+    // initial_guess tail: dvmavg = vm(dsim), lvmavg = dvmavg (various_functions.f90:50-51)
+    c->dvmavg = vm_strain(p1->dsim);
+    c->lvmavg = c->dvmavg;
+    // initial C0 (LS-EVPFFT.f90:156-173)
+    double s066[36];
+    for (int i = 0; i < 36; i++) c->c066[i] = sums[i] / c->wphcsol * p1->tdot;
+    memcpy(s066, c->c066, sizeof s066);
+    bool ok = inv6(s066);
+    to4(c->c066, c->c0);
+    to4(s066, c->s0);
+    memcpy(c->de_ds, c->s0, sizeof c->s0);
+    c->sgsum_valid = 0;
+    c->lold_same = 0;
+    c->initialised = true;
+    HIPCHK(c, hipGetLastError());
+    if (!ok) return fail(c, 1, "Singular matrix in lu_inverse (initial C0)");
+    return 0;
+}
+
+static void tstart(cg_ctx* c, int k) {
+    if (c->timing) hipEventRecord(c->ev[2 * k], c->s);
+}
+static void tstop(cg_ctx* c, int k) {
+    if (c->timing) hipEventRecord(c->ev[2 * k + 1], c->s);
+}
+static void tcollect(cg_ctx* c, int k) {
+    if (!c->timing) return;
+    float ms = 0.0f;
+    hipEventSynchronize(c->ev[2 * k + 1]);
+    hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]);
+    c->st.t_ms[k] += ms;
+}
+
+int cg_begin_increment(cg_ctx* c, const cg_process* p, int do_schmid) {
+    if (!c || !c->initialised || !p) return -1;
+    tstart(c, 3);
+    memcpy(c->dsim_snap, p->dsim, sizeof c->dsim_snap);
+    memcpy(c->scauchy_snap, p->scauchy, sizeof c->scauchy_snap);
+    launch_snapshot(c->f, 0, c->s);
+    PhaseTables pt{c->d_ph, c->d_tab};
+    if (do_schmid) launch_update_schmid(c->f, pt, c->s);
+    if (c->modal) {  // form_G: Goperr0 = (1/V) sum Gamma(xi)
+        int nb;
+        launch_goperr0(c->f, c->c0, c->s0, p->igamma, c->s, &nb);
+        double g[81];
+        if (reduce_to_host(c, nb, 81, g)) return -2;
+        for (int i = 0; i < 81; i++) c->goperr0[i] = g[i] / (double)c->V;
+    }
+    launch_c066mod(c->f, c->c066, c->goperr0, c->modal, c->s);
+    tstop(c, 3);
+    HIPCHK(c, hipGetLastError());
+    tcollect(c, 3);
+    return 0;
+}
+
+int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
+    if (!c || !c->initialised || !p || !it) return -1;
+    Fields& f = c->f;
+    const long V = c->V;
+    if (it->iter == 0) {  // LS-EVPFFT.f90:222-231
+        if (c->ngas) launch_gas_compl(f, p->tdot, c->complgas, c->modal, c->s);
+        memset(c->dvelgradavg, 0, sizeof c->dvelgradavg);
+        c->edotvmmx = 0.0;
+        it->mix = 1.0;
+    }
+    it->iter++;
+    int nb;
+    tstart(c, 2);
+    if (!c->sgsum_valid) {
+        launch_sigma_sum(f, c->s, &nb);
+        launch_reduce_final(f.part, nb, 9, c->d_sgsum, c->s);
+    }
+    tstop(c, 2);
+    // stress norm used to scale the FP32 spectral stage of the mixed form
+    double scale = 1.0;
+    if (c->mixed) scale = std::max(c->svmavg, 1.0);
+    tstart(c, 0);
+    nb = fft_green(f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, scale, c->s);
+    launch_reduce_final(f.part, nb, 9, c->d_fluct, c->s);
+    tstop(c, 0);
+    // correction + mixing + constitutive solve (one kernel)
+    PassConst pc;
+    pc.tdot = p->tdot; pc.complgas = c->complgas; pc.error = p->error; pc.erroral = p->erroral;
+    pc.mix = it->mix; pc.lscale = std::max(c->lvmavg, 2.2250738585072014e-308);
+    pc.modal = c->modal; pc.do_mix = c->do_mix; pc.kin_ctrl = (p->iscau[2] == 0);
+    pc.lold_same = c->lold_same; pc.iuphard = p->iuphard;
+    double finv_inv[9];
+    inv3(c->defgradinvavgc, finv_inv);
+    PhaseTables pt{c->d_ph, c->d_tab};
+    tstart(c, 1);
+    launch_correct_solve_full(f, pt, pc, c->dvelgradavg, finv_inv, c->d_fluct, c->s, &nb, c->int_pow);
+    tstop(c, 1);
+    launch_reduce_final(f.part, nb, R_NSOLVE, f.red, c->s);
+    // next pass' <sigma> = sum_all w sigma
+    launch_copy(c->d_sgsum, f.red + R_SGALL, 9, c->s);
+    HIPCHK(c, hipMemcpyAsync(c->h_red, f.red, R_NSOLVE * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    HIPCHK(c, hipGetLastError());
+    tcollect(c, 0); tcollect(c, 1); tcollect(c, 2);
+    c->sgsum_valid = 1;
+    c->lold_same = 1;
+    const double* r = c->h_red;
+    c->st.passes++;
+    c->st.voxel_iters += V;
+    c->st.nr_steps += (int64_t)r[R_NR];
+    c->st.evals += (int64_t)r[R_EVAL];
+    c->st.outer += (int64_t)r[R_OUTER];
+    // correction_to_current averages (various_functions.f90:1132-1137)
+    for (int i = 0; i < 9; i++) {
+        c->velgradavg[i] = r[R_LALL + i];
+        c->velgradavggas[i] = c->wphcgas > 0.0 ? r[R_LGAS + i] / c->wphcgas : r[R_LGAS + i];
+        c->velgradavgsol[i] = r[R_LSOL + i] / c->wphcsol;
+    }
+    // dL_iter -> mix for the next pass (LS-EVPFFT.f90:313-327)
+    double dl = 0.0;
+    if (!pc.kin_ctrl) {
+        const double ds = r[R_WSOL] > 0.0 ? sqrt(r[R_DLSOL] / r[R_WSOL]) : 0.0;
+        const double dg = r[R_WGAS] > 0.0 ? sqrt(r[R_DLGAS] / r[R_WGAS]) : 0.0;
+        dl = std::max(ds, dg);
+        it->mix = std::max(1.0e-8, std::min(1.0, 5.0e-2 / std::max(dl, 5.0e-2)));
+    }
+    // solve_res error sums (various_functions.f90:1357-1360)
+    if (r[R_ERRE] < 0.0 || r[R_ERRS] < 0.0) return fail(c, 2, "ERRE/ERRS negative -> convergence failed!");
+    double erre = sqrt(r[R_ERRE]), errs = sqrt(r[R_ERRS]);
+    // calc_velgradavg_corr (various_functions.f90:1526-1635)
+    for (int i = 0; i < 9; i++) {
+        c->sgavg[i] = r[R_SGSOL + i];
+        c->sgavgsol[i] = r[R_SGSOL + i] / c->wphcsol;
+    }
+    c->svmavg = vm_stress(c->sgavg);
+    double a33[9];
+    for (int i = 0; i < 6; i++) {
+        const int ii = IJV[i][0], jj = IJV[i][1];
+        a33[3 * ii + jj] = p->iscau[i] * (p->scauchy[3 * ii + jj] - c->sgavg[3 * ii + jj]);
+        a33[3 * jj + ii] = p->iscau[i] * (p->scauchy[3 * jj + ii] - c->sgavg[3 * jj + ii]);
+    }
+    double errsbc = 0.0;
+    for (int i = 0; i < 9; i++) errsbc += a33[i] * a33[i];
+    errsbc = sqrt(errsbc);
+    double vs[9], va[9], dbar[6], sbar[6], mev[36], devp0[6];
+    sym3(c->velgradavg, vs);
+    skw3(c->velgradavg, va);
+    to6(vs, dbar);
+    to6(c->sgavg, sbar);
+    from4(c->de_ds, mev);
+    for (int a = 0; a < 6; a++) {
+        double s = 0.0;
+        for (int b = 0; b < 6; b++) s += mev[6 * a + b] * sbar[b];
+        devp0[a] = dbar[a] - s;
+    }
+    bool ok = state6x6(p->idsim, p->iscau, mev, devp0, p->scauchy, p->dsim);
+    double wasim[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int ii = 0; ii < 3; ii++)
+        for (int jj = 0; jj < 3; jj++) {
+            if (ii == jj) continue;
+            const int a = p->iudot[3 * ii + jj], b = p->iudot[3 * jj + ii];
+            if (a == 1 && b == 0) {
+                wasim[3 * ii + jj] = p->udot[3 * ii + jj] - p->dsim[3 * ii + jj];
+                wasim[3 * jj + ii] = -wasim[3 * ii + jj];
+            } else if (a == 1 && b == 1) {
+                wasim[3 * ii + jj] = 0.5 * (p->udot[3 * ii + jj] - p->udot[3 * jj + ii]);
+            }
+        }
+    for (int i = 0; i < 9; i++) c->dvelgradavg[i] = (p->dsim[i] - vs[i]) + (wasim[i] - va[i]);
+    // error normalisation (LS-EVPFFT.f90:341-343)
+    erre = erre / c->lvmavg;
+    errs = errs / c->svmavg;
+    errsbc = errsbc / c->svmavg;
+    if (out) {
+        out->errs = errs; out->erre = erre; out->errsbc = errsbc;
+        memcpy(out->velgradavg, c->velgradavg, sizeof out->velgradavg);
+        memcpy(out->sgavg, c->sgavg, sizeof out->sgavg);
+        memcpy(out->sgavgsol, c->sgavgsol, sizeof out->sgavgsol);
+        memcpy(out->dvelgradavg, c->dvelgradavg, sizeof out->dvelgradavg);
+        out->svmavg = c->svmavg;
+        out->dL_iter = dl;
+        out->nr_steps = (int64_t)r[R_NR];
+        out->evals = (int64_t)r[R_EVAL];
+        out->outer = (int64_t)r[R_OUTER];
+    }
+    if (!ok) return fail(c, 1, "Failed system solve in lu_eqsystem (state_6x6_evpsc)");
+    return 0;
+}
+
+int cg_rollback(cg_ctx* c, cg_process* p) {
+    if (!c || !c->initialised || !p) return -1;
+    memcpy(p->dsim, c->dsim_snap, sizeof c->dsim_snap);
+    memcpy(p->scauchy, c->scauchy_snap, sizeof c->scauchy_snap);
+    launch_snapshot(c->f, 1, c->s);
+    c->sgsum_valid = 0;
+    c->lold_same = 0;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int cg_kinematic_substep(cg_ctx* c, double tdot, double* out4, int* nsub) {
+    if (!c || !c->initialised) return -1;
+    int nb;
+    if (ensure_part(c, (size_t)((c->V + 255) / 256 + 64) * 81)) return -3;
+    launch_kinematic(c->f, tdot, c->s, &nb);
+    double r[4];
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->f.red, 4 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    memcpy(r, c->h_red, sizeof r);
+    if (out4) memcpy(out4, r, sizeof r);
+    double worst = std::max(1.0, std::max(r[0] / 5.0e-2, std::max(r[1] / 2.0, r[2] / 5.0e-2)));
+    if (nsub) *nsub = std::max(1, (int)ceil(worst));
+    return 0;
+}
+
+int cg_end_increment(cg_ctx* c, const cg_process* p, int scan_diagnostics, cg_incr_out* out) {
+    if (!c || !c->initialised || !p) return -1;
+    Fields& f = c->f;
+    const long V = c->V;
+    const double tdot = p->tdot;
+    PhaseTables pt{c->d_ph, c->d_tab};
+    tstart(c, 3);
+    if (c->lold_same) {  // velgradold == velgrad after the last pass
+        launch_copy(f.Lold, f.L, 9 * V, c->s);
+        c->lold_same = 0;
+    }
+    int nb;
+    double r[128];
+    // accumulation (LS-EVPFFT.f90:431-446)
+    {
+        launch_accumulate(f, tdot, c->s, &nb);
+        // max reduction through the partials: read all partials is avoided by a final op kernel
+        std::vector<double> hp(nb);
+        HIPCHK(c, hipMemcpyAsync(hp.data(), f.part, nb * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        double mx = c->edotvmmx;
+        for (int b = 0; b < nb; b++) mx = std::max(mx, hp[b]);
+        c->edotvmmx = mx;
+    }
+    if (p->iupdate == 1) {
+        launch_plastic_defgrad(f, pt, tdot, c->s);
+        launch_update_defgrad(f, tdot, c->s, &nb);
+        if (reduce_to_host(c, nb, 39, r)) return -2;
+        const double detfavg = r[38];
+        memcpy(c->defgradavg, r, 9 * sizeof(double));
+        c->wphcsol = r[36] / detfavg;
+        c->wphcgas = r[37] / detfavg;
+        const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        for (int i = 0; i < 9; i++) {
+            c->defgradinvavgc[i] = r[9 + i] / detfavg;
+            c->defgradinvavgcs[i] = r[18 + i] / detfavg / c->wphcsol;
+            c->defgradinvavgcg[i] = c->wphcgas > 0.0 ? r[27 + i] / detfavg / c->wphcgas : I3[i];
+        }
+        launch_div_w(f, detfavg, c->s);
+        launch_elastic_stiff(f, pt, c->s);
+        launch_tensor_config_a(f, c->s, &nb);
+        if (reduce_to_host(c, nb, 9, r)) return -2;
+        double inv[9], dm[9], dud[9];
+        inv3(c->defgradinvavgc, inv);
+        for (int i = 0; i < 9; i++) dm[i] = c->velgradavg[i] - r[i];
+        mm3(dm, inv, dud);
+        launch_tensor_config_b(f, dud, c->s);
+    }
+    if (p->iuphard == 1) launch_hardening(f, pt, tdot, c->s);
+    // macro averages (LS-EVPFFT.f90:460-523)
+    double vs[9];
+    sym3(c->velgradavg, vs);
+    c->lvmavg = vm_strain(c->velgradavg);
+    c->dvmavg = vm_strain(vs);
+    c->evmavg += c->dvmavg * tdot;
+    launch_averages(f, pt, c->s, &nb);
+    if (reduce_to_host(c, nb, 50, r)) return -2;
+    // eelavg = eavg + (disgradsym - ept) w of the last solid voxel in loop order (:482)
+    double last[13] = {0};
+    if (c->last_solid >= 0) {
+        for (int k = 0; k < 6; k++) {
+            HIPCHK(c, hipMemcpy(&last[k], f.dgs + k * V + c->last_solid, sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(c, hipMemcpy(&last[6 + k], f.ept + k * V + c->last_solid, sizeof(double), hipMemcpyDeviceToHost));
+        }
+        HIPCHK(c, hipMemcpy(&last[12], f.w + c->last_solid, sizeof(double), hipMemcpyDeviceToHost));
+    }
+    double dl6[6], dl3[9];
+    for (int k = 0; k < 6; k++) dl6[k] = last[k] - last[6 + k];
+    from6(dl6, dl3);
+    for (int i = 0; i < 9; i++) {
+        c->epavg[i] = r[i] / c->wphcsol;
+        c->eavg[i] = r[9 + i] / c->wphcsol;
+        c->eelavg[i] = c->last_solid >= 0 ? (r[9 + i] + dl3[i] * last[12]) / c->wphcsol : 0.0;
+        c->edotpavg[i] = r[27 + i] / c->wphcsol;
+        c->sgsolavg[i] = r[36 + i] / c->wphcsol;
+    }
+    const double gammaavg = r[46] > 0.0 ? r[45] / r[46] : 0.0;
+    const double gavg = c->nsys_solid > 0.0 ? r[47] / c->nsys_solid : 0.0;
+    const double geffavg = r[48] > 0.0 ? r[49] / r[48] : gavg;
+    c->evmpavg = vm_strain(c->epavg);
+    c->dvmpavg = vm_strain(c->edotpavg);
+    if (out) {
+        memset(out, 0, sizeof *out);
+        memcpy(out->eavg, c->eavg, 72); memcpy(out->sgsolavg, c->sgsolavg, 72); memcpy(out->epavg, c->epavg, 72);
+        memcpy(out->eelavg, c->eelavg, 72); memcpy(out->edotpavg, c->edotpavg, 72);
+        memcpy(out->velgradavg, c->velgradavg, 72); memcpy(out->sgavg, c->sgavg, 72);
+        out->evmavg = c->evmavg; out->evmpavg = c->evmpavg; out->dvmavg = c->dvmavg; out->dvmpavg = c->dvmpavg;
+        out->svmavg = c->svmavg; out->lvmavg = c->lvmavg; out->edotvmmx = c->edotvmmx;
+        out->gammaavg = gammaavg; out->gavg = gavg; out->geffavg = geffavg;
+    }
+    // update_diagnostics (types.f90:345-758)
+    if (out) {
+        const double Lref = std::max(c->lvmavg, 2.2250738585072014e-308);
+        const double lfloor = std::max(1.0e-12 * Lref, 2.2250738585072014e-308);
+        if (ensure_part(c, (size_t)((V + 255) / 256) * 24 + 64)) return -3;
+        launch_diagnostics(f, lfloor, scan_diagnostics, c->s, &nb);
+        HIPCHK(c, hipMemcpyAsync(c->h_red, f.red, 23 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        const double* d = c->h_red;
+        const double HUGEV = 1.7976931348623157e308;
+        out->dL_step_rms = d[1] > 0.0 ? sqrt(d[0] / d[1]) : 0.0;
+        out->dL_step_rms_sol = d[3] > 0.0 ? sqrt(d[2] / d[3]) : 0.0;
+        out->dL_step_rms_gas = d[5] > 0.0 ? sqrt(d[4] / d[5]) : 0.0;
+        out->dL_step_max = d[6]; out->dL_step_max_sol = d[7]; out->dL_step_max_gas = d[8];
+        out->volinc_max_sol = d[9]; out->volinc_max_gas = d[10];
+        out->stretchinc_max_sol = d[11]; out->stretchinc_max_gas = d[12];
+        out->rotang_max_sol = d[13]; out->rotang_max_gas = d[14];
+        out->cond_c066mod_max_sol = d[15]; out->cond_c066mod_max_gas = d[16]; out->cond_cgas_max = d[17];
+        out->detF_min = d[18] >= HUGEV * 0.5 ? 0.0 : d[18];
+        out->detF_min_sol = (d[20] < 0 || d[19] >= HUGEV * 0.5) ? 0.0 : d[19];
+        out->detF_min_gas = (d[22] < 0 || d[21] >= HUGEV * 0.5) ? 0.0 : d[21];
+        for (int k = 0; k < 2; k++) {
+            const long idx = (long)d[20 + 2 * k];
+            int* im = k == 0 ? out->imin_sol : out->imin_gas;
+            if (idx >= 0) {
+                im[0] = (int)(idx % c->n1) + 1;
+                im[1] = (int)((idx / c->n1) % c->n2) + 1;
+                im[2] = (int)(idx / ((long)c->n1 * c->n2)) + 1;
+            } else {
+                im[0] = im[1] = im[2] = 0;
+            }
+        }
+        double n2 = 0.0;
+        for (int i = 0; i < 9; i++) n2 += c->dvelgradavg[i] * c->dvelgradavg[i];
+        out->dvelgradavg_mag = sqrt(n2);
+    }
+    c->sgsum_valid = 0;  // weights changed
+    tstop(c, 3);
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    HIPCHK(c, hipGetLastError());
+    tcollect(c, 3);
+    return 0;
+}
+
+int cg_calc_c0(cg_ctx* c, const cg_process* p) {
+    if (!c || !c->initialised || !p) return -1;
+    PhaseTables pt{c->d_ph, c->d_tab};
+    int nb;
+    tstart(c, 3);
+    launch_calc_c0(c->f, pt, p->tdot, p->tdotref, c->s, &nb, c->int_pow);
+    double r[72];
+    if (reduce_to_host(c, nb, 72, r)) return -2;
+    tstop(c, 3);
+    tcollect(c, 3);
+    double dsde[36], dP[36], a66[36];
+    for (int i = 0; i < 36; i++) {
+        dsde[i] = r[i] / c->wphcsol;
+        dP[i] = r[36 + i] / c->wphsol;
+    }
+    bool ok = inv6(dsde);
+    to4(dsde, c->de_ds);
+    for (int i = 0; i < 36; i++) {
+        c->c066[i] = dP[i] * p->xc0;
+        a66[i] = dP[i] * p->xc0;
+    }
+    ok = inv6(a66) && ok;
+    to4(c->c066, c->c0);
+    to4(a66, c->s0);
+    if (!ok) return fail(c, 1, "Singular matrix in lu_inverse (calc_c0)");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// fields
+// ---------------------------------------------------------------------------
+namespace {
+struct FieldDesc {
+    double* p;
+    int stored;   // components stored on device
+    int ncomp;    // components exposed
+    int sym;      // 1: stored as b-basis 6-vector, exposed as 3x3
+};
+bool field_desc(cg_ctx* c, const char* name, FieldDesc& d) {
+    Fields& f = c->f;
+    const int ns = f.ns;
+    struct E { const char* n; double* p; int st; int nc; int sym; } T[] = {
+        {"sg", f.sg, 6, 9, 1}, {"sgt", f.sgt, 6, 9, 1}, {"velgrad", f.L, 9, 9, 0}, {"velgradold", f.Lold, 9, 9, 0},
+        {"dvelgradref", f.dLref, 9, 9, 0}, {"defgrad", f.F, 9, 9, 0}, {"defgradinv", f.Fi, 9, 9, 0},
+        {"detF", f.J, 1, 1, 0}, {"wgtc", f.w, 1, 1, 0}, {"defgrade", f.Fe, 9, 9, 0}, {"defgradp", f.Fp, 9, 9, 0},
+        {"ag", f.ag, 9, 9, 0}, {"sg66", f.sg66, 36, 36, 0}, {"c066mod", f.c66m, 36, 36, 0}, {"cgm", f.cgm, 36, 36, 0},
+        {"cgas", f.cgas, 36, 36, 0}, {"sch", f.sch, 5 * ns, 5 * ns, 0}, {"crss", f.crss, 2 * ns, 2 * ns, 0},
+        {"trialtau", f.ttau, 2 * ns, 2 * ns, 0}, {"gamdot", f.gdot, ns, ns, 0}, {"gacumgr", f.gacc, 1, 1, 0},
+        {"edotp", f.edotp, 6, 9, 1}, {"ept", f.ept, 6, 9, 1}, {"disgradsym", f.dgs, 6, 9, 1}, {"epvm", f.epvm, 1, 1, 0},
+        {"defgradinc", f.Finc, 9, 9, 0}, {"defgradeinc", f.Feinc, 9, 9, 0},
+    };
+    for (auto& e : T)
+        if (!strcmp(e.n, name)) {
+            d.p = e.p; d.stored = e.st; d.ncomp = e.nc; d.sym = e.sym;
+            return true;
+        }
+    return false;
+}
+}  // namespace
+
+int cg_field_ncomp(cg_ctx* c, const char* name) {
+    if (!c || !name) return -1;
+    FieldDesc d;
+    if (!field_desc(c, name, d)) return -1;
+    return d.ncomp;
+}
+
+int cg_get_field(cg_ctx* c, const char* name, double* out, size_t n) {
+    if (!c || !c->initialised || !name || !out) return -1;
+    FieldDesc d;
+    if (!field_desc(c, name, d) || !d.p) return fail(c, -1, "unknown or absent field %s", name);
+    const long V = c->V;
+    if (n < (size_t)d.ncomp * V) return fail(c, -1, "output buffer too small");
+    std::vector<double> h((size_t)d.stored * V);
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    HIPCHK(c, hipMemcpy(h.data(), d.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (long v = 0; v < V; v++) {
+        if (d.sym) {
+            double s6[6];
+            for (int k = 0; k < 6; k++) s6[k] = h[(size_t)k * V + v];
+            from6(s6, out + 9 * v);
+        } else {
+            for (int k = 0; k < d.stored; k++) out[(size_t)d.ncomp * v + k] = h[(size_t)k * V + v];
+        }
+    }
+    return 0;
+}
+
+int cg_set_field(cg_ctx* c, const char* name, const double* in, size_t n) {
+    if (!c || !c->initialised || !name || !in) return -1;
+    FieldDesc d;
+    if (!field_desc(c, name, d) || !d.p) return fail(c, -1, "unknown or absent field %s", name);
+    const long V = c->V;
+    if (n < (size_t)d.ncomp * V) return fail(c, -1, "input buffer too small");
+    std::vector<double> h((size_t)d.stored * V);
+    for (long v = 0; v < V; v++) {
+        if (d.sym) {
+            double s6[6];
+            to6(in + 9 * v, s6);
+            for (int k = 0; k < 6; k++) h[(size_t)k * V + v] = s6[k];
+        } else {
+            for (int k = 0; k < d.stored; k++) h[(size_t)k * V + v] = in[(size_t)d.ncomp * v + k];
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    HIPCHK(c, hipMemcpy(d.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->sgsum_valid = 0;
+    c->lold_same = 0;
+    return 0;
+}
+
+int cg_get_macro(cg_ctx* c, double* c0, double* s0, double* de_ds, double* c066, double* goperr0) {
+    if (!c) return -1;
+    if (c0) memcpy(c0, c->c0, sizeof c->c0);
+    if (s0) memcpy(s0, c->s0, sizeof c->s0);
+    if (de_ds) memcpy(de_ds, c->de_ds, sizeof c->de_ds);
+    if (c066) memcpy(c066, c->c066, sizeof c->c066);
+    if (goperr0) memcpy(goperr0, c->goperr0, sizeof c->goperr0);
+    return 0;
+}
+
+int cg_get_stats(cg_ctx* c, cg_stats* st, int reset) {
+    if (!c) return -1;
+    if (st) *st = c->st;
+    if (reset) memset(&c->st, 0, sizeof c->st);
+    return 0;
+}
+
+int cg_set_timing(cg_ctx* c, int on) {
+    if (!c) return -1;
+    c->timing = on ? 1 : 0;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,544 @@
+// fft_kernels.hip -- the FFT + Green-operator stage of one fixed-point pass.
+//
+// Reference: convolution_with_Goper (various_functions.f90:966-1059) with the
+// FFTW plans of fourier_functions.f90:141-146 (9 in-place 3-D c2c transforms,
+// forward e^{-i xi.x}, inverse unnormalised then Re()/V) and the Green operator
+// of form_G (various_functions.f90:609-809).
+//
+// MI355X form: real-to-complex along x (Gamma is real and even in xi, so the
+// Hermitian half spectrum carries the whole convolution), five LDS-staged
+// Stockham passes over the 9 components:
+//   x-fwd (fused PK1 = J (sigma - <sigma>) F^-T)  ->  y-fwd  ->
+//   z-fwd + Gamma(xi) computed on the fly from C0 + z-inv  ->  y-inv  ->
+//   x-inv (c2r, /V) writing dLref and the block partial sums of w dLref F^-1.
+// The half spectrum is component-interleaved: spec[((k*n2 + j)*h1 + kx)*9 + c].
+// Template parameter R selects the arithmetic of the spectral stage (double:
+// FP64 form; float: mixed form with the input scaled by a stress norm).
+#include <vector>
+
+#include "internal.h"
+#include "reduce.h"
+#include "tensor.h"
+
+namespace cg {
+
+constexpr int FBS = 256;
+
+template <typename R> struct C2;
+template <> struct C2<double> { using T = double2; };
+template <> struct C2<float> { using T = float2; };
+
+template <typename T2> __device__ __forceinline__ T2 cmul(T2 a, T2 b) {
+    T2 r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+template <typename T2> __device__ __forceinline__ T2 cadd(T2 a, T2 b) { T2 r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename T2> __device__ __forceinline__ T2 csub(T2 a, T2 b) { T2 r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+
+struct Axis {
+    int n, np;
+    int rad[12];
+    int tw_off;  // offset of this axis' twiddle table (n entries) in the table buffer
+};
+struct Plan3 {
+    Axis ax[3];
+    const double2* tw64;
+    const float2* tw32;
+};
+
+template <typename R> __device__ __forceinline__ typename C2<R>::T twid(const Plan3& p, int off, int idx);
+template <> __device__ __forceinline__ double2 twid<double>(const Plan3& p, int off, int idx) { return p.tw64[off + idx]; }
+template <> __device__ __forceinline__ float2 twid<float>(const Plan3& p, int off, int idx) { return p.tw32[off + idx]; }
+
+// radix-R butterfly (forward: e^{-2 pi i rk/R}; inverse: conjugate)
+template <typename T2, typename R>
+__device__ __forceinline__ void butterfly(T2* x, int Rr, bool inv) {
+    if (Rr == 2) {
+        T2 a = x[0], b = x[1];
+        x[0] = cadd(a, b);
+        x[1] = csub(a, b);
+    } else if (Rr == 4) {
+        T2 a = cadd(x[0], x[2]), b = csub(x[0], x[2]), c = cadd(x[1], x[3]), d = csub(x[1], x[3]);
+        // -i d (forward) / +i d (inverse)
+        T2 md;
+        if (!inv) { md.x = d.y; md.y = -d.x; } else { md.x = -d.y; md.y = d.x; }
+        x[0] = cadd(a, c);
+        x[2] = csub(a, c);
+        x[1] = cadd(b, md);
+        x[3] = csub(b, md);
+    } else {  // 3
+        const R h = (R)0.86602540378443864676;  // sqrt(3)/2
+        T2 t1 = cadd(x[1], x[2]);
+        T2 t2;
+        t2.x = x[0].x - (R)0.5 * t1.x;
+        t2.y = x[0].y - (R)0.5 * t1.y;
+        T2 d = csub(x[1], x[2]);
+        T2 s;  // -i h d (forward)
+        if (!inv) { s.x = h * d.y; s.y = -h * d.x; } else { s.x = -h * d.y; s.y = h * d.x; }
+        x[0] = cadd(x[0], t1);
+        x[1] = cadd(t2, s);
+        x[2] = csub(t2, s);
+    }
+}
+
+// Stockham passes over nsig signals of length n in LDS (signal stride ss).
+// Returns the buffer that holds the result.
+template <typename R>
+__device__ typename C2<R>::T* lds_fft(typename C2<R>::T* src, typename C2<R>::T* dst, int nsig, int ss,
+                                      const Plan3& p, int axis, bool inv) {
+    using T2 = typename C2<R>::T;
+    const Axis& A = p.ax[axis];
+    const int n = A.n;
+    int Ns = 1;
+    for (int ps = 0; ps < A.np; ps++) {
+        const int Rr = A.rad[ps];
+        const int m = n / Rr;
+        const int stepN = n / (Ns * Rr);
+        const int items = nsig * m;
+        for (int it = threadIdx.x; it < items; it += blockDim.x) {
+            const int sig = it / m, j = it - sig * m;
+            const int k = j % Ns;
+            T2 x[4];
+            const T2* s = src + sig * ss;
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if (r < Rr) x[r] = s[j + r * m];
+#pragma unroll
+            for (int r = 1; r < 4; r++)
+                if (r < Rr) {
+                    T2 w = twid<R>(p, A.tw_off, (k * r * stepN) % n);
+                    if (inv) w.y = -w.y;
+                    x[r] = cmul(x[r], w);
+                }
+            butterfly<T2, R>(x, Rr, inv);
+            const int d = (j / Ns) * Ns * Rr + k;
+            T2* o = dst + sig * ss;
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if (r < Rr) o[d + r * Ns] = x[r];
+        }
+        __syncthreads();
+        T2* t = src; src = dst; dst = t;
+        Ns *= Rr;
+    }
+    return src;
+}
+
+// ---------------------------------------------------------------------------
+// x forward with fused PK1 (LS-EVPFFT.f90:256-269; CauchyToPK1 :954-963)
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(FBS) void k_x_fwd_pk1(Fields f, Plan3 p, const double* __restrict__ sgsum, int rows,
+                                                   double inv_scale) {
+    using T2 = typename C2<R>::T;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int n1 = f.n1, h1 = f.h1;
+    const int ss = n1 + 1;
+    const int nsig = rows * 9;
+    T2* b0 = (T2*)smem;
+    T2* b1 = b0 + nsig * ss;
+    const long V = f.V;
+    double sa[9];
+    for (int c = 0; c < 9; c++) sa[c] = sgsum[c];
+    const long row0 = (long)blockIdx.x * rows;
+    for (int e = threadIdx.x; e < rows * n1; e += blockDim.x) {
+        const int r = e / n1, i = e - r * n1;
+        const long v = (row0 + r) * n1 + i;
+        double s6[6], s[9], Fi[9], pk[9];
+        for (int c = 0; c < 6; c++) s6[c] = f.sg[c * V + v];
+        for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * V + v];
+        from6(s6, s);
+        const double J = f.J[v];
+        for (int c = 0; c < 9; c++) s[c] -= sa[c];
+        mmt3(s, Fi, pk);  // (sigma - <sigma>) F^-T
+        for (int c = 0; c < 9; c++) {
+            T2 z;
+            z.x = (R)(pk[c] * J * inv_scale);
+            z.y = (R)0;
+            b0[(r * 9 + c) * ss + i] = z;
+        }
+    }
+    __syncthreads();
+    T2* res = lds_fft<R>(b0, b1, nsig, ss, p, 0, false);
+    T2* spec = (T2*)(sizeof(R) == 8 ? (void*)f.spec : (void*)f.specf);
+    for (int e = threadIdx.x; e < rows * h1 * 9; e += blockDim.x) {
+        const int r = e / (h1 * 9), q = e - r * h1 * 9;
+        const int kx = q / 9, c = q - kx * 9;
+        spec[((row0 + r) * h1 + kx) * 9 + c] = res[(r * 9 + c) * ss + kx];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// y pass (forward or inverse) over tiles of T kx columns of one z plane
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(FBS) void k_y(Fields f, Plan3 p, int T, int inv) {
+    using T2 = typename C2<R>::T;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int n2 = f.n2, h1 = f.h1;
+    const int ntile = (h1 + T - 1) / T;
+    const int k = blockIdx.x / ntile, tile = blockIdx.x - k * ntile;
+    const int kx0 = tile * T;
+    const int Te = min(T, h1 - kx0);
+    const int nsig = Te * 9, ss = n2 + 1;
+    T2* b0 = (T2*)smem;
+    T2* b1 = b0 + T * 9 * ss;
+    T2* spec = (T2*)(sizeof(R) == 8 ? (void*)f.spec : (void*)f.specf);
+    const int w = Te * 9;
+    for (int e = threadIdx.x; e < n2 * w; e += blockDim.x) {
+        const int j = e / w, q = e - j * w;
+        b0[q * ss + j] = spec[(((long)k * n2 + j) * h1 + kx0) * 9 + q];
+    }
+    __syncthreads();
+    T2* res = lds_fft<R>(b0, b1, nsig, ss, p, 1, inv != 0);
+    for (int e = threadIdx.x; e < n2 * w; e += blockDim.x) {
+        const int j = e / w, q = e - j * w;
+        spec[(((long)k * n2 + j) * h1 + kx0) * 9 + q] = res[q * ss + j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// z pass with the Green operator: forward z, Gamma(xi) : P^, inverse z
+// ---------------------------------------------------------------------------
+struct GreenDev {
+    double c0[81];
+    double s0[81];
+    int igamma;
+};
+
+// full 81-component Gamma at 0-based frequency (kx, ky, kz) for igamma = 1, 2
+__device__ void green81(const GreenDev& g, int kx, int ky, int kz, int n1, int n2, int n3, double* G) {
+    const double tp = 2.0 * PI_D;
+    double xi[3];
+    xi[0] = tp * (double)((kx + 1) <= n1 / 2 ? kx : kx - n1) / n1;
+    xi[1] = tp * (double)((ky + 1) <= n2 / 2 ? ky : ky - n2) / n2;
+    xi[2] = (n3 == 1) ? 0.0 : tp * (double)((kz + 1) <= n3 / 2 ? kz : kz - n3) / n3;
+    const bool nx = (n1 % 2 == 0) && kx == n1 / 2, ny = (n2 % 2 == 0) && ky == n2 / 2,
+               nz = (n3 % 2 == 0) && kz == n3 / 2;
+    if (g.igamma >= 2 && ((nx && ny) || (nx && nz) || (ny && nz))) {
+        for (int i = 0; i < 81; i++) G[i] = 0.0;
+        return;
+    }
+    double D[9];
+    if (g.igamma == 1) {
+        const double xs = tp * kx / n1, ys = tp * ky / n2, zs = tp * kz / n3;
+        D[0] = 2.0 * (cos(xs) - 1.0); D[4] = 2.0 * (cos(ys) - 1.0); D[8] = 2.0 * (cos(zs) - 1.0);
+        D[1] = D[3] = -sin(xs) * sin(ys);
+        D[2] = D[6] = -sin(xs) * sin(zs);
+        D[5] = D[7] = -sin(ys) * sin(zs);
+    } else {
+        double pr = 1.0, pim = 0.0;
+        for (int l = 0; l < 3; l++) {
+            const double cr = 1.0 + cos(xi[l]), ci = sin(xi[l]);
+            const double nr = pr * cr - pim * ci, ni = pr * ci + pim * cr;
+            pr = nr; pim = ni;
+        }
+        const double f2 = 0.0625 * (pr * pr + pim * pim);
+        double tn[3];
+        for (int l = 0; l < 3; l++) tn[l] = tan(xi[l] * 0.5);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) D[3 * i + j] = tn[i] * tn[j] * f2;
+    }
+    double A[9], Ai[9];
+    if (kx == 0 && ky == 0 && kz == 0) {
+        for (int i = 0; i < 9; i++) Ai[i] = 0.0;
+    } else {
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) {
+                double s = 0.0;
+                for (int j = 0; j < 3; j++)
+                    for (int l = 0; l < 3; l++) s += g.c0[27 * i + 9 * j + 3 * k + l] * D[3 * j + l];
+                A[3 * i + k] = s;
+            }
+        inv3(A, Ai);
+    }
+    for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++)
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) G[27 * p + 9 * q + 3 * i + j] = -Ai[3 * p + i] * D[3 * q + j];
+    if (g.igamma == 1) {
+        double gt[81], aux[81];
+        for (int i = 0; i < 81; i++) gt[i] = G[i];
+        for (int it = 0; it < 4; it++) {
+            for (int ij = 0; ij < 9; ij++) {
+                double cgv[9];
+                for (int op = 0; op < 9; op++) {
+                    double s = 0.0;
+                    for (int mn = 0; mn < 9; mn++) s += G[9 * ij + mn] * g.c0[9 * mn + op];
+                    cgv[op] = s;
+                }
+                for (int kl = 0; kl < 9; kl++) {
+                    double s = 0.0;
+                    for (int op = 0; op < 9; op++) s += cgv[op] * gt[9 * op + kl];
+                    aux[9 * ij + kl] = -s;
+                }
+            }
+            for (int i = 0; i < 81; i++) gt[i] = aux[i];
+        }
+        for (int i = 0; i < 81; i++) G[i] = gt[i];
+    }
+}
+
+// out_ij = sum_kl Gamma_ijkl(xi) P_kl at one frequency (9 complex in place, stride st)
+template <typename R>
+__device__ __forceinline__ void green_apply(const GreenDev& g, int kx, int ky, int kz, int n1, int n2, int n3,
+                                            typename C2<R>::T* P, int st) {
+    using T2 = typename C2<R>::T;
+    if (kx == 0 && ky == 0 && kz == 0) {
+        for (int c = 0; c < 9; c++) { P[c * st].x = (R)0; P[c * st].y = (R)0; }
+        return;
+    }
+    const bool nx = (n1 % 2 == 0) && kx == n1 / 2, ny = (n2 % 2 == 0) && ky == n2 / 2,
+               nz = (n3 % 2 == 0) && kz == n3 / 2 && n3 > 1;
+    T2 in[9];
+    for (int c = 0; c < 9; c++) in[c] = P[c * st];
+    if (g.igamma == 0) {
+        if (nx || ny || nz) {  // Gamma = -S0 on any Nyquist plane (form_G :646-651)
+            for (int ij = 0; ij < 9; ij++) {
+                double sr = 0.0, si = 0.0;
+                for (int kl = 0; kl < 9; kl++) {
+                    sr -= g.s0[9 * ij + kl] * (double)in[kl].x;
+                    si -= g.s0[9 * ij + kl] * (double)in[kl].y;
+                }
+                P[ij * st].x = (R)sr;
+                P[ij * st].y = (R)si;
+            }
+            return;
+        }
+        const double tp = 2.0 * PI_D;
+        double xi[3];
+        xi[0] = tp * (double)((kx + 1) <= n1 / 2 ? kx : kx - n1) / n1;
+        xi[1] = tp * (double)((ky + 1) <= n2 / 2 ? ky : ky - n2) / n2;
+        xi[2] = (n3 == 1) ? 0.0 : tp * (double)((kz + 1) <= n3 / 2 ? kz : kz - n3) / n3;
+        // A = C0 : (xi x xi), out = -(A^-1 (P xi)) x xi
+        double A[9], Ai[9];
+        const double d00 = xi[0] * xi[0], d11 = xi[1] * xi[1], d22 = xi[2] * xi[2], d01 = xi[0] * xi[1],
+                     d02 = xi[0] * xi[2], d12 = xi[1] * xi[2];
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) {
+                const double* c = g.c0 + 27 * i + 3 * k;  // c[9*j + l] = c0_{i j k l}
+                A[3 * i + k] = c[0] * d00 + c[10] * d11 + c[20] * d22 + (c[1] + c[9]) * d01 + (c[2] + c[18]) * d02 +
+                               (c[11] + c[19]) * d12;
+            }
+        inv3(A, Ai);
+        double qr[3], qi[3];
+        for (int k = 0; k < 3; k++) {
+            qr[k] = (double)in[3 * k].x * xi[0] + (double)in[3 * k + 1].x * xi[1] + (double)in[3 * k + 2].x * xi[2];
+            qi[k] = (double)in[3 * k].y * xi[0] + (double)in[3 * k + 1].y * xi[1] + (double)in[3 * k + 2].y * xi[2];
+        }
+        for (int i = 0; i < 3; i++) {
+            const double ur = -(Ai[3 * i] * qr[0] + Ai[3 * i + 1] * qr[1] + Ai[3 * i + 2] * qr[2]);
+            const double ui = -(Ai[3 * i] * qi[0] + Ai[3 * i + 1] * qi[1] + Ai[3 * i + 2] * qi[2]);
+            for (int j = 0; j < 3; j++) {
+                P[(3 * i + j) * st].x = (R)(ur * xi[j]);
+                P[(3 * i + j) * st].y = (R)(ui * xi[j]);
+            }
+        }
+        return;
+    }
+    // The reference keeps Re() of a full c2c inverse, i.e. it applies the even part
+    // (Gamma(xi) + Gamma(-xi))/2 to the Hermitian spectrum; igamma = 2 is not even on
+    // the Nyquist planes (xi = -pi is not mirrored), so both halves are evaluated.
+    double G[81], Gm[81];
+    green81(g, kx, ky, kz, n1, n2, n3, G);
+    green81(g, (n1 - kx) % n1, (n2 - ky) % n2, (n3 - kz) % n3, n1, n2, n3, Gm);
+    for (int i = 0; i < 81; i++) G[i] = 0.5 * (G[i] + Gm[i]);
+    for (int ij = 0; ij < 9; ij++) {
+        double sr = 0.0, si = 0.0;
+        for (int kl = 0; kl < 9; kl++) {
+            sr += G[9 * ij + kl] * (double)in[kl].x;
+            si += G[9 * ij + kl] * (double)in[kl].y;
+        }
+        P[ij * st].x = (R)sr;
+        P[ij * st].y = (R)si;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(FBS) void k_z_green(Fields f, Plan3 p, GreenDev g, int T) {
+    using T2 = typename C2<R>::T;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int n2 = f.n2, n3 = f.n3, h1 = f.h1;
+    const int ntile = (h1 + T - 1) / T;
+    const int j = blockIdx.x / ntile, tile = blockIdx.x - j * ntile;
+    const int kx0 = tile * T;
+    const int Te = min(T, h1 - kx0);
+    const int nsig = Te * 9, ss = n3 + 1;
+    T2* b0 = (T2*)smem;
+    T2* b1 = b0 + T * 9 * ss;
+    T2* spec = (T2*)(sizeof(R) == 8 ? (void*)f.spec : (void*)f.specf);
+    const int w = Te * 9;
+    for (int e = threadIdx.x; e < n3 * w; e += blockDim.x) {
+        const int k = e / w, q = e - k * w;
+        b0[q * ss + k] = spec[(((long)k * n2 + j) * h1 + kx0) * 9 + q];
+    }
+    __syncthreads();
+    T2* res = lds_fft<R>(b0, b1, nsig, ss, p, 2, false);
+    for (int e = threadIdx.x; e < Te * n3; e += blockDim.x) {
+        const int t = e / n3, kz = e - t * n3;
+        green_apply<R>(g, kx0 + t, j, kz, f.n1, n2, n3, res + (t * 9) * ss + kz, ss);
+    }
+    __syncthreads();
+    T2* other = (res == b0) ? b1 : b0;
+    res = lds_fft<R>(res, other, nsig, ss, p, 2, true);
+    for (int e = threadIdx.x; e < n3 * w; e += blockDim.x) {
+        const int k = e / w, q = e - k * w;
+        spec[(((long)k * n2 + j) * h1 + kx0) * 9 + q] = res[q * ss + k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x inverse (c2r): dLref = Re(IFFT)/V, block partial sums of w dLref F^-1
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(FBS) void k_x_inv(Fields f, Plan3 p, int rows, double out_scale) {
+    using T2 = typename C2<R>::T;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int n1 = f.n1, h1 = f.h1;
+    const int ss = n1 + 1;
+    const int nsig = rows * 9;
+    T2* b0 = (T2*)smem;
+    T2* b1 = b0 + nsig * ss;
+    const long V = f.V;
+    const long row0 = (long)blockIdx.x * rows;
+    T2* spec = (T2*)(sizeof(R) == 8 ? (void*)f.spec : (void*)f.specf);
+    for (int e = threadIdx.x; e < rows * h1 * 9; e += blockDim.x) {
+        const int r = e / (h1 * 9), q = e - r * h1 * 9;
+        const int kx = q / 9, c = q - kx * 9;
+        T2 z = spec[((row0 + r) * h1 + kx) * 9 + c];
+        T2* s = b0 + (r * 9 + c) * ss;
+        s[kx] = z;
+        if (kx > 0 && kx < n1 - kx) {  // Hermitian mirror X[n-k] = conj X[k]
+            T2 cz;
+            cz.x = z.x;
+            cz.y = -z.y;
+            s[n1 - kx] = cz;
+        }
+    }
+    __syncthreads();
+    T2* res = lds_fft<R>(b0, b1, nsig, ss, p, 0, true);
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int e = threadIdx.x; e < rows * n1; e += blockDim.x) {
+        const int r = e / n1, i = e - r * n1;
+        const long v = (row0 + r) * n1 + i;
+        double d[9], Fi[9], q[9];
+        for (int c = 0; c < 9; c++) {
+            d[c] = (double)res[(r * 9 + c) * ss + i].x * out_scale;
+            f.dLref[c * V + v] = d[c];
+            Fi[c] = f.Fi[c * V + v];
+        }
+        mm3(d, Fi, q);
+        const double w = f.w[v];
+        for (int c = 0; c < 9; c++) acc[c] += q[c] * w;
+    }
+    block_reduce_store<9, FBS>(acc, f.part);
+}
+
+// ---------------------------------------------------------------------------
+// host side: plans, twiddles, launches
+// ---------------------------------------------------------------------------
+static bool factor(int n, Axis& a) {
+    a.n = n;
+    a.np = 0;
+    int m = n;
+    while (m % 4 == 0 && m > 1) { a.rad[a.np++] = 4; m /= 4; }
+    while (m % 3 == 0) { a.rad[a.np++] = 3; m /= 3; }
+    while (m % 2 == 0) { a.rad[a.np++] = 2; m /= 2; }
+    return m == 1 && a.np <= 12;
+}
+
+struct FftCtx {
+    Plan3 plan;
+    double2* tw64;
+    float2* tw32;
+    int rows, ty, tz;
+};
+
+FftCtx* fft_create(const Fields& f, char* err, int errlen) {
+    FftCtx* c = new FftCtx();
+    int ns[3] = {f.n1, f.n2, f.n3};
+    int off = 0;
+    for (int a = 0; a < 3; a++) {
+        if (!factor(ns[a], c->plan.ax[a])) {
+            snprintf(err, errlen, "grid size %d is not of the form 2^a 3^b", ns[a]);
+            delete c;
+            return nullptr;
+        }
+        c->plan.ax[a].tw_off = off;
+        off += ns[a];
+    }
+    std::vector<double2> h64(off);
+    std::vector<float2> h32(off);
+    for (int a = 0; a < 3; a++) {
+        const int n = ns[a];
+        for (int k = 0; k < n; k++) {
+            long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
+            double2 z;
+            z.x = (double)cosl(ang);
+            z.y = (double)sinl(ang);
+            h64[c->plan.ax[a].tw_off + k] = z;
+            float2 zf;
+            zf.x = (float)z.x;
+            zf.y = (float)z.y;
+            h32[c->plan.ax[a].tw_off + k] = zf;
+        }
+    }
+    hipMalloc(&c->tw64, sizeof(double2) * off);
+    hipMalloc(&c->tw32, sizeof(float2) * off);
+    hipMemcpy(c->tw64, h64.data(), sizeof(double2) * off, hipMemcpyHostToDevice);
+    hipMemcpy(c->tw32, h32.data(), sizeof(float2) * off, hipMemcpyHostToDevice);
+    c->plan.tw64 = c->tw64;
+    c->plan.tw32 = c->tw32;
+    // rows per x block and kx tile widths (two complex buffers must fit the LDS)
+    c->rows = 2;
+    while ((long)(f.n2) * f.n3 % c->rows) c->rows /= 2;
+    c->ty = 2;
+    c->tz = 2;
+    return c;
+}
+void fft_destroy(FftCtx* c) {
+    if (!c) return;
+    hipFree(c->tw64);
+    hipFree(c->tw32);
+    delete c;
+}
+int fft_rows(const FftCtx* c) { return c->rows; }
+
+template <typename R>
+static int run_green(const Fields& f, FftCtx* c, const double* sgsum, const GreenDev& g, double scale,
+                     hipStream_t s) {
+    using T2 = typename C2<R>::T;
+    const int rows = c->rows;
+    const int nrow = f.n2 * f.n3 / rows;
+    const size_t shx = 2 * (size_t)rows * 9 * (f.n1 + 1) * sizeof(T2);
+    const size_t shy = 2 * (size_t)c->ty * 9 * (f.n2 + 1) * sizeof(T2);
+    const size_t shz = 2 * (size_t)c->tz * 9 * (f.n3 + 1) * sizeof(T2);
+    const int ty_blocks = f.n3 * ((f.h1 + c->ty - 1) / c->ty);
+    const int tz_blocks = f.n2 * ((f.h1 + c->tz - 1) / c->tz);
+    hipLaunchKernelGGL(k_x_fwd_pk1<R>, dim3(nrow), dim3(FBS), shx, s, f, c->plan, sgsum, rows, 1.0 / scale);
+    if (f.n2 > 1) hipLaunchKernelGGL(k_y<R>, dim3(ty_blocks), dim3(FBS), shy, s, f, c->plan, c->ty, 0);
+    hipLaunchKernelGGL(k_z_green<R>, dim3(tz_blocks), dim3(FBS), shz, s, f, c->plan, g, c->tz);
+    if (f.n2 > 1) hipLaunchKernelGGL(k_y<R>, dim3(ty_blocks), dim3(FBS), shy, s, f, c->plan, c->ty, 1);
+    hipLaunchKernelGGL(k_x_inv<R>, dim3(nrow), dim3(FBS), shx, s, f, c->plan, rows, scale / (double)f.V);
+    return nrow;
+}
+
+int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
+              int mixed, double scale, hipStream_t s) {
+    GreenDev g;
+    for (int i = 0; i < 81; i++) { g.c0[i] = c0[i]; g.s0[i] = s0[i]; }
+    g.igamma = igamma;
+    if (mixed) return run_green<float>(f, c, sgsum_dev, g, scale, s);
+    return run_green<double>(f, c, sgsum_dev, g, 1.0, s);
+}
+
+}  // namespace cg
+
+namespace cg {
+int fft_integrate(const Fields& f, FftCtx* c, int ideriv, double* dv, hipStream_t s) {
+    (void)f; (void)c; (void)ideriv; (void)dv; (void)s;
+    return -1;
+}
+}  // namespace cg

@@ -1,0 +1,110 @@
+// internal.h -- device state of one RVE context and the kernel launchers
+// shared between the translation units of libcreep_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cg {
+
+constexpr int MAXPH = 4;
+constexpr int MAXMODES = 8;
+
+// Per-phase constants, resident in device memory (read-only in kernels).
+struct PhaseDev {
+    int igas, nsyst, nmodes;
+    int nsm[MAXMODES];
+    double c66cr[36];     // crystal stiffness in the b-basis (crystal axes)
+    double mode_tau1[MAXMODES], mode_thet0[MAXMODES], mode_thet1[MAXMODES];
+    // offsets (in doubles) into PhaseTables::data
+    int off_dnca, off_dbca, off_nrs, off_gamd0, off_hard, off_tau, off_hpfac;
+};
+
+struct PhaseTables {
+    const PhaseDev* ph;   // device array [MAXPH]
+    const double* data;   // device: concatenated per-system tables
+};
+
+// Field arrays (SoA: component-major, a[c*V + v]); symmetric tensors as
+// b-basis 6-vectors, general 3x3 tensors as 9 row-major components.
+struct Fields {
+    int n1, n2, n3, h1;   // h1 = n1/2 + 1 (half spectrum along x)
+    long V;
+    int ns;               // slip systems per voxel (max over phases)
+    uint8_t* phase;       // 0-based phase id
+    uint8_t* gas;
+    double *w;            // wgtc
+    double *sg, *sgt;     // 6
+    double *L, *Lold, *dLref;  // 9
+    double *F, *Fi, *J, *Fe, *Fp, *ag;  // 9,9,1,9,9,9
+    double *sg66, *c66m, *cgm, *cgas;   // 36 each (cgm only if modal, cgas only if gas voxels)
+    double *sch, *crss, *ttau, *gdot, *gacc;  // 5ns, 2ns, 2ns, ns, 1
+    double *edotp, *ept, *dgs, *epvm;   // 6,6,6,1
+    double *Finc, *Feinc;               // 9,9
+    double *s_L, *s_Lold, *s_sg, *s_edotp, *s_gdot, *s_ttau;  // snapshots
+    double2* spec;        // FP64 half spectrum, [point][9] (point = (k*n2+j)*h1+kx)
+    float2* specf;        // mixed form: FP32 half spectrum
+    double* xnode;        // 3*(n1+1)*(n2+1)*n3node
+    double* part;         // block partial sums
+    double* red;          // final reductions (device)
+};
+
+// Constants of one fixed-point pass.
+struct PassConst {
+    double tdot, complgas, error, erroral, mix, lscale;
+    int modal, do_mix, kin_ctrl, lold_same, iuphard;
+};
+
+// reduction slots written by the fused correction + constitutive kernel
+enum {
+    R_ERRS = 0, R_ERRE = 1,
+    R_SGSOL = 2,     // 9: sum_solid w sigma
+    R_SGALL = 11,    // 9: sum_all w sigma (next pass' <sigma>)
+    R_LALL = 20,     // 9: sum w L (before mixing)
+    R_LGAS = 29,     // 9
+    R_LSOL = 38,     // 9
+    R_DLSOL = 47, R_WSOL = 48, R_DLGAS = 49, R_WGAS = 50,
+    R_NR = 51, R_EVAL = 52, R_OUTER = 53,
+    R_NSOLVE = 54
+};
+
+// ---- solver_kernels.hip ----
+void launch_reduce_final(const double* part, int nblocks, int nvals, double* out, hipStream_t s);
+void launch_sigma_sum(const Fields& f, hipStream_t s, int* nblocks_out);
+void launch_correct_solve_full(const Fields& f, const PhaseTables& pt, const PassConst& pc,
+                               const double* dvelgradavg, const double* finv_inv, const double* fluct_dev,
+                               hipStream_t s, int* nb, int int_pow);
+void launch_update_schmid(const Fields& f, const PhaseTables& pt, hipStream_t s);
+void launch_c066mod(const Fields& f, const double* c066, const double* goperr0, int modal, hipStream_t s);
+void launch_gas_compl(const Fields& f, double tdot, double complgas, int modal, hipStream_t s);
+void launch_goperr0(const Fields& f, const double* c0, const double* s0, int igamma, hipStream_t s, int* nb);
+void launch_accumulate(const Fields& f, double tdot, hipStream_t s, int* nb);
+void launch_plastic_defgrad(const Fields& f, const PhaseTables& pt, double tdot, hipStream_t s);
+void launch_update_defgrad(const Fields& f, double tdot, hipStream_t s, int* nb);
+void launch_div_w(const Fields& f, double detfavg, hipStream_t s);
+void launch_elastic_stiff(const Fields& f, const PhaseTables& pt, hipStream_t s);
+void launch_tensor_config_a(const Fields& f, hipStream_t s, int* nb);
+void launch_tensor_config_b(const Fields& f, const double* dud, hipStream_t s);
+void launch_hardening(const Fields& f, const PhaseTables& pt, double tdot, hipStream_t s);
+void launch_averages(const Fields& f, const PhaseTables& pt, hipStream_t s, int* nb);
+void launch_diagnostics(const Fields& f, double lfloor, int scan_cond, hipStream_t s, int* nb);
+void launch_calc_c0(const Fields& f, const PhaseTables& pt, double tdot, double tdotref, hipStream_t s, int* nb,
+                    int int_pow);
+void launch_kinematic(const Fields& f, double tdot, hipStream_t s, int* nb);
+void launch_snapshot(const Fields& f, int restore, hipStream_t s);
+void launch_init_voxels(const Fields& f, const PhaseTables& pt, const double* euler, const int* grain,
+                        const double* hp_grain, const double* udot, double tdot, hipStream_t s, int* nb);
+void launch_copy(double* dst, const double* src, size_t n, hipStream_t s);
+
+// ---- fft_kernels.hip ----
+struct FftCtx;
+FftCtx* fft_create(const Fields& f, char* err, int errlen);
+void fft_destroy(FftCtx* c);
+// PK1 -> Gamma convolution -> dLref; writes the 9 block partial sums of
+// w dLref F^-1 to f.part and returns the number of blocks.
+int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
+              int mixed, double scale, hipStream_t s);
+// spectral integration of L F for the nodal update: dv (3 x V) and 9 block partial
+// sums of L F / V into f.part; returns the number of blocks.
+int fft_integrate(const Fields& f, FftCtx* c, int ideriv, double* dv, hipStream_t s);
+
+}  // namespace cg

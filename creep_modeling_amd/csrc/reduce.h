@@ -1,0 +1,35 @@
+// reduce.h -- deterministic block reductions (64-wide wavefront shuffles,
+// then a fixed-order pass over the block's waves through LDS).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace cg {
+
+struct OpSum { __device__ static double init() { return 0.0; } __device__ static double f(double a, double b) { return a + b; } };
+struct OpMax { __device__ static double init() { return -1.7976931348623157e308; } __device__ static double f(double a, double b) { return a > b ? a : b; } };
+struct OpMin { __device__ static double init() { return 1.7976931348623157e308; } __device__ static double f(double a, double b) { return a < b ? a : b; } };
+
+__device__ __forceinline__ double wave_shfl_down(double v, int o) { return __shfl_down(v, o, 64); }
+
+// Reduce acc[0..NV) over the block; thread 0..NV-1 write part[blockIdx.x*NV + i].
+template <int NV, int BS, class Op = OpSum>
+__device__ __forceinline__ void block_reduce_store(const double* acc, double* part) {
+    static_assert(NV <= BS, "NV must not exceed the block size");
+    __shared__ double sm[(BS / 64) * NV];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        double v = acc[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = Op::f(v, wave_shfl_down(v, o));
+        if (lane == 0) sm[wid * NV + i] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = sm[threadIdx.x];
+        for (int w = 1; w < BS / 64; w++) s = Op::f(s, sm[w * NV + threadIdx.x]);
+        part[(size_t)blockIdx.x * NV + threadIdx.x] = s;
+    }
+}
+
+}  // namespace cg

@@ -1,0 +1,1238 @@
+// solver_kernels.hip -- per-voxel kernels of the fixed-point loop and of the
+// accepted-increment updates (one thread per voxel, FP64).
+//
+// Reference operators: various_functions.f90 (solve_res :1142-1364,
+// plastic_strain_rate :1367-1419, harden :1439-1491, correction_to_current
+// :1062-1139, update_* :1748-2492, calc_c0 :2537-2649, kinematic_increments
+// :2766-2837), LS-EVPFFT.f90 (mixing :277-327, accumulation :431-446,
+// averages :460-523) and types.f90 (update_diagnostics :345-758).
+#include "internal.h"
+#include "reduce.h"
+#include "tensor.h"
+
+namespace cg {
+
+constexpr int BS = 256;
+
+static inline int nblk(long V) { return (int)((V + BS - 1) / BS); }
+
+// |x|^(n-1) for an integer n (exponentiation by squaring) or through pow().
+template <bool INTPOW>
+__device__ __forceinline__ double powm1(double x, double n) {
+    if (INTPOW) {
+        int e = (int)n - 1;
+        double r = 1.0, b = x;
+        while (e > 0) {
+            if (e & 1) r *= b;
+            b *= b;
+            e >>= 1;
+        }
+        return r;
+    } else {
+        return pow(x, n - 1.0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// final reduction of block partials: out[i] = sum_b part[b*nvals + i]
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_reduce_final(const double* __restrict__ part, int nblocks, int nvals,
+                                                     double* __restrict__ out) {
+    const int i = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += BS) s += part[(size_t)b * nvals + i];
+    __shared__ double sm[BS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (lane == 0) sm[wid] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = sm[0];
+        for (int w = 1; w < BS / 64; w++) t += sm[w];
+        out[i] = t;
+    }
+}
+void launch_reduce_final(const double* part, int nblocks, int nvals, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_final, dim3(nvals), dim3(BS), 0, s, part, nblocks, nvals, out);
+}
+
+// max/min variants for the kinematic / diagnostic scans
+template <class Op>
+__global__ __launch_bounds__(BS) void k_reduce_final_op(const double* __restrict__ part, int nblocks, int nvals,
+                                                        double* __restrict__ out) {
+    const int i = blockIdx.x;
+    double s = Op::init();
+    for (int b = threadIdx.x; b < nblocks; b += BS) s = Op::f(s, part[(size_t)b * nvals + i]);
+    __shared__ double sm[BS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) s = Op::f(s, __shfl_down(s, o, 64));
+    if (lane == 0) sm[wid] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = sm[0];
+        for (int w = 1; w < BS / 64; w++) t = Op::f(t, sm[w]);
+        out[i] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// <sigma> = sum_v w sigma  (LS-EVPFFT.f90:238-254)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_sigma_sum(Fields f) {
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (v < f.V) {
+        double s6[6], s[9];
+        for (int c = 0; c < 6; c++) s6[c] = f.sg[c * f.V + v];
+        from6(s6, s);
+        const double w = f.w[v];
+        for (int c = 0; c < 9; c++) acc[c] = s[c] * w;
+    }
+    block_reduce_store<9, BS>(acc, f.part);
+}
+void launch_sigma_sum(const Fields& f, hipStream_t s, int* nb) {
+    *nb = nblk(f.V);
+    hipLaunchKernelGGL(k_sigma_sum, dim3(*nb), dim3(BS), 0, s, f);
+}
+
+// ---------------------------------------------------------------------------
+// Fused pass: correction_to_current + dL_iter/mixing + solve_res
+// ---------------------------------------------------------------------------
+struct SolveArgs {
+    PassConst pc;
+    double dvelgradavg[9];
+    double finv_inv[9];
+    const double* fluct;  // 9 reduced sums of w dLref Fi
+};
+
+// Power-law slip rates at sigma (b-basis) -> edotp (5), dedotp (sym 5x5, 15), optional gdot store.
+template <bool INTPOW, bool STORE>
+__device__ __forceinline__ void slip_eval(const Fields& f, const double* __restrict__ tab, const PhaseDev& P,
+                                          long v, const double* sg, double* ep, double* dd) {
+    const long V = f.V;
+    for (int i = 0; i < 5; i++) ep[i] = 0.0;
+    for (int i = 0; i < 15; i++) dd[i] = 0.0;
+    const double* nrs = tab + P.off_nrs;
+    const double* g0 = tab + P.off_gamd0;
+    for (int is = 0; is < P.nsyst; is++) {
+        double s0 = f.sch[(5 * is + 0) * V + v], s1 = f.sch[(5 * is + 1) * V + v], s2 = f.sch[(5 * is + 2) * V + v],
+               s3 = f.sch[(5 * is + 3) * V + v], s4 = f.sch[(5 * is + 4) * V + v];
+        double rss = s0 * sg[0] + s1 * sg[1] + s2 * sg[2] + s3 * sg[3] + s4 * sg[4];
+        const int isg = rss < 0.0 ? 1 : 0;
+        const double tau = f.ttau[(2 * is + isg) * V + v];
+        rss = rss / tau;
+        const double n = nrs[is];
+        const double dum = g0[is] * powm1<INTPOW>(fabs(rss), n);
+        const double r1 = n * dum / tau;
+        const double r2 = rss * dum;
+        if (STORE) f.gdot[is * V + v] = r2;
+        ep[0] += s0 * r2; ep[1] += s1 * r2; ep[2] += s2 * r2; ep[3] += s3 * r2; ep[4] += s4 * r2;
+        const double t0 = s0 * r1, t1 = s1 * r1, t2 = s2 * r1, t3 = s3 * r1, t4 = s4 * r1;
+        dd[0] += s0 * t0; dd[1] += s0 * t1; dd[2] += s0 * t2; dd[3] += s0 * t3; dd[4] += s0 * t4;
+        dd[5] += s1 * t1; dd[6] += s1 * t2; dd[7] += s1 * t3; dd[8] += s1 * t4;
+        dd[9] += s2 * t2; dd[10] += s2 * t3; dd[11] += s2 * t4;
+        dd[12] += s3 * t3; dd[13] += s3 * t4;
+        dd[14] += s4 * t4;
+    }
+}
+// gdot only (for harden and the final store): same arithmetic as slip_eval
+template <bool INTPOW>
+__device__ __forceinline__ void slip_rates(const Fields& f, const double* __restrict__ tab, const PhaseDev& P,
+                                           long v, const double* sg) {
+    const long V = f.V;
+    const double* nrs = tab + P.off_nrs;
+    const double* g0 = tab + P.off_gamd0;
+    for (int is = 0; is < P.nsyst; is++) {
+        double s0 = f.sch[(5 * is + 0) * V + v], s1 = f.sch[(5 * is + 1) * V + v], s2 = f.sch[(5 * is + 2) * V + v],
+               s3 = f.sch[(5 * is + 3) * V + v], s4 = f.sch[(5 * is + 4) * V + v];
+        double rss = s0 * sg[0] + s1 * sg[1] + s2 * sg[2] + s3 * sg[3] + s4 * sg[4];
+        const int isg = rss < 0.0 ? 1 : 0;
+        const double tau = f.ttau[(2 * is + isg) * V + v];
+        rss = rss / tau;
+        const double dum = g0[is] * powm1<INTPOW>(fabs(rss), nrs[is]);
+        f.gdot[is * V + v] = rss * dum;
+    }
+}
+__device__ __forceinline__ int sym5(int i, int j) {  // packed index of the symmetric 5x5
+    if (i > j) { int t = i; i = j; j = t; }
+    return i * 5 - (i * (i - 1)) / 2 + (j - i);
+}
+
+// Voce hardening (harden, various_functions.f90:1439-1491) from f.gdot; writes the
+// new trial strengths to ttau; returns ||ttau_old - ttau_new||^2.
+__device__ __forceinline__ double harden_voxel(const Fields& f, const double* __restrict__ tab, const PhaseDev& P,
+                                               long v, double tdot, double* gamma1_out, bool store_crss) {
+    const long V = f.V;
+    double deltgam = 0.0;
+    for (int is = 0; is < P.nsyst; is++) deltgam += fabs(f.gdot[is * V + v]) * tdot;
+    const double gamma0 = f.gacc[v];
+    const double gamma1 = gamma0 + deltgam;
+    *gamma1_out = gamma1;
+    const double* hard = tab + P.off_hard;
+    double e2 = 0.0;
+    int is = 0;
+    for (int m = 0; m < P.nmodes; m++) {
+        const double tau1 = P.mode_tau1[m], th0 = P.mode_thet0[m], th1 = P.mode_thet1[m];
+        const double fact = fabs(th0 / tau1);
+        const double ex0 = exp(-fact * gamma0), ex1 = exp(-fact * gamma1);
+        const double voce = deltgam * th1 + tau1 * (ex0 - ex1) + th1 * (gamma0 * ex0 - gamma1 * ex1);
+        const double sc = voce * tdot / deltgam;
+        for (int k = 0; k < P.nsm[m]; k++, is++) {
+            double dtau = 0.0;
+            for (int js = 0; js < P.nsyst; js++) dtau += hard[is * P.nsyst + js] * fabs(f.gdot[js * V + v]);
+            const double t0 = f.crss[(2 * is) * V + v] + dtau * sc;
+            const double t1 = f.crss[(2 * is + 1) * V + v] + dtau * sc;
+            const double o0 = f.ttau[(2 * is) * V + v], o1 = f.ttau[(2 * is + 1) * V + v];
+            e2 += (o0 - t0) * (o0 - t0) + (o1 - t1) * (o1 - t1);
+            f.ttau[(2 * is) * V + v] = t0;
+            f.ttau[(2 * is + 1) * V + v] = t1;
+            if (store_crss) {
+                f.crss[(2 * is) * V + v] = t0;
+                f.crss[(2 * is + 1) * V + v] = t1;
+            }
+        }
+    }
+    return e2;
+}
+
+template <bool MODAL, bool INTPOW>
+__global__ __launch_bounds__(BS) void k_correct_solve(Fields f, PhaseTables pt, SolveArgs a) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    double acc[R_NSOLVE];
+#pragma unroll
+    for (int i = 0; i < R_NSOLVE; i++) acc[i] = 0.0;
+    const PassConst& pc = a.pc;
+    if (v < V) {
+        // ---- correction_to_current (dudot from the reduced fluctuation average)
+        double dud[9];
+        {
+            double dm[9];
+            for (int i = 0; i < 9; i++) dm[i] = a.dvelgradavg[i] - a.fluct[i];
+            mm3(dm, a.finv_inv, dud);
+        }
+        double L[9], Lo[9], Fi[9], t[9];
+        for (int c = 0; c < 9; c++) {
+            L[c] = f.L[c * V + v];
+            Fi[c] = f.Fi[c * V + v];
+            t[c] = f.dLref[c * V + v] + dud[c];
+        }
+        if (pc.lold_same) {
+            for (int c = 0; c < 9; c++) Lo[c] = L[c];
+        } else {
+            for (int c = 0; c < 9; c++) Lo[c] = f.Lold[c * V + v];
+        }
+        double q[9];
+        mm3(t, Fi, q);
+        for (int c = 0; c < 9; c++) L[c] += q[c];
+        const double w = f.w[v];
+        const bool gas = f.gas[v] != 0;
+        for (int c = 0; c < 9; c++) {
+            acc[R_LALL + c] = L[c] * w;
+            acc[(gas ? R_LGAS : R_LSOL) + c] = L[c] * w;
+        }
+        // ---- dL_iter and mixing (LS-EVPFFT.f90:277-327)
+        if (!pc.kin_ctrl) {
+            double d2 = 0.0;
+            for (int c = 0; c < 9; c++) d2 += (L[c] - Lo[c]) * (L[c] - Lo[c]);
+            const double rel = sqrt(d2) / pc.lscale;
+            acc[gas ? R_DLGAS : R_DLSOL] = rel * rel * w;
+            acc[gas ? R_WGAS : R_WSOL] = w;
+        }
+        if (pc.do_mix) {
+            for (int c = 0; c < 9; c++) L[c] = pc.mix * L[c] + (1.0 - pc.mix) * Lo[c];
+        } else {
+            for (int c = 0; c < 9; c++) L[c] = pc.mix * L[c];
+        }
+        for (int c = 0; c < 9; c++) f.L[c * V + v] = L[c];
+
+        // ---- solve_res (various_functions.f90:1142-1364)
+        double lam[6], sgt[6], D[6];
+        for (int c = 0; c < 6; c++) {
+            lam[c] = f.sg[c * V + v];
+            sgt[c] = f.sgt[c * V + v];
+        }
+        to6(L, D);
+        const double tdot = pc.tdot;
+        double sgf[6];
+        if (!gas) {
+            const PhaseDev& P = pt.ph[f.phase[v]];
+            const double* tab = pt.data;
+            double cm[36], Jb[36], r0[6];
+            for (int i = 0; i < 36; i++) cm[i] = f.c66m[(long)i * V + v];
+            {
+                // Jb = I + cm s66 / tdot [- cGm];  r0 = -lambda - (cm s66/tdot) sgt - cm D [+ cGm lambda]
+                double s66[36];
+                for (int i = 0; i < 36; i++) s66[i] = f.sg66[(long)i * V + v] / tdot;
+                mm6(cm, s66, Jb);
+                for (int i = 0; i < 6; i++) {
+                    double s = -lam[i];
+                    for (int k = 0; k < 6; k++) s -= Jb[6 * i + k] * sgt[k] + cm[6 * i + k] * D[k];
+                    r0[i] = s;
+                }
+                for (int i = 0; i < 6; i++) Jb[7 * i] += 1.0;
+                if (MODAL) {
+                    for (int i = 0; i < 6; i++) {
+                        double s = 0.0;
+                        for (int k = 0; k < 6; k++) {
+                            const double g = f.cgm[(long)(6 * i + k) * V + v];
+                            Jb[6 * i + k] -= g;
+                            s += g * lam[k];
+                        }
+                        r0[i] += s;
+                    }
+                }
+            }
+            double sg[6], sgold[6] = {0, 0, 0, 0, 0, 0}, sev[6], ep[5], dd[15];
+            for (int c = 0; c < 6; c++) sg[c] = lam[c];
+            double sgnorm = 0.0, taun = 0.0;
+            for (int c = 0; c < 6; c++) sgnorm += sg[c] * sg[c];
+            sgnorm = sqrt(sgnorm);
+            for (int k = 0; k < 2 * P.nsyst; k++) {
+                const double x = f.ttau[(long)k * V + v];
+                taun += x * x;
+            }
+            taun = sqrt(taun);
+            const double tol = fmax(pc.error, pc.erroral);
+            int itertau = 0;
+            double errtau = 2.0 * tol;
+            double dsgn = 0.0;
+            long nnr = 0, nev = 0, nout = 0;
+            while (itertau < 10 && errtau > tol) {
+                itertau++;
+                nout++;
+                int iternr = 0;
+                double erral = 2.0 * pc.erroral, resoldn = 1.0;
+                double res[6];
+                while (iternr < 100 && erral > pc.erroral) {
+                    iternr++;
+                    nnr++;
+                    int icut = 0;
+                    double resn = 2.0 * resoldn;
+                    while (resn > resoldn && icut < 10) {
+                        nev++;
+                        for (int c = 0; c < 6; c++) sev[c] = sg[c];
+                        slip_eval<INTPOW, false>(f, tab, P, v, sg, ep, dd);
+                        double rn = 0.0;
+                        for (int i = 0; i < 6; i++) {
+                            double s = r0[i];
+                            for (int k = 0; k < 6; k++) s += Jb[6 * i + k] * sg[k];
+                            for (int k = 0; k < 5; k++) s += cm[6 * i + k] * ep[k];
+                            res[i] = s;
+                            rn += s * s;
+                        }
+                        resn = sqrt(rn);
+                        if (iternr == 1) resoldn = resn;
+                        if (resn > resoldn)
+                            for (int c = 0; c < 6; c++) sg[c] = sgold[c] + (sg[c] - sgold[c]) * 0.5;
+                        icut++;
+                    }
+                    resoldn = resn;
+                    for (int c = 0; c < 6; c++) sgold[c] = sg[c];
+                    double Jm[36];
+                    for (int i = 0; i < 6; i++)
+                        for (int j = 0; j < 6; j++) {
+                            double s = Jb[6 * i + j];
+                            if (j < 5)
+                                for (int k = 0; k < 5; k++) s += cm[6 * i + k] * dd[sym5(k, j)];
+                            Jm[6 * i + j] = s;
+                        }
+                    solve6(Jm, res);
+                    double d1 = 0.0;
+                    for (int c = 0; c < 6; c++) {
+                        sg[c] -= res[c];
+                        d1 += (sg[c] - sgold[c]) * (sg[c] - sgold[c]);
+                    }
+                    erral = sqrt(d1) / sgnorm;
+                }
+                // harden with the slip rates of the last evaluation
+                slip_rates<INTPOW>(f, tab, P, v, sev);
+                if (pc.iuphard == 1) {
+                    double g1;
+                    const double e2 = harden_voxel(f, tab, P, v, tdot, &g1, false);
+                    errtau = sqrt(e2) / taun;
+                } else {
+                    errtau = 0.0;
+                }
+            }
+            // errors: dsgnorm = |sg - lambda|^2, ddgnorm = |edot(last eval) - D|^2
+            double s66sev[6], dd2 = 0.0;
+            for (int c = 0; c < 6; c++) {
+                dsgn += (sg[c] - lam[c]) * (sg[c] - lam[c]);
+            }
+            for (int i = 0; i < 6; i++) {
+                double s = 0.0;
+                for (int k = 0; k < 6; k++) s += f.sg66[(long)(6 * i + k) * V + v] * ((sev[k] - sgt[k]) / tdot);
+                s66sev[i] = s;
+            }
+            for (int i = 0; i < 6; i++) {
+                const double e = (i < 5 ? ep[i] : 0.0) + s66sev[i] - D[i];
+                dd2 += e * e;
+            }
+            acc[R_ERRS] = dsgn * w;
+            acc[R_ERRE] = dd2 * w;
+            for (int c = 0; c < 6; c++) sgf[c] = sg[c];
+            for (int c = 0; c < 5; c++) f.edotp[c * V + v] = ep[c];
+            f.edotp[5 * V + v] = 0.0;
+            acc[R_NR] = (double)nnr;
+            acc[R_EVAL] = (double)nev;
+            acc[R_OUTER] = (double)nout;
+        } else {
+            // gas voxel (various_functions.f90:1326-1345)
+            double rr[6], tt[6];
+            for (int i = 0; i < 6; i++) tt[i] = D[i] + sgt[i] / tdot * pc.complgas;
+            for (int i = 0; i < 6; i++) {
+                double s = lam[i];
+                for (int k = 0; k < 6; k++) s += f.c66m[(long)(6 * i + k) * V + v] * tt[k];
+                if (MODAL)
+                    for (int k = 0; k < 6; k++) s -= f.cgm[(long)(6 * i + k) * V + v] * lam[k];
+                rr[i] = s;
+            }
+            for (int i = 0; i < 6; i++) {
+                double s = 0.0;
+                for (int k = 0; k < 6; k++) s += f.cgas[(long)(6 * i + k) * V + v] * rr[k];
+                sgf[i] = s;
+            }
+            for (int c = 0; c < 6; c++) f.edotp[c * V + v] = 0.0;
+            for (int is = 0; is < f.ns; is++) f.gdot[is * V + v] = 0.0;
+        }
+        for (int c = 0; c < 6; c++) f.sg[c * V + v] = sgf[c];
+        double s3[9];
+        from6(sgf, s3);
+        for (int c = 0; c < 9; c++) {
+            acc[R_SGALL + c] = s3[c] * w;
+            if (!gas) acc[R_SGSOL + c] = s3[c] * w;
+        }
+    }
+    block_reduce_store<R_NSOLVE, BS>(acc, f.part);
+}
+
+// Public launcher with the full argument set.
+void launch_correct_solve_full(const Fields& f, const PhaseTables& pt, const PassConst& pc,
+                               const double* dvelgradavg, const double* finv_inv, const double* fluct_dev,
+                               hipStream_t s, int* nb, int int_pow) {
+    SolveArgs a;
+    a.pc = pc;
+    for (int i = 0; i < 9; i++) { a.dvelgradavg[i] = dvelgradavg[i]; a.finv_inv[i] = finv_inv[i]; }
+    a.fluct = fluct_dev;
+    *nb = nblk(f.V);
+    if (pc.modal) {
+        if (int_pow) hipLaunchKernelGGL((k_correct_solve<true, true>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
+        else hipLaunchKernelGGL((k_correct_solve<true, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
+    } else {
+        if (int_pow) hipLaunchKernelGGL((k_correct_solve<false, true>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
+        else hipLaunchKernelGGL((k_correct_solve<false, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// update_schmid (various_functions.f90:472-539): sch = b5(sym(A (b x n) A^-1)), A = Fe ag
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_update_schmid(Fields f, PhaseTables pt) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    if (v >= V || f.gas[v]) return;
+    const PhaseDev& P = pt.ph[f.phase[v]];
+    const double* dn = pt.data + P.off_dnca;
+    const double* db = pt.data + P.off_dbca;
+    double Fe[9], ag[9], A[9], Ai[9];
+    for (int c = 0; c < 9; c++) { Fe[c] = f.Fe[c * V + v]; ag[c] = f.ag[c * V + v]; }
+    mm3(Fe, ag, A);
+    inv3(A, Ai);
+    for (int is = 0; is < P.nsyst; is++) {
+        // A (b x n) A^-1 = (A b) x (A^-T n)
+        double ab[3], an[3];
+        for (int i = 0; i < 3; i++) {
+            ab[i] = A[3 * i] * db[3 * is] + A[3 * i + 1] * db[3 * is + 1] + A[3 * i + 2] * db[3 * is + 2];
+            an[i] = Ai[i] * dn[3 * is] + Ai[3 + i] * dn[3 * is + 1] + Ai[6 + i] * dn[3 * is + 2];
+        }
+        double m[9], s6[6];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) m[3 * i + j] = ab[i] * an[j];
+        to6(m, s6);
+        for (int c = 0; c < 5; c++) f.sch[(5 * is + c) * V + v] = s6[c];
+    }
+}
+void launch_update_schmid(const Fields& f, const PhaseTables& pt, hipStream_t s) {
+    hipLaunchKernelGGL(k_update_schmid, dim3(nblk(f.V)), dim3(BS), 0, s, f, pt);
+}
+
+// ---------------------------------------------------------------------------
+// calc_c066mod_Goperr066mod (various_functions.f90:812-906)
+//   c066mod = M(F) c066 M(F)^T / J,  M(X)_{ca} = b_c : (b_a X^T)
+//   cGm     = c066mod * [J Y_c : G0 : Y_d],  Y_c = b_c Fi^T
+// ---------------------------------------------------------------------------
+struct C066Args {
+    double c066[36];
+    double g0[81];
+};
+__global__ __launch_bounds__(BS) void k_c066mod(Fields f, C066Args a, int modal) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    if (v >= V) return;
+    double F[9], M[36], c[36];
+    for (int i = 0; i < 9; i++) F[i] = f.F[i * V + v];
+    const double J = f.J[v];
+    push_matrix(F, M);
+    congr6(M, a.c066, c);
+    for (int i = 0; i < 36; i++) {
+        c[i] /= J;
+        f.c66m[(long)i * V + v] = c[i];
+    }
+    if (modal) {
+        double Fi[9], Y[6][9], g66[36];
+        for (int i = 0; i < 9; i++) Fi[i] = f.Fi[i * V + v];
+        for (int cc = 0; cc < 6; cc++) {
+            double b[9];
+            basis(cc, b);
+            mmt3(b, Fi, Y[cc]);
+        }
+        for (int cc = 0; cc < 6; cc++) {
+            double z[9];
+            for (int kn = 0; kn < 9; kn++) {  // (G0 : Y_d) with index pair (i,m) free
+                double s = 0.0;
+                for (int im = 0; im < 9; im++) s += Y[cc][im] * a.g0[9 * im + kn];
+                z[kn] = s;
+            }
+            for (int d = 0; d < 6; d++) {
+                double s = 0.0;
+                for (int kn = 0; kn < 9; kn++) s += z[kn] * Y[d][kn];
+                g66[6 * cc + d] = s * J;
+            }
+        }
+        double r[36];
+        mm6(c, g66, r);
+        for (int i = 0; i < 36; i++) f.cgm[(long)i * V + v] = r[i];
+    }
+}
+void launch_c066mod(const Fields& f, const double* c066, const double* goperr0, int modal, hipStream_t s) {
+    C066Args a;
+    for (int i = 0; i < 36; i++) a.c066[i] = c066[i];
+    for (int i = 0; i < 81; i++) a.g0[i] = goperr0 ? goperr0[i] : 0.0;
+    hipLaunchKernelGGL(k_c066mod, dim3(nblk(f.V)), dim3(BS), 0, s, f, a, modal);
+}
+
+// calc_gas_compl (various_functions.f90:909-951)
+__global__ __launch_bounds__(BS) void k_gas_compl(Fields f, double tdot, double complgas, int modal) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    if (v >= V) return;
+    double a[36];
+    if (!f.gas[v]) {
+        for (int i = 0; i < 36; i++) f.cgas[(long)i * V + v] = 0.0;
+        return;
+    }
+    for (int i = 0; i < 36; i++) {
+        a[i] = (i % 7 == 0 ? 1.0 : 0.0) + f.c66m[(long)i * V + v] / tdot * complgas;
+        if (modal) a[i] -= f.cgm[(long)i * V + v];
+    }
+    inv6(a);
+    for (int i = 0; i < 36; i++) f.cgas[(long)i * V + v] = a[i];
+}
+void launch_gas_compl(const Fields& f, double tdot, double complgas, int modal, hipStream_t s) {
+    hipLaunchKernelGGL(k_gas_compl, dim3(nblk(f.V)), dim3(BS), 0, s, f, tdot, complgas, modal);
+}
+
+// ---------------------------------------------------------------------------
+// Goperr0 = (1/V) sum_xi Gamma(xi)  (the origin value of the inverse DFT of Gamma,
+// which form_G obtains through an 81-component inverse FFT, :784-807).
+// One thread per frequency of the full spectrum; 81 block partial sums.
+// ---------------------------------------------------------------------------
+struct GArgs { double c0[81]; double s0[81]; int igamma; };
+__device__ void green_full(const GArgs& a, int kx, int ky, int kz, int n1, int n2, int n3, double* g) {
+    // 0-based frequency indices; spatial_freq (fourier_functions.f90:340-371)
+    const double tp = 2.0 * PI_D;
+    double xi[3];
+    // spatial_freq: 1-based index x = kx+1; x <= n/2 -> kx, else kx - n
+    xi[0] = tp * (double)((kx + 1) <= n1 / 2 ? kx : kx - n1) / n1;
+    xi[1] = tp * (double)((ky + 1) <= n2 / 2 ? ky : ky - n2) / n2;
+    xi[2] = (n3 == 1) ? 0.0 : tp * (double)((kz + 1) <= n3 / 2 ? kz : kz - n3) / n3;
+    const bool nx = (n1 % 2 == 0) && kx == n1 / 2, ny = (n2 % 2 == 0) && ky == n2 / 2,
+               nz = (n3 % 2 == 0) && kz == n3 / 2;
+    if (a.igamma == 0 && (nx || ny || (nz && n3 > 1))) {
+        for (int i = 0; i < 81; i++) g[i] = -a.s0[i];
+        return;
+    }
+    if (a.igamma >= 2 && ((nx && ny) || (nx && nz) || (ny && nz))) {
+        for (int i = 0; i < 81; i++) g[i] = 0.0;
+        return;
+    }
+    double D[9];
+    if (a.igamma == 0) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) D[3 * i + j] = xi[i] * xi[j];
+    } else if (a.igamma == 1) {
+        const double xs = tp * kx / n1, ys = tp * ky / n2, zs = tp * kz / n3;
+        D[0] = 2.0 * (cos(xs) - 1.0); D[4] = 2.0 * (cos(ys) - 1.0); D[8] = 2.0 * (cos(zs) - 1.0);
+        D[1] = D[3] = -sin(xs) * sin(ys);
+        D[2] = D[6] = -sin(xs) * sin(zs);
+        D[5] = D[7] = -sin(ys) * sin(zs);
+    } else {
+        // kmod = i/4 prod_l (1 + e^{i xi_l}) * tan(xi/2);  D = Re(kmod kmod^H)
+        double pr = 1.0, pim = 0.0;
+        for (int l = 0; l < 3; l++) {
+            const double cr = 1.0 + cos(xi[l]), ci = sin(xi[l]);
+            const double nr = pr * cr - pim * ci, ni = pr * ci + pim * cr;
+            pr = nr; pim = ni;
+        }
+        // f = i*0.25*(pr + i pim) = 0.25*(-pim + i pr); |f|^2 = 0.0625 (pr^2 + pim^2)
+        const double f2 = 0.0625 * (pr * pr + pim * pim);
+        double tn[3];
+        for (int l = 0; l < 3; l++) tn[l] = tan(xi[l] * 0.5);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) D[3 * i + j] = tn[i] * tn[j] * f2;
+    }
+    double A[9], Ai[9];
+    if (kx == 0 && ky == 0 && kz == 0) {
+        for (int i = 0; i < 9; i++) Ai[i] = 0.0;
+    } else {
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) {
+                double s = 0.0;
+                for (int j = 0; j < 3; j++)
+                    for (int l = 0; l < 3; l++) s += a.c0[27 * i + 9 * j + 3 * k + l] * D[3 * j + l];
+                A[3 * i + k] = s;
+            }
+        inv3(A, Ai);
+    }
+    for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++)
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) g[27 * p + 9 * q + 3 * i + j] = -Ai[3 * p + i] * D[3 * q + j];
+    if (a.igamma == 1) {  // G <- -G : C0 : G, four times (:744-780)
+        double gt[81], aux[81];
+        for (int i = 0; i < 81; i++) gt[i] = g[i];
+        for (int it = 0; it < 4; it++) {
+            for (int ij = 0; ij < 9; ij++) {
+                double cg[9];
+                for (int op = 0; op < 9; op++) {
+                    double s = 0.0;
+                    for (int mn = 0; mn < 9; mn++) s += g[9 * ij + mn] * a.c0[9 * mn + op];
+                    cg[op] = s;
+                }
+                for (int kl = 0; kl < 9; kl++) {
+                    double s = 0.0;
+                    for (int op = 0; op < 9; op++) s += cg[op] * gt[9 * op + kl];
+                    aux[9 * ij + kl] = -s;
+                }
+            }
+            for (int i = 0; i < 81; i++) gt[i] = aux[i];
+        }
+        for (int i = 0; i < 81; i++) g[i] = gt[i];
+    }
+}
+__global__ __launch_bounds__(BS) void k_goperr0(Fields f, GArgs a) {
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    double acc[81];
+    for (int i = 0; i < 81; i++) acc[i] = 0.0;
+    if (v < f.V) {
+        const int kx = v % f.n1, ky = (v / f.n1) % f.n2, kz = v / ((long)f.n1 * f.n2);
+        green_full(a, kx, ky, kz, f.n1, f.n2, f.n3, acc);
+    }
+    block_reduce_store<81, BS>(acc, f.part);
+}
+void launch_goperr0(const Fields& f, const double* c0, const double* s0, int igamma, hipStream_t s, int* nb) {
+    GArgs a;
+    for (int i = 0; i < 81; i++) { a.c0[i] = c0[i]; a.s0[i] = s0[i]; }
+    a.igamma = igamma;
+    *nb = nblk(f.V);
+    hipLaunchKernelGGL(k_goperr0, dim3(*nb), dim3(BS), 0, s, f, a);
+}
+
+// ---------------------------------------------------------------------------
+// accepted-step accumulation (LS-EVPFFT.f90:431-446); max solid vm(sym L)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_accumulate(Fields f, double tdot) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    double acc[1] = {0.0};
+    if (v < V) {
+        double ep[6], L[9], d6[6];
+        for (int c = 0; c < 6; c++) {
+            ep[c] = f.edotp[c * V + v];
+            f.ept[c * V + v] += ep[c] * tdot;
+            f.sgt[c * V + v] = f.sg[c * V + v];
+        }
+        for (int c = 0; c < 9; c++) L[c] = f.L[c * V + v];
+        to6(L, d6);
+        for (int c = 0; c < 6; c++) f.dgs[c * V + v] += d6[c] * tdot;
+        f.epvm[v] += vm_strain6(ep) * tdot;
+        if (!f.gas[v]) acc[0] = vm_strain6(d6);
+    }
+    block_reduce_store<1, BS, OpMax>(acc, f.part);
+}
+void launch_accumulate(const Fields& f, double tdot, hipStream_t s, int* nb) {
+    *nb = nblk(f.V);
+    hipLaunchKernelGGL(k_accumulate, dim3(*nb), dim3(BS), 0, s, f, tdot);
+}
+
+// update_plastic_defgrad (various_functions.f90:1748-1827)
+__global__ __launch_bounds__(BS) void k_plastic_defgrad(Fields f, PhaseTables pt, double tdot) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    if (v >= V || f.gas[v]) return;
+    const PhaseDev& P = pt.ph[f.phase[v]];
+    const double* dn = pt.data + P.off_dnca;
+    const double* db = pt.data + P.off_dbca;
+    double ag[9], Lp[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < 9; c++) ag[c] = f.ag[c * V + v];
+    for (int is = 0; is < P.nsyst; is++) {
+        double n[3], b[3];
+        for (int i = 0; i < 3; i++) {
+            n[i] = ag[3 * i] * dn[3 * is] + ag[3 * i + 1] * dn[3 * is + 1] + ag[3 * i + 2] * dn[3 * is + 2];
+            b[i] = ag[3 * i] * db[3 * is] + ag[3 * i + 1] * db[3 * is + 1] + ag[3 * i + 2] * db[3 * is + 2];
+        }
+        const double g = f.gdot[is * V + v];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) Lp[3 * i + j] += b[i] * n[j] * g;
+    }
+    double d[9], E[9], Fp[9];
+    for (int c = 0; c < 9; c++) { d[c] = Lp[c] * tdot; Fp[c] = f.Fp[c * V + v]; }
+    expm3(d, E);
+    mm3(E, Fp, Fp);
+    for (int c = 0; c < 9; c++) f.Fp[c * V + v] = Fp[c];
+}
+void launch_plastic_defgrad(const Fields& f, const PhaseTables& pt, double tdot, hipStream_t s) {
+    hipLaunchKernelGGL(k_plastic_defgrad, dim3(nblk(f.V)), dim3(BS), 0, s, f, pt, tdot);
+}
+
+// update_defgrad (various_functions.f90:2152-2282): 39 sums
+//  0..8 sum w F, 9..17 sum wJ Fi, 18..26 solid, 27..35 gas, 36 sum_solid wJ, 37 sum_gas wJ, 38 sum wJ
+__global__ __launch_bounds__(BS) void k_update_defgrad(Fields f, double tdot) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    double acc[39];
+    for (int i = 0; i < 39; i++) acc[i] = 0.0;
+    if (v < V) {
+        double L[9], F[9], d[9], E[9], Fi[9];
+        for (int c = 0; c < 9; c++) { L[c] = f.L[c * V + v]; F[c] = f.F[c * V + v]; d[c] = L[c] * tdot; }
+        expm3(d, E);
+        mm3(E, F, F);
+        // determinant33 (tensor_functions.f90:442-456) term order
+        const double J = F[0] * F[4] * F[8] + F[1] * F[5] * F[6] + F[2] * F[3] * F[7] - F[6] * F[4] * F[2] -
+                         F[7] * F[5] * F[0] - F[8] * F[3] * F[1];
+        inv3(F, Fi);
+        const double wg = 1.0 / (double)V;
+        for (int c = 0; c < 9; c++) {
+            f.F[c * V + v] = F[c];
+            f.Fi[c * V + v] = Fi[c];
+            f.Finc[c * V + v] = E[c];
+            acc[c] = F[c] * wg;
+            acc[9 + c] = Fi[c] * wg * J;
+            acc[(f.gas[v] ? 27 : 18) + c] = Fi[c] * wg * J;
+        }
+        f.J[v] = J;
+        f.w[v] = wg * J;
+        acc[f.gas[v] ? 37 : 36] = wg * J;
+        acc[38] = J * wg;
+    }
+    block_reduce_store<39, BS>(acc, f.part);
+}
+void launch_update_defgrad(const Fields& f, double tdot, hipStream_t s, int* nb) {
+    *nb = nblk(f.V);
+    hipLaunchKernelGGL(k_update_defgrad, dim3(*nb), dim3(BS), 0, s, f, tdot);
+}
+__global__ __launch_bounds__(BS) void k_div_w(Fields f, double detfavg) {
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    if (v < f.V) f.w[v] = f.w[v] / detfavg;
+}
+void launch_div_w(const Fields& f, double detfavg, hipStream_t s) {
+    hipLaunchKernelGGL(k_div_w, dim3(nblk(f.V)), dim3(BS), 0, s, f, detfavg);
+}
+
+// update_elstic_defgrad (:2285-2324) + update_el_stiff (:2327-2402)
+//   Fe = F Fp^-1 (F_ini = I), Fe_inc = Fe Fe_old^-1,
+//   cg66 = T(A) C66cr T(A)^T / det Fe, sg66 = cg66^-1, A = Fe ag
+__global__ __launch_bounds__(BS) void k_elastic_stiff(Fields f, PhaseTables pt) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    if (v >= V || f.gas[v]) return;
+    const PhaseDev& P = pt.ph[f.phase[v]];
+    double F[9], Fp[9], Fpi[9], Feo[9], Feoi[9], Fe[9], Fei[9];
+    for (int c = 0; c < 9; c++) { F[c] = f.F[c * V + v]; Fp[c] = f.Fp[c * V + v]; Feo[c] = f.Fe[c * V + v]; }
+    inv3(Feo, Feoi);
+    inv3(Fp, Fpi);
+    mm3(F, Fpi, Fe);
+    mm3(Fe, Feoi, Fei);
+    for (int c = 0; c < 9; c++) { f.Fe[c * V + v] = Fe[c]; f.Feinc[c * V + v] = Fei[c]; }
+    double ag[9], A[9], T[36], c66[36];
+    for (int c = 0; c < 9; c++) ag[c] = f.ag[c * V + v];
+    mm3(Fe, ag, A);
+    rot_matrix(A, T);
+    congr6(T, P.c66cr, c66);
+    const double dFe = det3(Fe);
+    for (int i = 0; i < 36; i++) c66[i] /= dFe;
+    inv6(c66);
+    for (int i = 0; i < 36; i++) f.sg66[(long)i * V + v] = c66[i];
+}
+void launch_elastic_stiff(const Fields& f, const PhaseTables& pt, hipStream_t s) {
+    hipLaunchKernelGGL(k_elastic_stiff, dim3(nblk(f.V)), dim3(BS), 0, s, f, pt);
+}
+
+// update_tensor_config (:2405-2492), part a: rotations and L Finc^-1, 9 sums of w L
+__global__ __launch_bounds__(BS) void k_tensor_config_a(Fields f) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (v < V) {
+        double Fn[9], Fni[9], L[9];
+        for (int c = 0; c < 9; c++) { Fn[c] = f.Finc[c * V + v]; L[c] = f.L[c * V + v]; }
+        inv3(Fn, Fni);
+        if (!f.gas[v]) {
+            double R[9], lam[3], Fe[9];
+            polar3(Fn, R, lam);
+            for (int c = 0; c < 9; c++) Fe[c] = f.Feinc[c * V + v];
+            const double dfe = det3(Fe);
+            double s6[6], s[9], t[9];
+            // disgradsym, ept <- R (.) R^T
+            for (int c = 0; c < 6; c++) s6[c] = f.dgs[c * V + v];
+            from6(s6, s); mm3(R, s, t); mmt3(t, R, s); to6(s, s6);
+            for (int c = 0; c < 6; c++) f.dgs[c * V + v] = s6[c];
+            for (int c = 0; c < 6; c++) s6[c] = f.ept[c * V + v];
+            from6(s6, s); mm3(R, s, t); mmt3(t, R, s); to6(s, s6);
+            for (int c = 0; c < 6; c++) f.ept[c * V + v] = s6[c];
+            // sgt <- Fe_inc sgt Fe_inc^T / det(Fe_inc)
+            for (int c = 0; c < 6; c++) s6[c] = f.sgt[c * V + v];
+            from6(s6, s); mm3(Fe, s, t); mmt3(t, Fe, s);
+            for (int c = 0; c < 9; c++) s[c] /= dfe;
+            to6(s, s6);
+            for (int c = 0; c < 6; c++) f.sgt[c * V + v] = s6[c];
+        }
+        mm3(L, Fni, L);
+        const double w = f.w[v];
+        for (int c = 0; c < 9; c++) {
+            f.L[c * V + v] = L[c];
+            acc[c] = L[c] * w;
+        }
+    }
+    block_reduce_store<9, BS>(acc, f.part);
+}
+void launch_tensor_config_a(const Fields& f, hipStream_t s, int* nb) {
+    *nb = nblk(f.V);
+    hipLaunchKernelGGL(k_tensor_config_a, dim3(*nb), dim3(BS), 0, s, f);
+}
+struct Mat9 { double m[9]; };
+__global__ __launch_bounds__(BS) void k_tensor_config_b(Fields f, Mat9 dud) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    if (v >= V) return;
+    double Fi[9], q[9];
+    for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * V + v];
+    mm3(dud.m, Fi, q);
+    for (int c = 0; c < 9; c++) f.L[c * V + v] += q[c];
+}
+void launch_tensor_config_b(const Fields& f, const double* dud, hipStream_t s) {
+    Mat9 m;
+    for (int i = 0; i < 9; i++) m.m[i] = dud[i];
+    hipLaunchKernelGGL(k_tensor_config_b, dim3(nblk(f.V)), dim3(BS), 0, s, f, m);
+}
+
+// update_hardening (:1493-1523)
+__global__ __launch_bounds__(BS) void k_hardening(Fields f, PhaseTables pt, double tdot) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    if (v >= V || f.gas[v]) return;
+    const PhaseDev& P = pt.ph[f.phase[v]];
+    double g1;
+    harden_voxel(f, pt.data, P, v, tdot, &g1, true);
+    f.gacc[v] = g1;
+}
+void launch_hardening(const Fields& f, const PhaseTables& pt, double tdot, hipStream_t s) {
+    hipLaunchKernelGGL(k_hardening, dim3(nblk(f.V)), dim3(BS), 0, s, f, pt, tdot);
+}
+
+// macro averages (LS-EVPFFT.f90:460-523): 50 sums over solid voxels
+//  0..8 w ept, 9..17 w disgradsym, 18..26 (unused), 27..35 w edotp, 36..44 w sgt,
+//  45 gacumgr, 46 nsolid, 47 sum g, 48 sum |gdot|, 49 sum |gdot| g, (nsys is host-side)
+__global__ __launch_bounds__(BS) void k_averages(Fields f, PhaseTables pt) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    double acc[50];
+    for (int i = 0; i < 50; i++) acc[i] = 0.0;
+    if (v < V && !f.gas[v]) {
+        const double w = f.w[v];
+        double s6[6], s[9];
+        for (int c = 0; c < 6; c++) s6[c] = f.ept[c * V + v];
+        from6(s6, s);
+        for (int c = 0; c < 9; c++) acc[c] = s[c] * w;
+        for (int c = 0; c < 6; c++) s6[c] = f.dgs[c * V + v];
+        from6(s6, s);
+        for (int c = 0; c < 9; c++) acc[9 + c] = s[c] * w;
+        for (int c = 0; c < 6; c++) s6[c] = f.edotp[c * V + v];
+        from6(s6, s);
+        for (int c = 0; c < 9; c++) acc[27 + c] = s[c] * w;
+        for (int c = 0; c < 6; c++) s6[c] = f.sgt[c * V + v];
+        from6(s6, s);
+        for (int c = 0; c < 9; c++) acc[36 + c] = s[c] * w;
+        acc[45] = f.gacc[v];
+        acc[46] = 1.0;
+        const PhaseDev& P = pt.ph[f.phase[v]];
+        for (int is = 0; is < P.nsyst; is++) {
+            const double g = 0.5 * (f.crss[(2 * is) * V + v] + f.crss[(2 * is + 1) * V + v]);
+            const double a = fabs(f.gdot[is * V + v]);
+            acc[47] += g;
+            acc[48] += a;
+            acc[49] += a * g;
+        }
+    }
+    block_reduce_store<50, BS>(acc, f.part);
+}
+void launch_averages(const Fields& f, const PhaseTables& pt, hipStream_t s, int* nb) {
+    *nb = nblk(f.V);
+    hipLaunchKernelGGL(k_averages, dim3(*nb), dim3(BS), 0, s, f, pt);
+}
+
+// ---------------------------------------------------------------------------
+// calc_c0 (:2537-2649): per solid voxel
+//   T = (s66 tdot/tdotref / tdot + dedotp)^-1  (both tangents from the tdotref-scaled one)
+//   sum w T (36) and sum wgt N T N^T J (36),  N_{ca} = b_c : (b_a Fi^T)
+// ---------------------------------------------------------------------------
+template <bool INTPOW>
+__global__ __launch_bounds__(BS) void k_calc_c0(Fields f, PhaseTables pt, double tdot, double tdotref) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    double acc[72];
+    for (int i = 0; i < 72; i++) acc[i] = 0.0;
+    if (v < V && !f.gas[v]) {
+        const PhaseDev& P = pt.ph[f.phase[v]];
+        double sg[6], ep[5], dd[15], T[36];
+        for (int c = 0; c < 6; c++) sg[c] = f.sg[c * V + v];
+        // plastic_strain_rate on a local copy (gdot is not modified: STORE=false)
+        slip_eval<INTPOW, false>(f, pt.data, P, v, sg, ep, dd);
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) {
+                double e = f.sg66[(long)(6 * i + j) * V + v] / tdot * tdot / tdotref;
+                if (i < 5 && j < 5) e += dd[sym5(i, j)];
+                T[6 * i + j] = e;
+            }
+        inv6(T);
+        double Fi[9], N[36], dP[36];
+        for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * V + v];
+        push_matrix(Fi, N);
+        congr6(N, T, dP);
+        const double J = f.J[v], w = f.w[v], wg = 1.0 / (double)V;
+        for (int i = 0; i < 36; i++) {
+            acc[i] = T[i] * w;
+            acc[36 + i] = dP[i] * J * wg;
+        }
+    }
+    block_reduce_store<72, BS>(acc, f.part);
+}
+void launch_calc_c0(const Fields& f, const PhaseTables& pt, double tdot, double tdotref, hipStream_t s, int* nb,
+                    int int_pow) {
+    *nb = nblk(f.V);
+    if (int_pow) hipLaunchKernelGGL(k_calc_c0<true>, dim3(*nb), dim3(BS), 0, s, f, pt, tdot, tdotref);
+    else hipLaunchKernelGGL(k_calc_c0<false>, dim3(*nb), dim3(BS), 0, s, f, pt, tdot, tdotref);
+}
+
+// ---------------------------------------------------------------------------
+// kinematic_increments (:2766-2837): max vol / rot / stretch increments, min J next
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void kin_voxel(const double* Finc, double* vol, double* rot, double* str) {
+    const double TINY = 2.2250738585072014e-308;
+    const double Jinc = Finc[0] * Finc[4] * Finc[8] + Finc[1] * Finc[5] * Finc[6] + Finc[2] * Finc[3] * Finc[7] -
+                        Finc[6] * Finc[4] * Finc[2] - Finc[7] * Finc[5] * Finc[0] - Finc[8] * Finc[3] * Finc[1];
+    *vol = fabs(log(fmax(Jinc, TINY)));
+    double R[9], lam[3];
+    polar3(Finc, R, lam);
+    double arg = 0.5 * (R[0] + R[4] + R[8] - 1.0);
+    arg = fmax(-1.0, fmin(1.0, arg));
+    *rot = acos(arg) * 180.0 / PI_D;
+    double s = 0.0;
+    for (int i = 0; i < 3; i++) {
+        const double l = log(fmax(lam[i], TINY));
+        s += l * l;
+    }
+    *str = sqrt(s);
+}
+__global__ __launch_bounds__(BS) void k_kinematic(Fields f, double tdot) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    double mx[3] = {0.0, 0.0, 0.0}, mn[1] = {1.7976931348623157e308};
+    if (v < V) {
+        double d[9], E[9];
+        for (int c = 0; c < 9; c++) d[c] = f.L[c * V + v] * tdot;
+        expm3(d, E);
+        const double Jinc = E[0] * E[4] * E[8] + E[1] * E[5] * E[6] + E[2] * E[3] * E[7] - E[6] * E[4] * E[2] -
+                            E[7] * E[5] * E[0] - E[8] * E[3] * E[1];
+        kin_voxel(E, &mx[0], &mx[1], &mx[2]);
+        mn[0] = f.J[v] * Jinc;
+    }
+    const int nb = gridDim.x;
+    block_reduce_store<3, BS, OpMax>(mx, f.part);
+    __syncthreads();
+    block_reduce_store<1, BS, OpMin>(mn, f.part + 3 * (size_t)nb);
+}
+void launch_kinematic(const Fields& f, double tdot, hipStream_t s, int* nb) {
+    *nb = nblk(f.V);
+    hipLaunchKernelGGL(k_kinematic, dim3(*nb), dim3(BS), 0, s, f, tdot);
+    hipLaunchKernelGGL(k_reduce_final_op<OpMax>, dim3(3), dim3(BS), 0, s, f.part, *nb, 3, f.red);
+    hipLaunchKernelGGL(k_reduce_final_op<OpMin>, dim3(1), dim3(BS), 0, s, f.part + 3 * (size_t)(*nb), *nb, 1,
+                       f.red + 3);
+}
+
+// ---------------------------------------------------------------------------
+// update_diagnostics (types.f90:345-758)
+// sums: 0 s2all 1 wall 2 s2sol 3 wsol 4 s2gas 5 wgas
+// max : 0 maxall 1 maxsol 2 maxgas 3 volsol 4 volgas 5 strsol 6 strgas 7 rotsol 8 rotgas 9 condsol 10 condgasc 11 condcgas
+// min : 0 detF all ; argmin (value,index) solid / gas
+// ---------------------------------------------------------------------------
+__device__ double cond1_est6(const double* A) {
+    // Hager-Higham 1-norm estimate of ||A^-1||_1 (Higham 1988, ACM TOMS 674) times ||A||_1
+    const double HUGEV = 1.7976931348623157e308;
+    double anorm = 0.0;
+    for (int j = 0; j < 6; j++) {
+        double s = 0.0;
+        for (int i = 0; i < 6; i++) s += fabs(A[6 * i + j]);
+        anorm = fmax(anorm, s);
+    }
+    double LU[36];
+    int piv[6];
+    for (int i = 0; i < 36; i++) LU[i] = A[i];
+    if (!lu6(LU, piv) || anorm == 0.0) return HUGEV;
+    double x[6], xs[6];
+    for (int i = 0; i < 6; i++) x[i] = 1.0 / 6.0;
+    lu6_solve(LU, piv, x);
+    double est = 0.0;
+    for (int i = 0; i < 6; i++) est += fabs(x[i]);
+    for (int i = 0; i < 6; i++) { xs[i] = x[i] >= 0.0 ? 1.0 : -1.0; x[i] = xs[i]; }
+    lu6_solve_t(LU, piv, x);
+    int j = 0;
+    for (int i = 1; i < 6; i++) if (fabs(x[i]) > fabs(x[j])) j = i;
+    int iter = 2;
+    for (;;) {
+        for (int i = 0; i < 6; i++) x[i] = 0.0;
+        x[j] = 1.0;
+        lu6_solve(LU, piv, x);
+        const double estold = est;
+        est = 0.0;
+        for (int i = 0; i < 6; i++) est += fabs(x[i]);
+        bool same = true;
+        for (int i = 0; i < 6; i++) if ((x[i] >= 0.0 ? 1.0 : -1.0) != xs[i]) { same = false; break; }
+        if (same || est <= estold) break;
+        for (int i = 0; i < 6; i++) { xs[i] = x[i] >= 0.0 ? 1.0 : -1.0; x[i] = xs[i]; }
+        lu6_solve_t(LU, piv, x);
+        const int jlast = j;
+        j = 0;
+        for (int i = 1; i < 6; i++) if (fabs(x[i]) > fabs(x[j])) j = i;
+        if (x[jlast] != fabs(x[j]) && iter < 5) { iter++; continue; }
+        break;
+    }
+    double alt = 1.0;
+    for (int i = 0; i < 6; i++) { x[i] = alt * (1.0 + (double)i / 5.0); alt = -alt; }
+    lu6_solve(LU, piv, x);
+    double temp = 0.0;
+    for (int i = 0; i < 6; i++) temp += fabs(x[i]);
+    temp = 2.0 * (temp / 18.0);
+    if (temp > est) est = temp;
+    const double rcond = (1.0 / est) / anorm;
+    if (!(rcond > 0.0)) return HUGEV;
+    return 1.0 / rcond;
+}
+
+__global__ __launch_bounds__(BS) void k_diagnostics(Fields f, double lfloor, int scan_cond) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    double sm[6] = {0, 0, 0, 0, 0, 0};
+    double mx[12];
+    for (int i = 0; i < 12; i++) mx[i] = 0.0;
+    double mnall[1] = {1.7976931348623157e308};
+    double js = 1.7976931348623157e308, jg = 1.7976931348623157e308;
+    long is_ = -1, ig_ = -1;
+    if (v < V) {
+        double d2 = 0.0, l0 = 0.0;
+        for (int c = 0; c < 9; c++) {
+            const double a = f.L[c * V + v], b = f.s_L[c * V + v];
+            d2 += (a - b) * (a - b);
+            l0 += b * b;
+        }
+        const double rel = sqrt(d2) / fmax(sqrt(l0), lfloor);
+        const double w = f.w[v], J = f.J[v];
+        const bool gas = f.gas[v] != 0;
+        sm[0] = rel * rel * w; sm[1] = w;
+        mx[0] = rel;
+        mnall[0] = J;
+        if (gas) { sm[4] = rel * rel * w; sm[5] = w; mx[2] = rel; jg = J; ig_ = v; }
+        else { sm[2] = rel * rel * w; sm[3] = w; mx[1] = rel; js = J; is_ = v; }
+        double Fn[9], vol, rot, str;
+        for (int c = 0; c < 9; c++) Fn[c] = f.Finc[c * V + v];
+        kin_voxel(Fn, &vol, &rot, &str);
+        mx[gas ? 4 : 3] = vol;
+        mx[gas ? 6 : 5] = str;
+        mx[gas ? 8 : 7] = rot;
+        if (scan_cond) {
+            double A[36];
+            for (int i = 0; i < 36; i++) A[i] = f.c66m[(long)i * V + v];
+            const double cA = cond1_est6(A);
+            if (gas) {
+                mx[10] = cA;
+                for (int i = 0; i < 36; i++) A[i] = f.cgas[(long)i * V + v];
+                mx[11] = cond1_est6(A);
+            } else {
+                mx[9] = cA;
+            }
+        }
+    }
+    const int nb = gridDim.x;
+    double* part = f.part;
+    block_reduce_store<6, BS>(sm, part);
+    __syncthreads();
+    block_reduce_store<12, BS, OpMax>(mx, part + 6 * (size_t)nb);
+    __syncthreads();
+    block_reduce_store<1, BS, OpMin>(mnall, part + 18 * (size_t)nb);
+    // arg-min of detF for solid and gas voxels: smallest value, ties -> larger index
+    __shared__ double sv[2][BS];
+    __shared__ long si[2][BS];
+    __syncthreads();
+    sv[0][threadIdx.x] = js; si[0][threadIdx.x] = is_;
+    sv[1][threadIdx.x] = jg; si[1][threadIdx.x] = ig_;
+    __syncthreads();
+    for (int o = BS / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            for (int k = 0; k < 2; k++) {
+                const double a = sv[k][threadIdx.x], b = sv[k][threadIdx.x + o];
+                const long ia = si[k][threadIdx.x], ib = si[k][threadIdx.x + o];
+                if (b < a || (b == a && ib > ia)) { sv[k][threadIdx.x] = b; si[k][threadIdx.x] = ib; }
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[19 * (size_t)nb + 4 * (size_t)blockIdx.x + 0] = sv[0][0];
+        part[19 * (size_t)nb + 4 * (size_t)blockIdx.x + 1] = (double)si[0][0];
+        part[19 * (size_t)nb + 4 * (size_t)blockIdx.x + 2] = sv[1][0];
+        part[19 * (size_t)nb + 4 * (size_t)blockIdx.x + 3] = (double)si[1][0];
+    }
+}
+__global__ void k_argmin_final(const double* p, int nb, double* out) {
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 2; k++) {
+        double best = 1.7976931348623157e308;
+        double bi = -1.0;
+        for (int b = 0; b < nb; b++) {
+            const double val = p[4 * b + 2 * k], idx = p[4 * b + 2 * k + 1];
+            if (idx < 0) continue;
+            if (val < best || (val == best && idx > bi)) { best = val; bi = idx; }
+        }
+        out[2 * k] = best;
+        out[2 * k + 1] = bi;
+    }
+}
+void launch_diagnostics(const Fields& f, double lfloor, int scan_cond, hipStream_t s, int* nb) {
+    *nb = nblk(f.V);
+    const int n = *nb;
+    hipLaunchKernelGGL(k_diagnostics, dim3(n), dim3(BS), 0, s, f, lfloor, scan_cond);
+    hipLaunchKernelGGL(k_reduce_final, dim3(6), dim3(BS), 0, s, f.part, n, 6, f.red);
+    hipLaunchKernelGGL(k_reduce_final_op<OpMax>, dim3(12), dim3(BS), 0, s, f.part + 6 * (size_t)n, n, 12, f.red + 6);
+    hipLaunchKernelGGL(k_reduce_final_op<OpMin>, dim3(1), dim3(BS), 0, s, f.part + 18 * (size_t)n, n, 1, f.red + 18);
+    hipLaunchKernelGGL(k_argmin_final, dim3(1), dim3(64), 0, s, f.part + 19 * (size_t)n, n, f.red + 19);
+}
+
+// take_snapshot / rollback (:2839-2895)
+void launch_snapshot(const Fields& f, int restore, hipStream_t s) {
+    const size_t V = f.V, ns = f.ns;
+    struct { double* live; double* snap; size_t n; } T[] = {
+        {f.L, f.s_L, 9 * V}, {f.Lold, f.s_Lold, 9 * V}, {f.sg, f.s_sg, 6 * V},
+        {f.edotp, f.s_edotp, 6 * V}, {f.gdot, f.s_gdot, ns * V}, {f.ttau, f.s_ttau, 2 * ns * V}};
+    for (auto& t : T) {
+        if (restore) hipMemcpyAsync(t.live, t.snap, t.n * sizeof(double), hipMemcpyDeviceToDevice, s);
+        else hipMemcpyAsync(t.snap, t.live, t.n * sizeof(double), hipMemcpyDeviceToDevice, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// initial voxel state: data_grain (IO_functions.f90:759-809), Hall-Petch
+// strengths (:100-199), initial_guess (various_functions.f90:8-53) and the
+// 36 sums of w cg66 over solid voxels for the initial C0 (LS-EVPFFT.f90:156-166)
+// ---------------------------------------------------------------------------
+struct InitArgs { double udot[9]; double tdot; };
+__global__ __launch_bounds__(BS) void k_init_voxels(Fields f, PhaseTables pt, const double* __restrict__ euler,
+                                                    const int* __restrict__ grain, const double* __restrict__ hp,
+                                                    InitArgs a) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    double acc[36];
+    for (int i = 0; i < 36; i++) acc[i] = 0.0;
+    if (v < V) {
+        const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        const double wg = 1.0 / (double)V;
+        f.w[v] = wg;
+        f.J[v] = 1.0;
+        for (int c = 0; c < 9; c++) {
+            f.F[c * V + v] = I3[c]; f.Fi[c * V + v] = I3[c]; f.Fe[c * V + v] = I3[c]; f.Fp[c * V + v] = I3[c];
+            f.Finc[c * V + v] = I3[c]; f.Feinc[c * V + v] = I3[c];
+            f.L[c * V + v] = a.udot[c];
+            f.Lold[c * V + v] = 0.0;
+            f.dLref[c * V + v] = 0.0;
+        }
+        for (int c = 0; c < 6; c++) {
+            f.sg[c * V + v] = 0.0; f.sgt[c * V + v] = 0.0; f.edotp[c * V + v] = 0.0;
+            f.ept[c * V + v] = 0.0; f.dgs[c * V + v] = 0.0;
+        }
+        f.epvm[v] = 0.0;
+        f.gacc[v] = 0.0;
+        for (int k = 0; k < f.ns; k++) f.gdot[(long)k * V + v] = 0.0;
+        for (int k = 0; k < 2 * f.ns; k++) { f.crss[(long)k * V + v] = 0.0; f.ttau[(long)k * V + v] = 0.0; }
+        for (int k = 0; k < 5 * f.ns; k++) f.sch[(long)k * V + v] = 0.0;
+        if (!f.gas[v]) {
+            const PhaseDev& P = pt.ph[f.phase[v]];
+            // Bunge angles -> a (sample -> crystal), tensor_functions.f90:525-569; ag = a^T
+            const double d2r = PI_D / 180.0;
+            const double ph = euler[3 * v] * d2r, th = euler[3 * v + 1] * d2r, tm = euler[3 * v + 2] * d2r;
+            const double sph = sin(ph), cph = cos(ph), sth = sin(th), cth = cos(th), stm = sin(tm), ctm = cos(tm);
+            double A[9];
+            A[0] = ctm * cph - sph * stm * cth;  A[1] = ctm * sph + cph * stm * cth;  A[2] = sth * stm;
+            A[3] = -stm * cph - sph * ctm * cth; A[4] = -sph * stm + cph * ctm * cth; A[5] = ctm * sth;
+            A[6] = sph * sth;                    A[7] = -sth * cph;                   A[8] = cth;
+            double ag[9];
+            transp3(A, ag);
+            for (int c = 0; c < 9; c++) f.ag[c * V + v] = ag[c];
+            double T[36], c66[36], s66[36];
+            rot_matrix(ag, T);
+            congr6(T, P.c66cr, c66);
+            for (int i = 0; i < 36; i++) { s66[i] = c66[i]; acc[i] = c66[i] * wg; }
+            inv6(s66);
+            for (int i = 0; i < 36; i++) f.sg66[(long)i * V + v] = s66[i];
+            // initial_guess: sigma = cg66 (udot6 tdot)
+            double u6[6], s6[6];
+            to6(a.udot, u6);
+            for (int i = 0; i < 6; i++) {
+                double s = 0.0;
+                for (int k = 0; k < 6; k++) s += c66[6 * i + k] * (u6[k] * a.tdot);
+                s6[i] = s;
+            }
+            for (int c = 0; c < 6; c++) f.sg[c * V + v] = s6[c];
+            // Hall-Petch + directional term; trialtau(:,1) = crss(:,2)
+            const double* tau = pt.data + P.off_tau;
+            const double* hpf = pt.data + P.off_hpfac;
+            const double* db = pt.data + P.off_dbca;
+            const double hps = hp[grain[v]];
+            for (int is = 0; is < P.nsyst; is++) {
+                const double by = ag[3] * db[3 * is] + ag[4] * db[3 * is + 1] + ag[5] * db[3 * is + 2];
+                const double dd = hpf[is] * sqrt(fabs(by));
+                const double c1 = tau[3 * is] + hps + dd, c2 = tau[3 * is + 1] + hps + dd;
+                f.crss[(2 * is) * V + v] = c1;
+                f.crss[(2 * is + 1) * V + v] = c2;
+                f.ttau[(2 * is) * V + v] = c2;
+                f.ttau[(2 * is + 1) * V + v] = c2;
+            }
+        } else {
+            for (int c = 0; c < 9; c++) f.ag[c * V + v] = 0.0;
+            for (int i = 0; i < 36; i++) f.sg66[(long)i * V + v] = 0.0;
+        }
+    }
+    block_reduce_store<36, BS>(acc, f.part);
+}
+void launch_init_voxels(const Fields& f, const PhaseTables& pt, const double* euler, const int* grain,
+                        const double* hp_grain, const double* udot, double tdot, hipStream_t s, int* nb) {
+    InitArgs a;
+    for (int i = 0; i < 9; i++) a.udot[i] = udot[i];
+    a.tdot = tdot;
+    *nb = nblk(f.V);
+    hipLaunchKernelGGL(k_init_voxels, dim3(*nb), dim3(BS), 0, s, f, pt, euler, grain, hp_grain, a);
+}
+void launch_copy(double* dst, const double* src, size_t n, hipStream_t s) {
+    hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, s);
+}
+
+}  // namespace cg

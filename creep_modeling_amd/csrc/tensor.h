@@ -1,0 +1,350 @@
+// tensor.h -- small dense tensor algebra shared by the HIP kernels and the
+// library's host-side macroscopic updates (FP64, __host__ __device__).
+//
+// Conventions: 3x3 tensors row-major a[3*i+j]; symmetric tensors as 6-vectors
+// in the orthonormal b-basis of chg_basis (tensor_functions.f90:21-198):
+//   b1=(e22-e11)/sqrt2, b2=(2e33-e11-e22)/sqrt6, b3=(e23+e32)/sqrt2,
+//   b4=(e13+e31)/sqrt2, b5=(e12+e21)/sqrt2, b6=I/sqrt3.
+// 6x6 operators act on b-basis vectors, row-major.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#define CG_HD __host__ __device__ __forceinline__
+
+namespace cg {
+
+constexpr double R2 = 0.70710678118654752440;   // 1/sqrt2
+constexpr double R3 = 0.57735026918962576451;   // 1/sqrt3
+constexpr double R6 = 0.40824829046386301637;   // 1/sqrt6
+constexpr double R23 = 0.81649658092772603273;  // 2/sqrt6
+constexpr double PI_D = 3.14159265358979323846;
+
+// ---- b-basis <-> 3x3 ------------------------------------------------------
+// 6-vector of the symmetric part of a general 3x3 tensor.
+CG_HD void to6(const double* a, double* s) {
+    s[0] = R2 * (a[4] - a[0]);
+    s[1] = R23 * a[8] - R6 * (a[0] + a[4]);
+    s[2] = R2 * (a[5] + a[7]);
+    s[3] = R2 * (a[2] + a[6]);
+    s[4] = R2 * (a[1] + a[3]);
+    s[5] = R3 * (a[0] + a[4] + a[8]);
+}
+// symmetric 3x3 from a 6-vector (kdim=6) or 5-vector (s[5] taken as 0).
+CG_HD void from6(const double* s, double* a, bool dev = false) {
+    const double h = dev ? 0.0 : R3 * s[5];
+    a[0] = h - R2 * s[0] - R6 * s[1];
+    a[4] = h + R2 * s[0] - R6 * s[1];
+    a[8] = h + R23 * s[1];
+    a[5] = a[7] = R2 * s[2];
+    a[2] = a[6] = R2 * s[3];
+    a[1] = a[3] = R2 * s[4];
+}
+// the six basis tensors as 3x3 (for contractions built from them)
+CG_HD void basis(int a, double* b) {
+    for (int i = 0; i < 9; i++) b[i] = 0.0;
+    switch (a) {
+        case 0: b[0] = -R2; b[4] = R2; break;
+        case 1: b[0] = -R6; b[4] = -R6; b[8] = R23; break;
+        case 2: b[5] = b[7] = R2; break;
+        case 3: b[2] = b[6] = R2; break;
+        case 4: b[1] = b[3] = R2; break;
+        default: b[0] = b[4] = b[8] = R3; break;
+    }
+}
+
+// ---- 3x3 ---------------------------------------------------------------------
+CG_HD void mm3(const double* a, const double* b, double* c) {
+    double t[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) t[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+#pragma unroll
+    for (int i = 0; i < 9; i++) c[i] = t[i];
+}
+// c = a * b^T
+CG_HD void mmt3(const double* a, const double* b, double* c) {
+    double t[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) t[3 * i + j] = a[3 * i] * b[3 * j] + a[3 * i + 1] * b[3 * j + 1] + a[3 * i + 2] * b[3 * j + 2];
+#pragma unroll
+    for (int i = 0; i < 9; i++) c[i] = t[i];
+}
+CG_HD void transp3(const double* a, double* b) {
+    double t[9] = {a[0], a[3], a[6], a[1], a[4], a[7], a[2], a[5], a[8]};
+#pragma unroll
+    for (int i = 0; i < 9; i++) b[i] = t[i];
+}
+CG_HD double det3(const double* m) {
+    return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) + m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+// inverse by the adjugate; returns the determinant (0 -> singular)
+CG_HD double inv3(const double* m, double* r) {
+    double c0 = m[4] * m[8] - m[5] * m[7], c1 = m[5] * m[6] - m[3] * m[8], c2 = m[3] * m[7] - m[4] * m[6];
+    double d = m[0] * c0 + m[1] * c1 + m[2] * c2;
+    double id = 1.0 / d;
+    double t[9];
+    t[0] = c0 * id; t[1] = (m[2] * m[7] - m[1] * m[8]) * id; t[2] = (m[1] * m[5] - m[2] * m[4]) * id;
+    t[3] = c1 * id; t[4] = (m[0] * m[8] - m[2] * m[6]) * id; t[5] = (m[2] * m[3] - m[0] * m[5]) * id;
+    t[6] = c2 * id; t[7] = (m[1] * m[6] - m[0] * m[7]) * id; t[8] = (m[0] * m[4] - m[1] * m[3]) * id;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r[i] = t[i];
+    return d;
+}
+// von Mises equivalents of a general tensor (tensor_functions.f90:266-289)
+CG_HD double vm_sq(const double* d) {
+    double h = (d[0] + d[4] + d[8]) * (1.0 / 3.0);
+    return (d[0] - h) * (d[0] - h) + (d[4] - h) * (d[4] - h) + (d[8] - h) * (d[8] - h) + d[1] * d[1] + d[2] * d[2] +
+           d[5] * d[5] + d[3] * d[3] + d[6] * d[6] + d[7] * d[7];
+}
+CG_HD double vm_strain(const double* d) { return sqrt(vm_sq(d) * (2.0 / 3.0)); }
+CG_HD double vm_stress(const double* d) { return sqrt(vm_sq(d) * 1.5); }
+// von Mises strain of a symmetric tensor given in the b-basis: the deviatoric
+// 5-vector norm (the b-basis is orthonormal): sqrt(2/3 |dev|^2)
+CG_HD double vm_strain6(const double* s) {
+    return sqrt((2.0 / 3.0) * (s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + s[3] * s[3] + s[4] * s[4]));
+}
+
+// ---- 6x6 dense ---------------------------------------------------------------
+// Gaussian elimination with partial pivoting; solves A x = b in place (A destroyed).
+// Returns false for an exactly singular pivot.
+CG_HD bool solve6(double* A, double* b) {
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        int p = k;
+        double mx = fabs(A[6 * k + k]);
+#pragma unroll
+        for (int i = k + 1; i < 6; i++) {
+            double v = fabs(A[6 * i + k]);
+            if (v > mx) { mx = v; p = i; }
+        }
+        if (mx == 0.0) ok = false;
+        if (p != k) {
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                double t = A[6 * k + j];
+                A[6 * k + j] = A[6 * p + j];
+                A[6 * p + j] = t;
+            }
+            double t = b[k]; b[k] = b[p]; b[p] = t;
+        }
+        double r = 1.0 / A[6 * k + k];
+#pragma unroll
+        for (int i = k + 1; i < 6; i++) {
+            double l = A[6 * i + k] * r;
+#pragma unroll
+            for (int j = k + 1; j < 6; j++) A[6 * i + j] -= l * A[6 * k + j];
+            b[i] -= l * b[k];
+        }
+    }
+#pragma unroll
+    for (int i = 5; i >= 0; i--) {
+        double s = b[i];
+#pragma unroll
+        for (int j = i + 1; j < 6; j++) s -= A[6 * i + j] * b[j];
+        b[i] = s / A[6 * i + i];
+    }
+    return ok;
+}
+// LU factorisation with partial pivoting of a 6x6 (in place; unit lower L)
+CG_HD bool lu6(double* A, int* piv) {
+    bool ok = true;
+    for (int k = 0; k < 6; k++) {
+        int p = k;
+        double mx = fabs(A[6 * k + k]);
+        for (int i = k + 1; i < 6; i++) {
+            double v = fabs(A[6 * i + k]);
+            if (v > mx) { mx = v; p = i; }
+        }
+        piv[k] = p;
+        if (mx == 0.0) { ok = false; continue; }
+        if (p != k)
+            for (int j = 0; j < 6; j++) { double t = A[6 * k + j]; A[6 * k + j] = A[6 * p + j]; A[6 * p + j] = t; }
+        double r = 1.0 / A[6 * k + k];
+        for (int i = k + 1; i < 6; i++) {
+            double l = A[6 * i + k] * r;
+            A[6 * i + k] = l;
+            for (int j = k + 1; j < 6; j++) A[6 * i + j] -= l * A[6 * k + j];
+        }
+    }
+    return ok;
+}
+CG_HD void lu6_solve(const double* LU, const int* piv, double* b) {
+    for (int k = 0; k < 6; k++) { int p = piv[k]; if (p != k) { double t = b[k]; b[k] = b[p]; b[p] = t; } }
+    for (int i = 1; i < 6; i++) { double s = b[i]; for (int j = 0; j < i; j++) s -= LU[6 * i + j] * b[j]; b[i] = s; }
+    for (int i = 5; i >= 0; i--) { double s = b[i]; for (int j = i + 1; j < 6; j++) s -= LU[6 * i + j] * b[j]; b[i] = s / LU[6 * i + i]; }
+}
+// solve A^T x = b given the LU of A
+CG_HD void lu6_solve_t(const double* LU, const int* piv, double* b) {
+    for (int i = 0; i < 6; i++) { double s = b[i]; for (int j = 0; j < i; j++) s -= LU[6 * j + i] * b[j]; b[i] = s / LU[6 * i + i]; }
+    for (int i = 4; i >= 0; i--) { double s = b[i]; for (int j = i + 1; j < 6; j++) s -= LU[6 * j + i] * b[j]; b[i] = s; }
+    for (int k = 5; k >= 0; k--) { int p = piv[k]; if (p != k) { double t = b[k]; b[k] = b[p]; b[p] = t; } }
+}
+// inverse of a 6x6 through its LU factors (in place)
+CG_HD bool inv6(double* A) {
+    double LU[36];
+    int piv[6];
+    for (int i = 0; i < 36; i++) LU[i] = A[i];
+    bool ok = lu6(LU, piv);
+    for (int j = 0; j < 6; j++) {
+        double e[6] = {0, 0, 0, 0, 0, 0};
+        e[j] = 1.0;
+        lu6_solve(LU, piv, e);
+        for (int i = 0; i < 6; i++) A[6 * i + j] = e[i];
+    }
+    return ok;
+}
+CG_HD void mm6(const double* a, const double* b, double* c) {
+    double t[36];
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            double s = 0.0;
+            for (int k = 0; k < 6; k++) s += a[6 * i + k] * b[6 * k + j];
+            t[6 * i + j] = s;
+        }
+    for (int i = 0; i < 36; i++) c[i] = t[i];
+}
+// c = m a m^T
+CG_HD void congr6(const double* m, const double* a, double* c) {
+    double t[36];
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            double s = 0.0;
+            for (int k = 0; k < 6; k++) s += m[6 * i + k] * a[6 * k + j];
+            t[6 * i + j] = s;
+        }
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            double s = 0.0;
+            for (int k = 0; k < 6; k++) s += t[6 * i + k] * m[6 * j + k];
+            c[6 * i + j] = s;
+        }
+}
+// M(X)_{ca} = b_c : (b_a X^T): the b-basis matrix of the map S -> sym(S X^T)
+// used for the push-forwards  c0mod = sum_ab c_ab (b_a F^T)(x)(b_b F^T)/J.
+CG_HD void push_matrix(const double* X, double* M) {
+    double ba[9], y[9];
+    for (int a = 0; a < 6; a++) {
+        basis(a, ba);
+        mmt3(ba, X, y);  // b_a X^T
+        double ys[6];
+        to6(y, ys);      // b_c : y for symmetric b_c equals b_c : sym(y)
+        for (int c = 0; c < 6; c++) M[6 * c + a] = ys[c];
+    }
+}
+// T(A)_{ca} = b_c : (A b_a A^T): the b-basis matrix of S -> A S A^T
+CG_HD void rot_matrix(const double* A, double* T) {
+    double ba[9], y[9];
+    for (int a = 0; a < 6; a++) {
+        basis(a, ba);
+        mm3(A, ba, y);
+        mmt3(y, A, y);
+        double ys[6];
+        to6(y, ys);
+        for (int c = 0; c < 6; c++) T[6 * c + a] = ys[c];
+    }
+}
+
+// ---- matrix exponential (various_functions.f90:1830-1967) --------------------
+CG_HD double l1n3(const double* m) {
+    double a = fabs(m[0]) + fabs(m[3]) + fabs(m[6]);
+    double b = fabs(m[1]) + fabs(m[4]) + fabs(m[7]);
+    double c = fabs(m[2]) + fabs(m[5]) + fabs(m[8]);
+    return fmax(a, fmax(b, c));
+}
+CG_HD void expm3(const double* A0, double* E) {
+    double n1 = l1n3(A0);
+    if (n1 <= 0.1) {  // cubic Taylor
+        double a2[9], a3[9];
+        mm3(A0, A0, a2);
+        mm3(a2, A0, a3);
+        for (int i = 0; i < 9; i++) E[i] = (i % 4 == 0 ? 1.0 : 0.0) + A0[i] + a2[i] * 0.5 + a3[i] * (1.0 / 6.0);
+        return;
+    }
+    // Pade [13/13] with scaling and squaring (Higham 2005 coefficients)
+    const double b[14] = {64764752532480000.0, 32382376266240000.0, 7771770303897600.0, 1187353796428800.0,
+                          129060195264000.0,   10559470521600.0,    670442572800.0,     33522128640.0,
+                          1323241920.0,        40840800.0,          960960.0,           16380.0,
+                          182.0,               1.0};
+    const double theta13 = 5.371920351148152;
+    int s = 0;
+    if (n1 > theta13) s = (int)ceil(log2(n1 / theta13));
+    double sc = ldexp(1.0, -s);
+    double A[9], A2[9], A4[9], A6[9], U[9], V[9], T[9];
+    for (int i = 0; i < 9; i++) A[i] = A0[i] * sc;
+    mm3(A, A, A2);
+    mm3(A2, A2, A4);
+    mm3(A2, A4, A6);
+    for (int i = 0; i < 9; i++) T[i] = b[13] * A6[i] + b[11] * A4[i] + b[9] * A2[i];
+    mm3(A6, T, T);
+    for (int i = 0; i < 9; i++) T[i] += b[7] * A6[i] + b[5] * A4[i] + b[3] * A2[i] + (i % 4 == 0 ? b[1] : 0.0);
+    mm3(A, T, U);
+    for (int i = 0; i < 9; i++) T[i] = b[12] * A6[i] + b[10] * A4[i] + b[8] * A2[i];
+    mm3(A6, T, V);
+    for (int i = 0; i < 9; i++) V[i] += b[6] * A6[i] + b[4] * A4[i] + b[2] * A2[i] + (i % 4 == 0 ? b[0] : 0.0);
+    double P[9], Q[9], Qi[9];
+    for (int i = 0; i < 9; i++) { P[i] = V[i] + U[i]; Q[i] = V[i] - U[i]; }
+    inv3(Q, Qi);
+    mm3(Qi, P, E);
+    for (int k = 0; k < s; k++) mm3(E, E, E);
+}
+
+// ---- symmetric 3x3 eigenvalues / polar decomposition -------------------------
+// eigen-decomposition of a symmetric 3x3 by cyclic Jacobi rotations
+CG_HD void eig3(const double* S, double* w, double* Q) {
+    double a[9];
+    for (int i = 0; i < 9; i++) { a[i] = S[i]; Q[i] = (i % 4 == 0) ? 1.0 : 0.0; }
+    for (int sweep = 0; sweep < 32; sweep++) {
+        double off = a[1] * a[1] + a[2] * a[2] + a[5] * a[5];
+        double dg = a[0] * a[0] + a[4] * a[4] + a[8] * a[8];
+        if (!(off > 1e-36 * dg)) break;
+        for (int p = 0; p < 2; p++)
+            for (int r = p + 1; r < 3; r++) {
+                double apr = a[3 * p + r];
+                if (apr == 0.0) continue;
+                double th = (a[3 * r + r] - a[3 * p + p]) / (2.0 * apr);
+                double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < 3; k++) {
+                    double akp = a[3 * k + p], akr = a[3 * k + r];
+                    a[3 * k + p] = c * akp - s * akr;
+                    a[3 * k + r] = s * akp + c * akr;
+                }
+                for (int k = 0; k < 3; k++) {
+                    double apk = a[3 * p + k], ark = a[3 * r + k];
+                    a[3 * p + k] = c * apk - s * ark;
+                    a[3 * r + k] = s * apk + c * ark;
+                }
+                for (int k = 0; k < 3; k++) {
+                    double qkp = Q[3 * k + p], qkr = Q[3 * k + r];
+                    Q[3 * k + p] = c * qkp - s * qkr;
+                    Q[3 * k + r] = s * qkp + c * qkr;
+                }
+            }
+    }
+    w[0] = a[0]; w[1] = a[4]; w[2] = a[8];
+}
+// F = R U, U = sqrt(F^T F); also returns the principal stretches (eigenvalues of U)
+CG_HD void polar3(const double* F, double* R, double* lam) {
+    double C[9], w[3], Q[9];
+    double Ft[9];
+    transp3(F, Ft);
+    mm3(Ft, F, C);
+    eig3(C, w, Q);
+    double is[3];
+    for (int i = 0; i < 3; i++) {
+        lam[i] = sqrt(fmax(w[i], 0.0));
+        is[i] = 1.0 / lam[i];
+    }
+    double Ui[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            Ui[3 * i + j] = Q[3 * i] * is[0] * Q[3 * j] + Q[3 * i + 1] * is[1] * Q[3 * j + 1] + Q[3 * i + 2] * is[2] * Q[3 * j + 2];
+    mm3(F, Ui, R);
+}
+
+}  // namespace cg

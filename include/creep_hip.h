@@ -1,0 +1,158 @@
+/*
+ * creep_hip.h -- C-ABI of libcreep_hip.so, the MI355X implementation of the
+ * LS-EVPFFT per-increment fixed-point loop.
+ *
+ * The reference has no C interface on this path: its "operator API" is the set
+ * of Fortran module procedures `subroutine X(micro, props)` in
+ * lsevpfft/src/various_functions.f90 called from the increment loop of
+ * lsevpfft/src/LS-EVPFFT.f90:196-589, with state in the derived types of
+ * lsevpfft/src/types.f90:5-172 and iteration globals in global.f90:19-27.
+ * Each entry point below names the reference procedures it replaces.  Plain
+ * pointers and sizes only; a Fortran host binds it with ISO_C_BINDING
+ * (see creep_modeling_amd/host/creep_hip_iface.f90).
+ *
+ * Conventions
+ *   - voxel index v = i + n1*(j + n2*k), i.e. micro%voxel(i,j,k) with i fastest;
+ *   - 3x3 tensors row-major (a[3*r+c] = A(r+1,c+1)); 4th-order tensors
+ *     c[27*i+9*j+3*k+l]; 6x6 matrices in the orthonormal b-basis of chg_basis
+ *     (tensor_functions.f90:21-198), row-major;
+ *   - return codes: 0 OK, <0 HIP/runtime/argument error, >0 numerical failure
+ *     (1 singular LU as the reference's `stop 1`, 2 non-finite errors);
+ *     cg_last_error() gives the message;
+ *   - one host thread per context; every call is synchronous with respect to
+ *     the values it returns; contexts are independent.
+ */
+#ifndef CREEP_HIP_H
+#define CREEP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cg_ctx cg_ctx;
+
+/* One process block of fft.in after load_input's post-processing
+ * (IO_functions.f90:202-345; types.f90:120-152).  dsim and scauchy are
+ * updated by cg_iterate (state_6x6_evpsc writes them, various_functions.f90:1601). */
+typedef struct {
+    int iudot[9];
+    double udot[9];
+    int idsim[6], iscau[6];
+    double scauchy[9], dsim[9];
+    double tdot, tdotref, tdotmin, devmmx, devmapp;
+    int ictrl, nsteps;
+    double error, erroral;
+    int itmax, iupdate, iuphard, iwfields, iwstep;
+    double xc0;
+    int igamma, ithermo;
+} cg_process;
+
+/* Iteration state carried by the caller between cg_iterate calls
+ * (the driver locals `iter` and `mix` of LS-EVPFFT.f90:15, 218-233). */
+typedef struct {
+    int iter;      /* 0 at the start of an increment or after a rollback */
+    double mix;    /* relaxation factor applied at the next pass           */
+} cg_iter_state;
+
+/* Results of one fixed-point pass (LS-EVPFFT.f90:238-358). */
+typedef struct {
+    double errs, erre, errsbc;        /* normalised errors (:341-343)      */
+    double velgradavg[9];             /* micro%velgradavg                  */
+    double sgavg[9];                  /* micro%sgavg (solid, unnormalised) */
+    double sgavgsol[9];
+    double dvelgradavg[9];
+    double svmavg, dL_iter;
+    int64_t nr_steps, evals, outer;   /* constitutive work counters        */
+} cg_iter_out;
+
+/* Macroscopic quantities after an accepted increment (LS-EVPFFT.f90:460-523,
+ * results.csv and diagnostics.csv columns). */
+typedef struct {
+    double eavg[9], sgsolavg[9], epavg[9], eelavg[9], edotpavg[9], velgradavg[9], sgavg[9];
+    double evmavg, evmpavg, dvmavg, dvmpavg, svmavg, lvmavg, edotvmmx;
+    double gammaavg, gavg, geffavg;
+    /* update_diagnostics (types.f90:345-758) */
+    double detF_min, detF_min_sol, detF_min_gas;
+    double volinc_max_sol, volinc_max_gas, stretchinc_max_sol, stretchinc_max_gas;
+    double rotang_max_sol, rotang_max_gas;
+    double dL_step_rms_sol, dL_step_rms_gas, dL_step_rms, dL_step_max_sol, dL_step_max_gas, dL_step_max;
+    double dvelgradavg_mag, cond_c066mod_max_sol, cond_c066mod_max_gas, cond_cgas_max;
+    int imin_sol[3], imin_gas[3];
+} cg_incr_out;
+
+/* Replaces init_micro (types.f90:256-343) and allocation of the FFTW buffers
+ * (fourier_functions.f90:69-196).  precision: 0 = FP64 form, 1 = mixed form
+ * (FFT + Green operator in FP32).  modal/do_mix: the --modal/--mix flags
+ * (LS-EVPFFT.f90:114-126). */
+int cg_create(int n1, int n2, int n3, int nph, double complgas, int modal, int do_mix, int precision,
+              cg_ctx **out);
+int cg_destroy(cg_ctx *ctx);
+const char *cg_last_error(cg_ctx *ctx);
+
+/* Phase tables after data_crystal / data_crystal_elast (IO_functions.f90:367-719):
+ * ns systems; dnca/dbca (ns,3) unit vectors, tau (ns,3) = tau0f,tau0b,tau1,
+ * thet (ns,2), nrs, gamd0, hpfac (ns), hard (ns,ns), C (81) crystal stiffness. */
+int cg_set_phase(cg_ctx *ctx, int iph, int igas, int nsyst, int nmodes, const int *nsm, const double *dnca,
+                 const double *dbca, const double *tau, const double *thet, const double *nrs,
+                 const double *gamd0, const double *hpfac, const double *hard, const double *C);
+/* Microstructure (data_grain, IO_functions.f90:759-809): euler (V,3) Bunge degrees,
+ * grain and phase ids (1-based). */
+int cg_set_voxels(cg_ctx *ctx, const double *euler, const int *grain, const int *phase);
+/* Rotations, elastic stiffness, Hall-Petch strengths (IO_functions.f90:100-199,
+ * 347-363), initial_guess (various_functions.f90:8-53) and the initial C0
+ * (LS-EVPFFT.f90:156-173) for the first process p1. */
+int cg_init(cg_ctx *ctx, const cg_process *p1);
+
+/* take_snapshot (various_functions.f90:2839), update_schmid (:472) if
+ * do_schmid, form_G (:609; Gamma is evaluated on the fly in the FFT stage and
+ * Goperr0 by a reduction) and calc_c066mod_Goperr066mod (:812). */
+int cg_begin_increment(cg_ctx *ctx, const cg_process *p, int do_schmid);
+/* One pass of the fixed-point loop (LS-EVPFFT.f90:222-358: <sigma>, PK1,
+ * convolution_with_Goper, correction_to_current, dL_iter + mixing, solve_res,
+ * calc_velgradavg_corr, error normalisation).  Updates st->iter, st->mix and
+ * p->dsim / p->scauchy. */
+int cg_iterate(cg_ctx *ctx, cg_process *p, cg_iter_state *st, cg_iter_out *out);
+/* rollback (various_functions.f90:2868-2895), restoring p->dsim / p->scauchy. */
+int cg_rollback(cg_ctx *ctx, cg_process *p);
+/* kinematic_increments + kinematic_substep (various_functions.f90:2745-2837):
+ * out[4] = max volinc, max rotang (deg), max stretchinc, min detF_next; returns nsub in *nsub. */
+int cg_kinematic_substep(cg_ctx *ctx, double tdot, double *out4, int *nsub);
+/* Accepted-increment updates (LS-EVPFFT.f90:431-523 and types.f90:345-758):
+ * accumulation, update_plastic_defgrad, update_grid_velgrad_node, update_defgrad,
+ * update_elstic_defgrad, update_el_stiff, update_tensor_config (iupdate=1),
+ * update_hardening (iuphard=1), macro averages, update_diagnostics. */
+int cg_end_increment(cg_ctx *ctx, const cg_process *p, int scan_diagnostics, cg_incr_out *out);
+/* calc_c0 (various_functions.f90:2537-2649) with the process' current tdot/tdotref/xc0. */
+int cg_calc_c0(cg_ctx *ctx, const cg_process *p);
+
+/* Field access in voxel-major order (v*ncomp + comp), symmetric tensors as full 3x3.
+ * Names: sg sgt velgrad velgradold dvelgradref defgrad defgradinv detF wgtc
+ * defgrade defgradp ag sg66 c066mod cgm cgas sch crss trialtau gamdot gacumgr
+ * edotp ept disgradsym epvm defgradinc defgradeinc xnode.
+ * cg_field_ncomp returns the component count (-1 unknown). */
+int cg_field_ncomp(cg_ctx *ctx, const char *name);
+int cg_get_field(cg_ctx *ctx, const char *name, double *out, size_t n);
+int cg_set_field(cg_ctx *ctx, const char *name, const double *in, size_t n);
+/* Macroscopic operators: c0[81], s0[81], de_ds[81], c066[36], goperr0[81] (any may be NULL). */
+int cg_get_macro(cg_ctx *ctx, double *c0, double *s0, double *de_ds, double *c066, double *goperr0);
+
+/* Per-stage device time (ms, HIP events) accumulated since the last reset:
+ * t[0] FFT+Green stage, t[1] correction+constitutive kernel, t[2] reductions and
+ * copies, t[3] per-increment updates; launches[0..1] kernel launch counts of the
+ * first two stages; passes = fixed-point passes, voxel_iters = passes * V. */
+typedef struct {
+    double t_ms[4];
+    int64_t launches[4];
+    int64_t passes, voxel_iters, nr_steps, evals, outer;
+} cg_stats;
+int cg_get_stats(cg_ctx *ctx, cg_stats *st, int reset);
+/* Enable (1) / disable (0) the per-stage HIP-event timing (default off). */
+int cg_set_timing(cg_ctx *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
