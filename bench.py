@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark: voxel-iterations/s of the LS-EVPFFT fixed-point loop on one MI355X.
+
+Workload (default): the 128^3 / 512-grain FCC creep RVE (sigma33 = 300 MPa,
+n = 10, Voce hardening, Delta-t = 1 s), FP64 form, synthetic seeded
+microstructure.  One "step" is one accepted increment: all of its fixed-point
+passes (FFT + Green operator, correction, constitutive Newton solve,
+reductions, boundary-condition update) plus the accepted-increment updates.
+value = voxel-iterations/s = (passes in the timed increments) * V / T.
+
+Also reported: the mixed form (FFT + Green stage in FP32) on the same RVE, the
+roofline of the dominant kernel (HIP-event times of the library's own stream),
+and the CPU oracle (a C/OpenMP FP64 port of the reference loop) timed on a
+bounded sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+# algorithmic bytes per voxel-iteration of the FFT + Green stage: read the 9-component
+# FP64 PK1 field once, write the 9-component FP64 correction once, plus 2P-2 = 4
+# half-spectrum passes of 9 complex values (P = 3 when one component's complex xy slab fits
+# the LDS; r = (n/2+1)/n).
+def fft_alg_bytes(n, sc=16):
+    r = (n // 2 + 1) / n
+    return 144.0 + 4 * 9 * sc * r
+
+
+def flops_constitutive(stats, ns):
+    """FP64 flop model of the constitutive stage (evaluations, Newton steps, extra
+    line-search evaluations, hardening calls)."""
+    ev = stats.evals
+    nr = stats.nr_steps
+    outer = stats.outer
+    per_eval = 100 * ns
+    return nr * 932 + ev * per_eval + (ev - nr) * 198 + outer * (2 * ns * ns + 10 * ns + 20)
+
+
+def run_gpu(problem, precision, steps, warmup):
+    import torch
+
+    from creep_modeling_amd import driver
+
+    torch.cuda.init()
+    s = driver.Solver(problem, precision=precision, scan_cond=True)
+    lib = s.L
+    for _ in range(warmup):
+        s.step()
+    lib.cg_set_timing(s.h, 1)
+    s.ctx.stats(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rows = [s.step() for _ in range(steps)]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = s.ctx.stats()
+    s.close()
+    return rows, st, t1 - t0
+
+
+def cpu_baseline(problem, max_passes=3):
+    """The C/OpenMP oracle on the first increment of the same RVE, capped at max_passes passes."""
+    from oracle import bind
+
+    p = problem.processes[0]
+    saved = (p.itmax, p.tdotmin)
+    p.itmax = max_passes
+    p.tdotmin = p.tdot  # accept at itmax
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = bind.Oracle(problem, nthreads=threads)
+    t0 = time.perf_counter()
+    r = o.step()
+    t1 = time.perf_counter()
+    o.close()
+    p.itmax, p.tdotmin = saved
+    return dict(value=r.passes * problem.nvox / (t1 - t0), unit="voxel-iters/s", cores=threads, kind="port",
+                sample="first increment of the %dx%dx%d RVE capped at %d fixed-point passes (incl. per-increment "
+                       "setup and updates), %.1f s" % (*problem.n, r.passes, t1 - t0))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--grains", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--cpu-passes", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the multi-GPU layout (independent RVEs per GPU, or z-slab "
+              "decomposition with two all-to-alls per pass) is a design, not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    from creep_modeling_amd import inputs, parse
+
+    ms, mat, procs = inputs.config_case(1, n=args.n, ngrains=args.grains, nsteps=args.warmup + args.steps + 1)
+    problem = parse.problem_from_case(ms, mat, procs)
+    V = problem.nvox
+    ns = problem.phases[0].nsyst
+
+    rows, st, T = run_gpu(problem, 0, args.steps, args.warmup)
+    passes = sum(r["passes"] for r in rows)
+    value = passes * V / T
+    ms_per_step = T / args.steps * 1e3
+    t_fft = st.t_ms[0] / max(st.passes, 1)
+    t_solve = st.t_ms[1] / max(st.passes, 1)
+    fft_bytes = fft_alg_bytes(args.n) * V
+    fft_gbs = fft_bytes / (t_fft * 1e-3) / 1e9
+    flops = flops_constitutive(st, ns)
+    solve_tflops = flops / (st.t_ms[1] * 1e-3) / 1e12 if st.t_ms[1] > 0 else 0.0
+    dominant_solve = t_solve >= t_fft
+    if dominant_solve:
+        roof = dict(bound="mfma", kernel="k_correct_solve (constitutive Newton, FP64 vector ALU)",
+                    achieved=solve_tflops, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=solve_tflops / FP64_PEAK_TFLOPS,
+                    traffic=None, avg_ms=t_solve)
+    else:
+        roof = dict(bound="hbm", kernel="FFT + Green stage", achieved=fft_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=fft_gbs / HBM_PEAK_GBS, traffic=None, avg_ms=t_fft)
+    out = {
+        "metric": "voxel-iters/s at 128^3 creep RVE (mixed vs FP64 GPU) + % HBM roofline",
+        "value": value,
+        "unit": "voxel-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded periodic Voronoi RVE, uniform SO(3) orientations)",
+        "config": {"workload": "%d^3 RVE, %d grains, FCC 12 systems, sigma33=300 MPa creep, FP64 form; step = one "
+                               "accepted increment" % (args.n, args.grains),
+                   "passes": passes, "voxels": V, "rve_increments_per_s": args.steps / T},
+        "roofline": roof,
+        "stages_ms_per_pass": {"fft_green": t_fft, "correct_solve": t_solve,
+                               "reduce_copy": st.t_ms[2] / max(st.passes, 1),
+                               "per_increment_updates_ms": st.t_ms[3] / max(args.steps, 1)},
+        "fft_green_roofline": {"bound": "hbm", "achieved": fft_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": fft_gbs / HBM_PEAK_GBS, "alg_bytes_per_voxel": fft_alg_bytes(args.n)},
+        "constitutive": {"tflops": solve_tflops, "frac_fp64_peak": solve_tflops / FP64_PEAK_TFLOPS,
+                         "nr_steps_per_voxel_iter": st.nr_steps / max(st.voxel_iters, 1),
+                         "evals_per_voxel_iter": st.evals / max(st.voxel_iters, 1),
+                         "pow_per_voxel_iter": st.evals * ns / max(st.voxel_iters, 1)},
+    }
+    if not args.no_mixed:
+        rows_m, st_m, T_m = run_gpu(problem, 1, args.steps, args.warmup)
+        passes_m = sum(r["passes"] for r in rows_m)
+        vm = passes_m * V / T_m
+        t_fft_m = st_m.t_ms[0] / max(st_m.passes, 1)
+        out["mixed"] = {"value": vm, "ratio_vs_fp64": vm / value, "passes": passes_m,
+                        "fft_green_ms_per_pass": t_fft_m,
+                        "fft_green_gbs_alg": fft_alg_bytes(args.n, 8) * V / (t_fft_m * 1e-3) / 1e9,
+                        "creep_rate_rel_err_vs_fp64": abs(rows_m[-1]["edotpavg"][2] - rows[-1]["edotpavg"][2])
+                        / abs(rows[-1]["edotpavg"][2])}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(problem, args.cpu_passes)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -137,7 +137,7 @@ struct cg_ctx {
     double* d_sgsum = nullptr;
     double* d_fluct = nullptr;
     size_t part_cap = 0;
-    int sgsum_valid = 0, lold_same = 0, int_pow = 0, ngas = 0;
+    int sgsum_valid = 0, lold_same = 0, int_pow = 0, ngas = 0, n3node = 1;
     long last_solid = -1;
     double nsys_solid = 0.0;
     // macroscopic state (micro_type, types.f90:55-94)
@@ -364,8 +364,11 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     rc |= dalloc(c, &f.s_L, 9 * V); rc |= dalloc(c, &f.s_Lold, 9 * V); rc |= dalloc(c, &f.s_sg, 6 * V);
     rc |= dalloc(c, &f.s_edotp, 6 * V); rc |= dalloc(c, &f.s_gdot, ns * V); rc |= dalloc(c, &f.s_ttau, 2 * ns * V);
     const size_t nspec = (size_t)f.h1 * f.n2 * f.n3 * 9;
+    // the FP64 spectrum is also used by the nodal integration in the mixed form
     if (c->mixed) HIPCHK(c, hipMalloc(&f.specf, nspec * sizeof(float2)));
-    else HIPCHK(c, hipMalloc(&f.spec, nspec * sizeof(double2)));
+    HIPCHK(c, hipMalloc(&f.spec, nspec * sizeof(double2)));
+    c->n3node = f.n3 == 1 ? 1 : f.n3 + 1;
+    rc |= dalloc(c, &f.xnode, 3 * (size_t)(f.n1 + 1) * (f.n2 + 1) * c->n3node);
     rc |= dalloc(c, &f.red, 256);
     if (rc) return rc;
     if (ensure_part(c, (size_t)((V + 255) / 256 + 64) * 81)) return -3;
@@ -408,6 +411,7 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     HIPCHK(c, hipMemcpy(d_gr, c->grain.data(), V * sizeof(int), hipMemcpyHostToDevice));
     PhaseTables pt{c->d_ph, c->d_tab};
     int nb;
+    launch_init_xnode(f, c->n3node, c->s);
     launch_init_voxels(f, pt, d_eul, d_gr, d_hp, p1->udot, p1->tdot, c->s, &nb);
     double sums[36];
     if (reduce_to_host(c, nb, 36, sums)) return -2;
@@ -664,6 +668,10 @@ int cg_end_increment(cg_ctx* c, const cg_process* p, int scan_diagnostics, cg_in
     }
     if (p->iupdate == 1) {
         launch_plastic_defgrad(f, pt, tdot, c->s);
+        // update_grid_velgrad_node: dv in the (now free) dLref buffer
+        nb = fft_integrate(f, c->fft, p->igamma, f.dLref, c->s);
+        if (reduce_to_host(c, nb, 9, r)) return -2;
+        launch_xnode(f, f.dLref, r, tdot, p->igamma, c->n3node, c->s);
         launch_update_defgrad(f, tdot, c->s, &nb);
         if (reduce_to_host(c, nb, 39, r)) return -2;
         const double detfavg = r[38];
@@ -834,6 +842,7 @@ bool field_desc(cg_ctx* c, const char* name, FieldDesc& d) {
 
 int cg_field_ncomp(cg_ctx* c, const char* name) {
     if (!c || !name) return -1;
+    if (!strcmp(name, "xnode")) return 3;
     FieldDesc d;
     if (!field_desc(c, name, d)) return -1;
     return d.ncomp;
@@ -841,6 +850,13 @@ int cg_field_ncomp(cg_ctx* c, const char* name) {
 
 int cg_get_field(cg_ctx* c, const char* name, double* out, size_t n) {
     if (!c || !c->initialised || !name || !out) return -1;
+    if (!strcmp(name, "xnode")) {  // node coordinates (3, n1+1, n2+1, n3+1), x fastest
+        const size_t nn = 3 * (size_t)(c->n1 + 1) * (c->n2 + 1) * c->n3node;
+        if (n < nn) return fail(c, -1, "output buffer too small");
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        HIPCHK(c, hipMemcpy(out, c->f.xnode, nn * sizeof(double), hipMemcpyDeviceToHost));
+        return 0;
+    }
     FieldDesc d;
     if (!field_desc(c, name, d) || !d.p) return fail(c, -1, "unknown or absent field %s", name);
     const long V = c->V;

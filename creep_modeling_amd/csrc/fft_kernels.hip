@@ -174,7 +174,7 @@ __global__ __launch_bounds__(FBS) void k_x_fwd_pk1(Fields f, Plan3 p, const doub
 // y pass (forward or inverse) over tiles of T kx columns of one z plane
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(FBS) void k_y(Fields f, Plan3 p, int T, int inv) {
+__global__ __launch_bounds__(FBS) void k_y(Fields f, Plan3 p, int T, int inv, int nc) {
     using T2 = typename C2<R>::T;
     extern __shared__ __align__(16) unsigned char smem[];
     const int n2 = f.n2, h1 = f.h1;
@@ -182,20 +182,22 @@ __global__ __launch_bounds__(FBS) void k_y(Fields f, Plan3 p, int T, int inv) {
     const int k = blockIdx.x / ntile, tile = blockIdx.x - k * ntile;
     const int kx0 = tile * T;
     const int Te = min(T, h1 - kx0);
-    const int nsig = Te * 9, ss = n2 + 1;
+    const int nsig = Te * nc, ss = n2 + 1;
     T2* b0 = (T2*)smem;
     T2* b1 = b0 + T * 9 * ss;
     T2* spec = (T2*)(sizeof(R) == 8 ? (void*)f.spec : (void*)f.specf);
-    const int w = Te * 9;
+    const int w = Te * nc;
     for (int e = threadIdx.x; e < n2 * w; e += blockDim.x) {
         const int j = e / w, q = e - j * w;
-        b0[q * ss + j] = spec[(((long)k * n2 + j) * h1 + kx0) * 9 + q];
+        const int t = q / nc, c = q - t * nc;
+        b0[q * ss + j] = spec[(((long)k * n2 + j) * h1 + kx0 + t) * 9 + c];
     }
     __syncthreads();
     T2* res = lds_fft<R>(b0, b1, nsig, ss, p, 1, inv != 0);
     for (int e = threadIdx.x; e < n2 * w; e += blockDim.x) {
         const int j = e / w, q = e - j * w;
-        spec[(((long)k * n2 + j) * h1 + kx0) * 9 + q] = res[q * ss + j];
+        const int t = q / nc, c = q - t * nc;
+        spec[(((long)k * n2 + j) * h1 + kx0 + t) * 9 + c] = res[q * ss + j];
     }
 }
 
@@ -518,9 +520,9 @@ static int run_green(const Fields& f, FftCtx* c, const double* sgsum, const Gree
     const int ty_blocks = f.n3 * ((f.h1 + c->ty - 1) / c->ty);
     const int tz_blocks = f.n2 * ((f.h1 + c->tz - 1) / c->tz);
     hipLaunchKernelGGL(k_x_fwd_pk1<R>, dim3(nrow), dim3(FBS), shx, s, f, c->plan, sgsum, rows, 1.0 / scale);
-    if (f.n2 > 1) hipLaunchKernelGGL(k_y<R>, dim3(ty_blocks), dim3(FBS), shy, s, f, c->plan, c->ty, 0);
+    if (f.n2 > 1) hipLaunchKernelGGL(k_y<R>, dim3(ty_blocks), dim3(FBS), shy, s, f, c->plan, c->ty, 0, 9);
     hipLaunchKernelGGL(k_z_green<R>, dim3(tz_blocks), dim3(FBS), shz, s, f, c->plan, g, c->tz);
-    if (f.n2 > 1) hipLaunchKernelGGL(k_y<R>, dim3(ty_blocks), dim3(FBS), shy, s, f, c->plan, c->ty, 1);
+    if (f.n2 > 1) hipLaunchKernelGGL(k_y<R>, dim3(ty_blocks), dim3(FBS), shy, s, f, c->plan, c->ty, 1, 9);
     hipLaunchKernelGGL(k_x_inv<R>, dim3(nrow), dim3(FBS), shx, s, f, c->plan, rows, scale / (double)f.V);
     return nrow;
 }
@@ -537,8 +539,168 @@ int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double*
 }  // namespace cg
 
 namespace cg {
-int fft_integrate(const Fields& f, FftCtx* c, int ideriv, double* dv, hipStream_t s) {
-    (void)f; (void)c; (void)ideriv; (void)dv; (void)s;
-    return -1;
+// ---------------------------------------------------------------------------
+// spectral integration of L F for the nodal coordinates
+// (update_grid_velgrad_node :1970-2056, integrate_tensor2 :359-429):
+// u_l = sum_m conj(k_m) T_lm / |k|^2 with k = i xi (ideriv 0), i sin(xi) (1) or the
+// rotated operator (2).  The reference keeps Re() of a c2c inverse, i.e. the Hermitian
+// part of the spectrum; on the Nyquist planes k(-xi) is not -k(xi), so the operator
+// is averaged over xi and its mirror.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(FBS) void k_x_fwd_lf(Fields f, Plan3 p, int rows) {
+    using T2 = double2;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int n1 = f.n1, h1 = f.h1, ss = n1 + 1, nsig = rows * 9;
+    T2* b0 = (T2*)smem;
+    T2* b1 = b0 + nsig * ss;
+    const long V = f.V;
+    const long row0 = (long)blockIdx.x * rows;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int e = threadIdx.x; e < rows * n1; e += blockDim.x) {
+        const int r = e / n1, i = e - r * n1;
+        const long v = (row0 + r) * n1 + i;
+        double L[9], F[9], q[9];
+        for (int c = 0; c < 9; c++) { L[c] = f.L[c * V + v]; F[c] = f.F[c * V + v]; }
+        mm3(L, F, q);
+        for (int c = 0; c < 9; c++) {
+            b0[(r * 9 + c) * ss + i] = make_double2(q[c], 0.0);
+            acc[c] += q[c] * (1.0 / (double)V);
+        }
+    }
+    __syncthreads();
+    T2* res = lds_fft<double>(b0, b1, nsig, ss, p, 0, false);
+    for (int e = threadIdx.x; e < rows * h1 * 9; e += blockDim.x) {
+        const int r = e / (h1 * 9), q = e - r * h1 * 9;
+        const int kx = q / 9, c = q - kx * 9;
+        f.spec[((row0 + r) * h1 + kx) * 9 + c] = res[(r * 9 + c) * ss + kx];
+    }
+    block_reduce_store<9, FBS>(acc, f.part);
 }
+
+__device__ void kmod_at(int ideriv, int kx, int ky, int kz, int n1, int n2, int n3, double* kr, double* ki) {
+    const double tp = 2.0 * PI_D;
+    double xi[3];
+    xi[0] = tp * (double)((kx + 1) <= n1 / 2 ? kx : kx - n1) / n1;
+    xi[1] = tp * (double)((ky + 1) <= n2 / 2 ? ky : ky - n2) / n2;
+    xi[2] = (n3 == 1) ? 0.0 : tp * (double)((kz + 1) <= n3 / 2 ? kz : kz - n3) / n3;
+    if (ideriv == 2) {
+        double pr = 1.0, pim = 0.0;
+        for (int l = 0; l < 3; l++) {
+            const double cr = 1.0 + cos(xi[l]), ci = sin(xi[l]);
+            const double nr = pr * cr - pim * ci, ni = pr * ci + pim * cr;
+            pr = nr; pim = ni;
+        }
+        const double fr = -0.25 * pim, fi = 0.25 * pr;  // i/4 * (pr + i pim)
+        for (int l = 0; l < 3; l++) {
+            const double t = tan(xi[l] * 0.5);
+            kr[l] = t * fr;
+            ki[l] = t * fi;
+        }
+    } else {
+        for (int l = 0; l < 3; l++) {
+            kr[l] = 0.0;
+            ki[l] = ideriv == 0 ? xi[l] : sin(xi[l]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(FBS) void k_z_integrate(Fields f, Plan3 p, int T, int ideriv) {
+    using T2 = double2;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int n1 = f.n1, n2 = f.n2, n3 = f.n3, h1 = f.h1;
+    const int ntile = (h1 + T - 1) / T;
+    const int j = blockIdx.x / ntile, tile = blockIdx.x - j * ntile;
+    const int kx0 = tile * T;
+    const int Te = min(T, h1 - kx0);
+    const int nsig = Te * 9, ss = n3 + 1;
+    T2* b0 = (T2*)smem;
+    T2* b1 = b0 + T * 9 * ss;
+    const int w = Te * 9;
+    for (int e = threadIdx.x; e < n3 * w; e += blockDim.x) {
+        const int k = e / w, q = e - k * w;
+        b0[q * ss + k] = f.spec[(((long)k * n2 + j) * h1 + kx0) * 9 + q];
+    }
+    __syncthreads();
+    T2* res = lds_fft<double>(b0, b1, nsig, ss, p, 2, false);
+    for (int e = threadIdx.x; e < Te * n3; e += blockDim.x) {
+        const int t = e / n3, kz = e - t * n3;
+        const int kx = kx0 + t;
+        double Kr[3] = {0, 0, 0}, Ki[3] = {0, 0, 0};
+        for (int side = 0; side < 2; side++) {
+            double kr[3], ki[3];
+            if (side == 0) kmod_at(ideriv, kx, j, kz, n1, n2, n3, kr, ki);
+            else kmod_at(ideriv, (n1 - kx) % n1, (n2 - j) % n2, (n3 - kz) % n3, n1, n2, n3, kr, ki);
+            const double k2 = kr[0] * kr[0] + ki[0] * ki[0] + kr[1] * kr[1] + ki[1] * ki[1] + kr[2] * kr[2] + ki[2] * ki[2];
+            if (k2 > 0.0)
+                for (int m = 0; m < 3; m++) {  // side 0: conj(k)/|k|^2 ; side 1 (mirror): k(-xi)/|k(-xi)|^2
+                    Kr[m] += 0.5 * kr[m] / k2;
+                    Ki[m] += 0.5 * (side == 0 ? -ki[m] : ki[m]) / k2;
+                }
+        }
+        const bool allnyq = (n1 % 2 == 0 && kx == n1 / 2) && (n2 % 2 == 0 && j == n2 / 2) && (n3 % 2 == 0 && kz == n3 / 2) &&
+                            n3 > 1;
+        T2* P = res + (t * 9) * ss + kz;
+        double ur[3], ui[3];
+        for (int l = 0; l < 3; l++) {
+            double sr = 0.0, si = 0.0;
+            for (int m = 0; m < 3; m++) {
+                const T2 z = P[(3 * l + m) * ss];
+                sr += Kr[m] * z.x - Ki[m] * z.y;
+                si += Kr[m] * z.y + Ki[m] * z.x;
+            }
+            ur[l] = (ideriv == 2 && allnyq) ? 0.0 : sr;
+            ui[l] = (ideriv == 2 && allnyq) ? 0.0 : si;
+        }
+        for (int l = 0; l < 3; l++) P[l * ss] = make_double2(ur[l], ui[l]);
+    }
+    __syncthreads();
+    T2* other = (res == b0) ? b1 : b0;
+    res = lds_fft<double>(res, other, nsig, ss, p, 2, true);
+    for (int e = threadIdx.x; e < n3 * w; e += blockDim.x) {
+        const int k = e / w, q = e - k * w;
+        if (q % 9 < 3) f.spec[(((long)k * n2 + j) * h1 + kx0) * 9 + q] = res[q * ss + k];
+    }
+}
+
+__global__ __launch_bounds__(FBS) void k_x_inv3(Fields f, Plan3 p, int rows, double* __restrict__ dv) {
+    using T2 = double2;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int n1 = f.n1, h1 = f.h1, ss = n1 + 1, nsig = rows * 3;
+    T2* b0 = (T2*)smem;
+    T2* b1 = b0 + nsig * ss;
+    const long V = f.V;
+    const long row0 = (long)blockIdx.x * rows;
+    for (int e = threadIdx.x; e < rows * h1 * 3; e += blockDim.x) {
+        const int r = e / (h1 * 3), q = e - r * h1 * 3;
+        const int kx = q / 3, c = q - kx * 3;
+        const T2 z = f.spec[((row0 + r) * h1 + kx) * 9 + c];
+        T2* s = b0 + (r * 3 + c) * ss;
+        s[kx] = z;
+        if (kx > 0 && kx < n1 - kx) s[n1 - kx] = make_double2(z.x, -z.y);
+    }
+    __syncthreads();
+    T2* res = lds_fft<double>(b0, b1, nsig, ss, p, 0, true);
+    for (int e = threadIdx.x; e < rows * n1; e += blockDim.x) {
+        const int r = e / n1, i = e - r * n1;
+        const long v = (row0 + r) * n1 + i;
+        for (int c = 0; c < 3; c++) dv[c * V + v] = res[(r * 3 + c) * ss + i].x / (double)V;
+    }
+}
+
+int fft_integrate(const Fields& f, FftCtx* c, int ideriv, double* dv, hipStream_t s) {
+    const int rows = c->rows;
+    const int nrow = f.n2 * f.n3 / rows;
+    const size_t shx = 2 * (size_t)rows * 9 * (f.n1 + 1) * sizeof(double2);
+    const size_t shy = 2 * (size_t)c->ty * 9 * (f.n2 + 1) * sizeof(double2);
+    const size_t shz = 2 * (size_t)c->tz * 9 * (f.n3 + 1) * sizeof(double2);
+    const int ty_blocks = f.n3 * ((f.h1 + c->ty - 1) / c->ty);
+    const int tz_blocks = f.n2 * ((f.h1 + c->tz - 1) / c->tz);
+    hipLaunchKernelGGL(k_x_fwd_lf, dim3(nrow), dim3(FBS), shx, s, f, c->plan, rows);
+    if (f.n2 > 1) hipLaunchKernelGGL(k_y<double>, dim3(ty_blocks), dim3(FBS), shy, s, f, c->plan, c->ty, 0, 9);
+    hipLaunchKernelGGL(k_z_integrate, dim3(tz_blocks), dim3(FBS), shz, s, f, c->plan, c->tz, ideriv);
+    if (f.n2 > 1) hipLaunchKernelGGL(k_y<double>, dim3(ty_blocks), dim3(FBS), shy, s, f, c->plan, c->ty, 1, 3);
+    hipLaunchKernelGGL(k_x_inv3, dim3(nrow), dim3(FBS), shx, s, f, c->plan, rows, dv);
+    return nrow;
+}
+
 }  // namespace cg

@@ -94,6 +94,9 @@ void launch_snapshot(const Fields& f, int restore, hipStream_t s);
 void launch_init_voxels(const Fields& f, const PhaseTables& pt, const double* euler, const int* grain,
                         const double* hp_grain, const double* udot, double tdot, hipStream_t s, int* nb);
 void launch_copy(double* dst, const double* src, size_t n, hipStream_t s);
+void launch_init_xnode(const Fields& f, int n3node, hipStream_t s);
+void launch_xnode(const Fields& f, const double* dv, const double* lavg, double tdot, int ideriv, int n3node,
+                  hipStream_t s);
 
 // ---- fft_kernels.hip ----
 struct FftCtx;

@@ -1236,3 +1236,66 @@ void launch_copy(double* dst, const double* src, size_t n, hipStream_t s) {
 }
 
 }  // namespace cg
+
+namespace cg {
+// ---------------------------------------------------------------------------
+// nodal coordinates (init_micro types.f90:268-287; update_grid_velgrad_node
+// various_functions.f90:2026-2054 with cell2node_tensor1 :2652-2691)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_init_xnode(Fields f, int n3node) {
+    const long nn = (long)(f.n1 + 1) * (f.n2 + 1) * n3node;
+    const long q = (long)blockIdx.x * BS + threadIdx.x;
+    if (q >= nn) return;
+    const int i = q % (f.n1 + 1), j = (q / (f.n1 + 1)) % (f.n2 + 1), k = q / ((long)(f.n1 + 1) * (f.n2 + 1));
+    f.xnode[3 * q] = (i + 1) - 0.5;
+    f.xnode[3 * q + 1] = (j + 1) - 0.5;
+    f.xnode[3 * q + 2] = n3node == 1 ? 0.0 : (k + 1) - 0.5;
+}
+struct NodeArgs { double lavg[9]; double tdot; int ideriv, n3node; };
+__global__ __launch_bounds__(BS) void k_xnode(Fields f, const double* __restrict__ dv, NodeArgs a) {
+    const int n1 = f.n1, n2 = f.n2, n3 = f.n3;
+    const long V = f.V;
+    const long nn = (long)(n1 + 1) * (n2 + 1) * a.n3node;
+    const long q = (long)blockIdx.x * BS + threadIdx.x;
+    if (q >= nn) return;
+    const int i = q % (n1 + 1) + 1, j = (q / (n1 + 1)) % (n2 + 1) + 1, k = q / ((long)(n1 + 1) * (n2 + 1)) + 1;
+    auto D = [&](int l, int ii, int jj, int kk) {
+        return dv[l * V + (ii - 1) + (long)n1 * ((jj - 1) + (long)n2 * (kk - 1))];
+    };
+    double dn[3];
+    if (a.ideriv != 2) {
+        int k1 = k - 1, k2 = k, j1 = j - 1, j2 = j, i1 = i - 1, i2 = i;
+        if (k1 == 0) k1 = n3;
+        if (k2 == n3 + 1) k2 = 1;
+        if (j1 == 0) j1 = n2;
+        if (j2 == n2 + 1) j2 = 1;
+        if (i1 == 0) i1 = n1;
+        if (i2 == n2 + 1) i2 = 1;  // the reference wraps the x index with npts2 (:2678)
+        for (int l = 0; l < 3; l++)
+            dn[l] = (D(l, i2, j2, k2) + D(l, i1, j2, k2) + D(l, i2, j1, k2) + D(l, i2, j2, k1) + D(l, i1, j1, k2) +
+                     D(l, i1, j2, k1) + D(l, i2, j1, k1) + D(l, i1, j1, k1)) * 0.125;
+    } else {
+        const int ii = i > n1 ? 1 : i, jj = j > n2 ? 1 : j, kk = k > n3 ? 1 : k;
+        for (int l = 0; l < 3; l++) dn[l] = D(l, ii, jj, kk);
+    }
+    const double x[3] = {i - 0.5, j - 0.5, a.n3node == 1 ? 0.0 : k - 0.5};
+    for (int l = 0; l < 3; l++) {
+        const double lx = a.lavg[3 * l] * x[0] + a.lavg[3 * l + 1] * x[1] + a.lavg[3 * l + 2] * x[2];
+        f.xnode[3 * q + l] += (lx + dn[l]) * a.tdot;
+    }
+}
+void launch_init_xnode(const Fields& f, int n3node, hipStream_t s) {
+    const long nn = (long)(f.n1 + 1) * (f.n2 + 1) * n3node;
+    hipLaunchKernelGGL(k_init_xnode, dim3((unsigned)((nn + BS - 1) / BS)), dim3(BS), 0, s, f, n3node);
+}
+void launch_xnode(const Fields& f, const double* dv, const double* lavg, double tdot, int ideriv, int n3node,
+                  hipStream_t s) {
+    NodeArgs a;
+    for (int i = 0; i < 9; i++) a.lavg[i] = lavg[i];
+    a.tdot = tdot;
+    a.ideriv = ideriv;
+    a.n3node = n3node;
+    const long nn = (long)(f.n1 + 1) * (f.n2 + 1) * n3node;
+    hipLaunchKernelGGL(k_xnode, dim3((unsigned)((nn + BS - 1) / BS)), dim3(BS), 0, s, f, dv, a);
+}
+}  // namespace cg

@@ -11,6 +11,7 @@ program lsevpfft_host
     use iso_c_binding
     use creep_hip_iface
     use host_io
+    use host_vtk
     implicit none
 
     type(input_t) :: inp
@@ -125,6 +126,7 @@ program lsevpfft_host
     if (rc /= 0) call fatal('cg_set_voxels: '//trim(last_error(ctx)))
     rc = cg_init(ctx, inp%proc(1))
     if (rc /= 0) call fatal('cg_init: '//trim(last_error(ctx)))
+    call write_vtk_cell(ctx, inp, 0, output_dir)
 
     call system_clock(start, cnt_rate)
     ! initial rows (LS-EVPFFT.f90:176-178): averages are zero except dvm = vm(dsim of process 1)
@@ -227,6 +229,11 @@ program lsevpfft_host
                 if (p%ictrl == -1) then
                     proc_time_done = proc_time_done + tdot_used
                     p%tdot = tdot_plan
+                end if
+                ! field output cadence (LS-EVPFFT.f90:526-535)
+                if (p%iwfields == 1 .and. p%iwstep /= 0) then
+                    if (mod(imicro, abs(p%iwstep)) == 0 .or. (p%iwstep < 0 .and. imicro == p%nsteps)) &
+                        call write_vtk_cell(ctx, inp, imacroloop, output_dir)
                 end if
                 sim_time = sim_time + tdot_used
                 call system_clock(now)

@@ -180,7 +180,11 @@ class Context:
         k = self.L.cg_field_ncomp(self.h, name.encode())
         if k < 0:
             raise KeyError(name)
-        out = np.zeros(k * self.problem.nvox)
+        if name == "xnode":
+            n1, n2, n3 = self.problem.n
+            out = np.zeros(3 * (n1 + 1) * (n2 + 1) * (1 if n3 == 1 else n3 + 1))
+        else:
+            out = np.zeros(k * self.problem.nvox)
         self._chk(self.L.cg_get_field(self.h, name.encode(), dp(out), out.size), "cg_get_field")
         return out
 
