@@ -283,8 +283,34 @@ __device__ void green81(const GreenDev& g, int kx, int ky, int kz, int n1, int n
     }
 }
 
-// out_ij = sum_kl Gamma_ijkl(xi) P_kl at one frequency (9 complex in place, stride st)
+// general igamma (1, 2): the full 81-component operator, kept out of line so the
+// igamma = 0 kernel stays small
 template <typename R>
+__device__ __noinline__ void green_apply_general(const GreenDev& g, int kx, int ky, int kz, int n1, int n2, int n3,
+                                                 typename C2<R>::T* P, int st) {
+    using T2 = typename C2<R>::T;
+    T2 in[9];
+    for (int c = 0; c < 9; c++) in[c] = P[c * st];
+    // The reference keeps Re() of a full c2c inverse, i.e. it applies the even part
+    // (Gamma(xi) + Gamma(-xi))/2 to the Hermitian spectrum; igamma = 2 is not even on
+    // the Nyquist planes (xi = -pi is not mirrored), so both halves are evaluated.
+    double G[81], Gm[81];
+    green81(g, kx, ky, kz, n1, n2, n3, G);
+    green81(g, (n1 - kx) % n1, (n2 - ky) % n2, (n3 - kz) % n3, n1, n2, n3, Gm);
+    for (int i = 0; i < 81; i++) G[i] = 0.5 * (G[i] + Gm[i]);
+    for (int ij = 0; ij < 9; ij++) {
+        double sr = 0.0, si = 0.0;
+        for (int kl = 0; kl < 9; kl++) {
+            sr += G[9 * ij + kl] * (double)in[kl].x;
+            si += G[9 * ij + kl] * (double)in[kl].y;
+        }
+        P[ij * st].x = (R)sr;
+        P[ij * st].y = (R)si;
+    }
+}
+
+// out_ij = sum_kl Gamma_ijkl(xi) P_kl at one frequency (9 complex in place, stride st)
+template <typename R, bool GENERAL>
 __device__ __forceinline__ void green_apply(const GreenDev& g, int kx, int ky, int kz, int n1, int n2, int n3,
                                             typename C2<R>::T* P, int st) {
     using T2 = typename C2<R>::T;
@@ -292,11 +318,16 @@ __device__ __forceinline__ void green_apply(const GreenDev& g, int kx, int ky, i
         for (int c = 0; c < 9; c++) { P[c * st].x = (R)0; P[c * st].y = (R)0; }
         return;
     }
+    if (GENERAL) {
+        green_apply_general<R>(g, kx, ky, kz, n1, n2, n3, P, st);
+        return;
+    }
     const bool nx = (n1 % 2 == 0) && kx == n1 / 2, ny = (n2 % 2 == 0) && ky == n2 / 2,
                nz = (n3 % 2 == 0) && kz == n3 / 2 && n3 > 1;
     T2 in[9];
+#pragma unroll
     for (int c = 0; c < 9; c++) in[c] = P[c * st];
-    if (g.igamma == 0) {
+    {
         if (nx || ny || nz) {  // Gamma = -S0 on any Nyquist plane (form_G :646-651)
             for (int ij = 0; ij < 9; ij++) {
                 double sr = 0.0, si = 0.0;
@@ -340,25 +371,9 @@ __device__ __forceinline__ void green_apply(const GreenDev& g, int kx, int ky, i
         }
         return;
     }
-    // The reference keeps Re() of a full c2c inverse, i.e. it applies the even part
-    // (Gamma(xi) + Gamma(-xi))/2 to the Hermitian spectrum; igamma = 2 is not even on
-    // the Nyquist planes (xi = -pi is not mirrored), so both halves are evaluated.
-    double G[81], Gm[81];
-    green81(g, kx, ky, kz, n1, n2, n3, G);
-    green81(g, (n1 - kx) % n1, (n2 - ky) % n2, (n3 - kz) % n3, n1, n2, n3, Gm);
-    for (int i = 0; i < 81; i++) G[i] = 0.5 * (G[i] + Gm[i]);
-    for (int ij = 0; ij < 9; ij++) {
-        double sr = 0.0, si = 0.0;
-        for (int kl = 0; kl < 9; kl++) {
-            sr += G[9 * ij + kl] * (double)in[kl].x;
-            si += G[9 * ij + kl] * (double)in[kl].y;
-        }
-        P[ij * st].x = (R)sr;
-        P[ij * st].y = (R)si;
-    }
 }
 
-template <typename R>
+template <typename R, bool GENERAL>
 __global__ __launch_bounds__(FBS) void k_z_green(Fields f, Plan3 p, GreenDev g, int T) {
     using T2 = typename C2<R>::T;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -380,7 +395,7 @@ __global__ __launch_bounds__(FBS) void k_z_green(Fields f, Plan3 p, GreenDev g, 
     T2* res = lds_fft<R>(b0, b1, nsig, ss, p, 2, false);
     for (int e = threadIdx.x; e < Te * n3; e += blockDim.x) {
         const int t = e / n3, kz = e - t * n3;
-        green_apply<R>(g, kx0 + t, j, kz, f.n1, n2, n3, res + (t * 9) * ss + kz, ss);
+        green_apply<R, GENERAL>(g, kx0 + t, j, kz, f.n1, n2, n3, res + (t * 9) * ss + kz, ss);
     }
     __syncthreads();
     T2* other = (res == b0) ? b1 : b0;
@@ -521,7 +536,8 @@ static int run_green(const Fields& f, FftCtx* c, const double* sgsum, const Gree
     const int tz_blocks = f.n2 * ((f.h1 + c->tz - 1) / c->tz);
     hipLaunchKernelGGL(k_x_fwd_pk1<R>, dim3(nrow), dim3(FBS), shx, s, f, c->plan, sgsum, rows, 1.0 / scale);
     if (f.n2 > 1) hipLaunchKernelGGL(k_y<R>, dim3(ty_blocks), dim3(FBS), shy, s, f, c->plan, c->ty, 0, 9);
-    hipLaunchKernelGGL(k_z_green<R>, dim3(tz_blocks), dim3(FBS), shz, s, f, c->plan, g, c->tz);
+    if (g.igamma == 0) hipLaunchKernelGGL((k_z_green<R, false>), dim3(tz_blocks), dim3(FBS), shz, s, f, c->plan, g, c->tz);
+    else hipLaunchKernelGGL((k_z_green<R, true>), dim3(tz_blocks), dim3(FBS), shz, s, f, c->plan, g, c->tz);
     if (f.n2 > 1) hipLaunchKernelGGL(k_y<R>, dim3(ty_blocks), dim3(FBS), shy, s, f, c->plan, c->ty, 1, 9);
     hipLaunchKernelGGL(k_x_inv<R>, dim3(nrow), dim3(FBS), shx, s, f, c->plan, rows, scale / (double)f.V);
     return nrow;

@@ -55,15 +55,17 @@ struct PassConst {
 };
 
 // reduction slots written by the fused correction + constitutive kernel
+// (slots 0..22 come from the constitutive phase, 23..53 from the correction phase)
 enum {
     R_ERRS = 0, R_ERRE = 1,
     R_SGSOL = 2,     // 9: sum_solid w sigma
     R_SGALL = 11,    // 9: sum_all w sigma (next pass' <sigma>)
-    R_LALL = 20,     // 9: sum w L (before mixing)
-    R_LGAS = 29,     // 9
-    R_LSOL = 38,     // 9
-    R_DLSOL = 47, R_WSOL = 48, R_DLGAS = 49, R_WGAS = 50,
-    R_NR = 51, R_EVAL = 52, R_OUTER = 53,
+    R_NR = 20, R_EVAL = 21, R_OUTER = 22,
+    R_NPHASEB = 23,
+    R_LALL = 23,     // 9: sum w L (before mixing)
+    R_LGAS = 32,     // 9
+    R_LSOL = 41,     // 9
+    R_DLSOL = 50, R_WSOL = 51, R_DLGAS = 52, R_WGAS = 53,
     R_NSOLVE = 54
 };
 

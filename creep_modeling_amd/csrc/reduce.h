@@ -32,4 +32,25 @@ __device__ __forceinline__ void block_reduce_store(const double* acc, double* pa
     }
 }
 
+// Same, writing part[blockIdx.x*stride + off + i] (several reductions sharing one row).
+template <int NV, int BS, class Op = OpSum>
+__device__ __forceinline__ void block_reduce_store_at(const double* acc, double* part, int stride, int off) {
+    static_assert(NV <= BS, "NV must not exceed the block size");
+    __shared__ double sm[(BS / 64) * NV];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        double v = acc[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = Op::f(v, wave_shfl_down(v, o));
+        if (lane == 0) sm[wid * NV + i] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = sm[threadIdx.x];
+        for (int w = 1; w < BS / 64; w++) s = Op::f(s, sm[w * NV + threadIdx.x]);
+        part[(size_t)blockIdx.x * stride + off + threadIdx.x] = s;
+    }
+}
+
 }  // namespace cg

@@ -199,60 +199,73 @@ template <bool MODAL, bool INTPOW>
 __global__ __launch_bounds__(BS) void k_correct_solve(Fields f, PhaseTables pt, SolveArgs a) {
     const long V = f.V;
     const long v = (long)blockIdx.x * BS + threadIdx.x;
-    double acc[R_NSOLVE];
-#pragma unroll
-    for (int i = 0; i < R_NSOLVE; i++) acc[i] = 0.0;
+    const bool act = v < V;
     const PassConst& pc = a.pc;
-    if (v < V) {
-        // ---- correction_to_current (dudot from the reduced fluctuation average)
-        double dud[9];
-        {
-            double dm[9];
-            for (int i = 0; i < 9; i++) dm[i] = a.dvelgradavg[i] - a.fluct[i];
-            mm3(dm, a.finv_inv, dud);
+    // ---- phase A: correction_to_current + dL_iter/mixing; its 31 sums are reduced
+    // before the constitutive solve so their accumulators are not live across it
+    double D[6];
+    double w = 0.0;
+    bool gas = false;
+    {
+        double accA[R_NSOLVE - R_NPHASEB];
+#pragma unroll
+        for (int i = 0; i < R_NSOLVE - R_NPHASEB; i++) accA[i] = 0.0;
+        if (act) {
+            // dudot from the reduced fluctuation average
+            double dud[9];
+            {
+                double dm[9];
+                for (int i = 0; i < 9; i++) dm[i] = a.dvelgradavg[i] - a.fluct[i];
+                mm3(dm, a.finv_inv, dud);
+            }
+            double L[9], Lo[9], Fi[9], t[9];
+            for (int c = 0; c < 9; c++) {
+                L[c] = f.L[c * V + v];
+                Fi[c] = f.Fi[c * V + v];
+                t[c] = f.dLref[c * V + v] + dud[c];
+            }
+            if (pc.lold_same) {
+                for (int c = 0; c < 9; c++) Lo[c] = L[c];
+            } else {
+                for (int c = 0; c < 9; c++) Lo[c] = f.Lold[c * V + v];
+            }
+            double q[9];
+            mm3(t, Fi, q);
+            for (int c = 0; c < 9; c++) L[c] += q[c];
+            w = f.w[v];
+            gas = f.gas[v] != 0;
+            for (int c = 0; c < 9; c++) {
+                accA[R_LALL - R_NPHASEB + c] = L[c] * w;
+                accA[(gas ? R_LGAS : R_LSOL) - R_NPHASEB + c] = L[c] * w;
+            }
+            // dL_iter and mixing (LS-EVPFFT.f90:277-327)
+            if (!pc.kin_ctrl) {
+                double d2 = 0.0;
+                for (int c = 0; c < 9; c++) d2 += (L[c] - Lo[c]) * (L[c] - Lo[c]);
+                const double rel = sqrt(d2) / pc.lscale;
+                accA[(gas ? R_DLGAS : R_DLSOL) - R_NPHASEB] = rel * rel * w;
+                accA[(gas ? R_WGAS : R_WSOL) - R_NPHASEB] = w;
+            }
+            if (pc.do_mix) {
+                for (int c = 0; c < 9; c++) L[c] = pc.mix * L[c] + (1.0 - pc.mix) * Lo[c];
+            } else {
+                for (int c = 0; c < 9; c++) L[c] = pc.mix * L[c];
+            }
+            for (int c = 0; c < 9; c++) f.L[c * V + v] = L[c];
+            to6(L, D);
         }
-        double L[9], Lo[9], Fi[9], t[9];
-        for (int c = 0; c < 9; c++) {
-            L[c] = f.L[c * V + v];
-            Fi[c] = f.Fi[c * V + v];
-            t[c] = f.dLref[c * V + v] + dud[c];
-        }
-        if (pc.lold_same) {
-            for (int c = 0; c < 9; c++) Lo[c] = L[c];
-        } else {
-            for (int c = 0; c < 9; c++) Lo[c] = f.Lold[c * V + v];
-        }
-        double q[9];
-        mm3(t, Fi, q);
-        for (int c = 0; c < 9; c++) L[c] += q[c];
-        const double w = f.w[v];
-        const bool gas = f.gas[v] != 0;
-        for (int c = 0; c < 9; c++) {
-            acc[R_LALL + c] = L[c] * w;
-            acc[(gas ? R_LGAS : R_LSOL) + c] = L[c] * w;
-        }
-        // ---- dL_iter and mixing (LS-EVPFFT.f90:277-327)
-        if (!pc.kin_ctrl) {
-            double d2 = 0.0;
-            for (int c = 0; c < 9; c++) d2 += (L[c] - Lo[c]) * (L[c] - Lo[c]);
-            const double rel = sqrt(d2) / pc.lscale;
-            acc[gas ? R_DLGAS : R_DLSOL] = rel * rel * w;
-            acc[gas ? R_WGAS : R_WSOL] = w;
-        }
-        if (pc.do_mix) {
-            for (int c = 0; c < 9; c++) L[c] = pc.mix * L[c] + (1.0 - pc.mix) * Lo[c];
-        } else {
-            for (int c = 0; c < 9; c++) L[c] = pc.mix * L[c];
-        }
-        for (int c = 0; c < 9; c++) f.L[c * V + v] = L[c];
-
-        // ---- solve_res (various_functions.f90:1142-1364)
-        double lam[6], sgt[6], D[6];
+        block_reduce_store_at<R_NSOLVE - R_NPHASEB, BS>(accA, f.part, R_NSOLVE, R_NPHASEB);
+    }
+    // ---- phase B: solve_res (various_functions.f90:1142-1364)
+    double acc[R_NPHASEB];
+#pragma unroll
+    for (int i = 0; i < R_NPHASEB; i++) acc[i] = 0.0;
+    if (act) {
+        double lam[6], sgt[6];
         for (int c = 0; c < 6; c++) {
             lam[c] = f.sg[c * V + v];
             sgt[c] = f.sgt[c * V + v];
         }
-        to6(L, D);
         const double tdot = pc.tdot;
         double sgf[6];
         if (!gas) {
@@ -314,9 +327,12 @@ __global__ __launch_bounds__(BS) void k_correct_solve(Fields f, PhaseTables pt, 
                         for (int c = 0; c < 6; c++) sev[c] = sg[c];
                         slip_eval<INTPOW, false>(f, tab, P, v, sg, ep, dd);
                         double rn = 0.0;
+#pragma unroll
                         for (int i = 0; i < 6; i++) {
                             double s = r0[i];
+#pragma unroll
                             for (int k = 0; k < 6; k++) s += Jb[6 * i + k] * sg[k];
+#pragma unroll
                             for (int k = 0; k < 5; k++) s += cm[6 * i + k] * ep[k];
                             res[i] = s;
                             rn += s * s;
@@ -330,10 +346,13 @@ __global__ __launch_bounds__(BS) void k_correct_solve(Fields f, PhaseTables pt, 
                     resoldn = resn;
                     for (int c = 0; c < 6; c++) sgold[c] = sg[c];
                     double Jm[36];
+#pragma unroll
                     for (int i = 0; i < 6; i++)
+#pragma unroll
                         for (int j = 0; j < 6; j++) {
                             double s = Jb[6 * i + j];
                             if (j < 5)
+#pragma unroll
                                 for (int k = 0; k < 5; k++) s += cm[6 * i + k] * dd[sym5(k, j)];
                             Jm[6 * i + j] = s;
                         }
@@ -404,7 +423,8 @@ __global__ __launch_bounds__(BS) void k_correct_solve(Fields f, PhaseTables pt, 
             if (!gas) acc[R_SGSOL + c] = s3[c] * w;
         }
     }
-    block_reduce_store<R_NSOLVE, BS>(acc, f.part);
+    __syncthreads();
+    block_reduce_store_at<R_NPHASEB, BS>(acc, f.part, R_NSOLVE, 0);
 }
 
 // Public launcher with the full argument set.

@@ -110,32 +110,32 @@ CG_HD double vm_strain6(const double* s) {
 
 // ---- 6x6 dense ---------------------------------------------------------------
 // Gaussian elimination with partial pivoting; solves A x = b in place (A destroyed).
+// The pivot row is brought up by compare-and-swap over the rows below it, so every
+// index is a compile-time constant after unrolling (the matrix stays in registers);
+// the pivot is the first row of largest magnitude, as with an arg-max search.
 // Returns false for an exactly singular pivot.
 CG_HD bool solve6(double* A, double* b) {
     bool ok = true;
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-        int p = k;
-        double mx = fabs(A[6 * k + k]);
 #pragma unroll
         for (int i = k + 1; i < 6; i++) {
-            double v = fabs(A[6 * i + k]);
-            if (v > mx) { mx = v; p = i; }
-        }
-        if (mx == 0.0) ok = false;
-        if (p != k) {
+            const bool sw = fabs(A[6 * i + k]) > fabs(A[6 * k + k]);
 #pragma unroll
-            for (int j = 0; j < 6; j++) {
-                double t = A[6 * k + j];
-                A[6 * k + j] = A[6 * p + j];
-                A[6 * p + j] = t;
+            for (int j = k; j < 6; j++) {
+                const double t = A[6 * k + j];
+                A[6 * k + j] = sw ? A[6 * i + j] : t;
+                A[6 * i + j] = sw ? t : A[6 * i + j];
             }
-            double t = b[k]; b[k] = b[p]; b[p] = t;
+            const double t = b[k];
+            b[k] = sw ? b[i] : t;
+            b[i] = sw ? t : b[i];
         }
-        double r = 1.0 / A[6 * k + k];
+        if (A[6 * k + k] == 0.0) ok = false;
+        const double r = 1.0 / A[6 * k + k];
 #pragma unroll
         for (int i = k + 1; i < 6; i++) {
-            double l = A[6 * i + k] * r;
+            const double l = A[6 * i + k] * r;
 #pragma unroll
             for (int j = k + 1; j < 6; j++) A[6 * i + j] -= l * A[6 * k + j];
             b[i] -= l * b[k];
