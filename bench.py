@@ -88,6 +88,24 @@ def cpu_baseline(problem, max_passes=3):
                        "setup and updates), %.1f s" % (*problem.n, r.passes, t1 - t0))
 
 
+def copy_calibration(gib=1.0, reps=5):
+    """A plain device copy of known size (torch's elementwise copy kernel), to scale the
+    FETCH_SIZE / WRITE_SIZE counters: each launch reads and writes exactly `gib` GiB."""
+    import torch
+
+    n = int(gib * (1 << 30)) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "reps": reps,
+                      "gbs_read_plus_write": 2 * n * 8 / dt / 1e9}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,7 +116,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-passes", type=int, default=2)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the multi-GPU layout (independent RVEs per GPU, or z-slab "
               "decomposition with two all-to-alls per pass) is a design, not part of this build", file=sys.stderr)

@@ -9,6 +9,7 @@
 #include "internal.h"
 #include "reduce.h"
 #include "tensor.h"
+#include "vox.h"
 
 namespace cg {
 
@@ -108,24 +109,24 @@ struct SolveArgs {
 // Power-law slip rates at sigma (b-basis) -> edotp (5), dedotp (sym 5x5, 15), optional gdot store.
 template <bool INTPOW, bool STORE>
 __device__ __forceinline__ void slip_eval(const Fields& f, const double* __restrict__ tab, const PhaseDev& P,
-                                          long v, const double* sg, double* ep, double* dd) {
-    const long V = f.V;
+                                          const Vox& x, const double* sg, double* ep, double* dd) {
     for (int i = 0; i < 5; i++) ep[i] = 0.0;
     for (int i = 0; i < 15; i++) dd[i] = 0.0;
     const double* nrs = tab + P.off_nrs;
     const double* g0 = tab + P.off_gamd0;
+#pragma unroll 1
     for (int is = 0; is < P.nsyst; is++) {
-        double s0 = f.sch[(5 * is + 0) * V + v], s1 = f.sch[(5 * is + 1) * V + v], s2 = f.sch[(5 * is + 2) * V + v],
-               s3 = f.sch[(5 * is + 3) * V + v], s4 = f.sch[(5 * is + 4) * V + v];
+        const double s0 = x.ld(f.sch, 5 * is + 0), s1 = x.ld(f.sch, 5 * is + 1), s2 = x.ld(f.sch, 5 * is + 2),
+                     s3 = x.ld(f.sch, 5 * is + 3), s4 = x.ld(f.sch, 5 * is + 4);
         double rss = s0 * sg[0] + s1 * sg[1] + s2 * sg[2] + s3 * sg[3] + s4 * sg[4];
         const int isg = rss < 0.0 ? 1 : 0;
-        const double tau = f.ttau[(2 * is + isg) * V + v];
+        const double tau = x.ld(f.ttau, 2 * is + isg);
         rss = rss / tau;
         const double n = nrs[is];
         const double dum = g0[is] * powm1<INTPOW>(fabs(rss), n);
         const double r1 = n * dum / tau;
         const double r2 = rss * dum;
-        if (STORE) f.gdot[is * V + v] = r2;
+        if (STORE) x.st(f.gdot, is, r2);
         ep[0] += s0 * r2; ep[1] += s1 * r2; ep[2] += s2 * r2; ep[3] += s3 * r2; ep[4] += s4 * r2;
         const double t0 = s0 * r1, t1 = s1 * r1, t2 = s2 * r1, t3 = s3 * r1, t4 = s4 * r1;
         dd[0] += s0 * t0; dd[1] += s0 * t1; dd[2] += s0 * t2; dd[3] += s0 * t3; dd[4] += s0 * t4;
@@ -138,19 +139,18 @@ __device__ __forceinline__ void slip_eval(const Fields& f, const double* __restr
 // gdot only (for harden and the final store): same arithmetic as slip_eval
 template <bool INTPOW>
 __device__ __forceinline__ void slip_rates(const Fields& f, const double* __restrict__ tab, const PhaseDev& P,
-                                           long v, const double* sg) {
-    const long V = f.V;
+                                           const Vox& x, const double* sg) {
     const double* nrs = tab + P.off_nrs;
     const double* g0 = tab + P.off_gamd0;
     for (int is = 0; is < P.nsyst; is++) {
-        double s0 = f.sch[(5 * is + 0) * V + v], s1 = f.sch[(5 * is + 1) * V + v], s2 = f.sch[(5 * is + 2) * V + v],
-               s3 = f.sch[(5 * is + 3) * V + v], s4 = f.sch[(5 * is + 4) * V + v];
+        const double s0 = x.ld(f.sch, 5 * is + 0), s1 = x.ld(f.sch, 5 * is + 1), s2 = x.ld(f.sch, 5 * is + 2),
+                     s3 = x.ld(f.sch, 5 * is + 3), s4 = x.ld(f.sch, 5 * is + 4);
         double rss = s0 * sg[0] + s1 * sg[1] + s2 * sg[2] + s3 * sg[3] + s4 * sg[4];
         const int isg = rss < 0.0 ? 1 : 0;
-        const double tau = f.ttau[(2 * is + isg) * V + v];
+        const double tau = x.ld(f.ttau, 2 * is + isg);
         rss = rss / tau;
         const double dum = g0[is] * powm1<INTPOW>(fabs(rss), nrs[is]);
-        f.gdot[is * V + v] = rss * dum;
+        x.st(f.gdot, is, rss * dum);
     }
 }
 __device__ __forceinline__ int sym5(int i, int j) {  // packed index of the symmetric 5x5
@@ -161,11 +161,10 @@ __device__ __forceinline__ int sym5(int i, int j) {  // packed index of the symm
 // Voce hardening (harden, various_functions.f90:1439-1491) from f.gdot; writes the
 // new trial strengths to ttau; returns ||ttau_old - ttau_new||^2.
 __device__ __forceinline__ double harden_voxel(const Fields& f, const double* __restrict__ tab, const PhaseDev& P,
-                                               long v, double tdot, double* gamma1_out, bool store_crss) {
-    const long V = f.V;
+                                               const Vox& x, double tdot, double* gamma1_out, bool store_crss) {
     double deltgam = 0.0;
-    for (int is = 0; is < P.nsyst; is++) deltgam += fabs(f.gdot[is * V + v]) * tdot;
-    const double gamma0 = f.gacc[v];
+    for (int is = 0; is < P.nsyst; is++) deltgam += fabs(x.ld(f.gdot, is)) * tdot;
+    const double gamma0 = x.ld(f.gacc, 0);
     const double gamma1 = gamma0 + deltgam;
     *gamma1_out = gamma1;
     const double* hard = tab + P.off_hard;
@@ -179,31 +178,195 @@ __device__ __forceinline__ double harden_voxel(const Fields& f, const double* __
         const double sc = voce * tdot / deltgam;
         for (int k = 0; k < P.nsm[m]; k++, is++) {
             double dtau = 0.0;
-            for (int js = 0; js < P.nsyst; js++) dtau += hard[is * P.nsyst + js] * fabs(f.gdot[js * V + v]);
-            const double t0 = f.crss[(2 * is) * V + v] + dtau * sc;
-            const double t1 = f.crss[(2 * is + 1) * V + v] + dtau * sc;
-            const double o0 = f.ttau[(2 * is) * V + v], o1 = f.ttau[(2 * is + 1) * V + v];
+            for (int js = 0; js < P.nsyst; js++) dtau += hard[is * P.nsyst + js] * fabs(x.ld(f.gdot, js));
+            const double t0 = x.ld(f.crss, 2 * is) + dtau * sc;
+            const double t1 = x.ld(f.crss, 2 * is + 1) + dtau * sc;
+            const double o0 = x.ld(f.ttau, 2 * is), o1 = x.ld(f.ttau, 2 * is + 1);
             e2 += (o0 - t0) * (o0 - t0) + (o1 - t1) * (o1 - t1);
-            f.ttau[(2 * is) * V + v] = t0;
-            f.ttau[(2 * is + 1) * V + v] = t1;
+            x.st(f.ttau, 2 * is, t0);
+            x.st(f.ttau, 2 * is + 1, t1);
             if (store_crss) {
-                f.crss[(2 * is) * V + v] = t0;
-                f.crss[(2 * is + 1) * V + v] = t1;
+                x.st(f.crss, 2 * is, t0);
+                x.st(f.crss, 2 * is + 1, t1);
             }
         }
     }
     return e2;
 }
 
+// solve_res for a solid voxel without the modified-AL term (various_functions.f90:1142-1324).
+// The reference solves J d = R with J = I + cm (dedotp + s66/tdot) and
+// R = sg - lambda + cm (edotp + s66 (sg - sgt)/tdot - D).  Both factor through cm:
+// J = cm K, R = cm R' with the symmetric positive definite K = cm^-1 + s66/tdot + dedotp
+// and R' = A sg - b + edotp, A = cm^-1 + s66/tdot, b = cm^-1 lambda + s66 sgt/tdot + D,
+// so the Newton step d = K^-1 R' is the reference's step, and |R| = |cm R'| drives
+// the same line search.  Only the packed symmetric parts of cm and s66 (21 entries
+// each) are kept, and K is factored as L D L^T: the live state of the Newton loop is
+// about half of the J/LU form, which doubles the waves each SIMD can hold.
+template <bool INTPOW>
+__device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTables& pt, const PassConst& pc,
+                                                const Vox& x, double* sgf, double* acc) {
+    const PhaseDev& P = pt.ph[f.phase[x.off / 8u]];
+    const double* tab = pt.data;
+    const double tdot = pc.tdot;
+    double cmS[21], A[21], b[6];
+    {
+        double lam[6], sgt[6], D[6], L[9];
+        for (int c = 0; c < 6; c++) {
+            lam[c] = x.ld(f.sg, c);
+            sgt[c] = x.ld(f.sgt, c);
+        }
+        for (int c = 0; c < 9; c++) L[c] = x.ld(f.L, c);
+        to6(L, D);
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+#pragma unroll
+            for (int j = i; j < 6; j++)
+                cmS[sp6(i, j)] = i == j ? x.ld(f.c66m, 7 * i) : 0.5 * (x.ld(f.c66m, 6 * i + j) + x.ld(f.c66m, 6 * j + i));
+        {
+            double Lf[21];
+            for (int i = 0; i < 21; i++) Lf[i] = cmS[i];
+            ldl6(Lf);
+#pragma unroll
+            for (int j = 0; j < 6; j++) {  // columns of cm^-1 (upper part kept)
+                double e[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                e[j] = 1.0;
+                ldl6_solve(Lf, e);
+#pragma unroll
+                for (int i = 0; i <= j; i++) A[sp6(i, j)] = e[i];
+            }
+        }
+        spmv6(A, lam, b);
+        for (int i = 0; i < 6; i++) b[i] += D[i];
+        // + s66/tdot (symmetric part) into A, and s66 sgt/tdot into b
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+#pragma unroll
+            for (int j = i; j < 6; j++) {
+                const double sij = i == j ? x.ld(f.sg66, 7 * i) / tdot
+                                          : 0.5 * (x.ld(f.sg66, 6 * i + j) + x.ld(f.sg66, 6 * j + i)) / tdot;
+                A[sp6(i, j)] += sij;
+                b[i] += sij * sgt[j];
+                if (j != i) b[j] += sij * sgt[i];
+            }
+    }
+    // The stress of the last evaluation is sgold after the line search, unless the
+    // search ended on a cut (icut = 10); that rare stress is parked in the edotp
+    // columns (written with the final edotp below) and flagged.
+    double sg[6], sgold[6] = {0, 0, 0, 0, 0, 0}, ep[5], dd[15], rp[6];
+    bool parked = false;
+    for (int c = 0; c < 6; c++) sg[c] = x.ld(f.sg, c);
+    double sgnorm = 0.0, taun = 0.0;
+    for (int c = 0; c < 6; c++) sgnorm += sg[c] * sg[c];
+    sgnorm = sqrt(sgnorm);
+    for (int k = 0; k < 2 * P.nsyst; k++) {
+        const double t = x.ld(f.ttau, k);
+        taun += t * t;
+    }
+    taun = sqrt(taun);
+    const double tol = fmax(pc.error, pc.erroral);
+    int itertau = 0;
+    double errtau = 2.0 * tol;
+    int nnr = 0, nev = 0, nout = 0;
+    while (itertau < 10 && errtau > tol) {
+        itertau++;
+        nout++;
+        int iternr = 0;
+        double erral = 2.0 * pc.erroral, resoldn = 1.0;
+        while (iternr < 100 && erral > pc.erroral) {
+            iternr++;
+            nnr++;
+            int icut = 0;
+            double resn = 2.0 * resoldn;
+            while (resn > resoldn && icut < 10) {
+                nev++;
+                parked = false;
+                slip_eval<INTPOW, true>(f, tab, P, x, sg, ep, dd);
+                spmv6(A, sg, rp);
+#pragma unroll
+                for (int i = 0; i < 6; i++) rp[i] += (i < 5 ? ep[i] : 0.0) - b[i];
+                double rn = 0.0;
+#pragma unroll
+                for (int i = 0; i < 6; i++) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 6; k++) s += cmS[sp6(i, k)] * rp[k];
+                    rn += s * s;
+                }
+                resn = sqrt(rn);
+                if (iternr == 1) resoldn = resn;
+                if (resn > resoldn) {
+                    for (int c = 0; c < 6; c++) x.st(f.edotp, c, sg[c]);
+                    parked = true;
+                    for (int c = 0; c < 6; c++) sg[c] = sgold[c] + (sg[c] - sgold[c]) * 0.5;
+                }
+                icut++;
+            }
+            resoldn = resn;
+            for (int c = 0; c < 6; c++) sgold[c] = sg[c];
+            double K[21];
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+#pragma unroll
+                for (int j = i; j < 6; j++) K[sp6(i, j)] = A[sp6(i, j)] + (j < 5 ? dd[sym5(i, j)] : 0.0);
+            ldl6(K);
+            ldl6_solve(K, rp);
+            double d1 = 0.0;
+            for (int c = 0; c < 6; c++) {
+                sg[c] -= rp[c];
+                d1 += (sg[c] - sgold[c]) * (sg[c] - sgold[c]);
+            }
+            erral = sqrt(d1) / sgnorm;
+        }
+        // harden with the slip rates of the last evaluation (stored by slip_eval)
+        if (pc.iuphard == 1) {
+            double g1;
+            const double e2 = harden_voxel(f, tab, P, x, tdot, &g1, false);
+            errtau = sqrt(e2) / taun;
+        } else {
+            errtau = 0.0;
+        }
+    }
+    // errors: dsgnorm = |sg - lambda|^2, ddgnorm = |edot(last evaluation) - D|^2
+    double dsgn = 0.0, dd2 = 0.0;
+    {
+        double lam[6], sgt[6], L[9], D[6];
+        for (int c = 0; c < 6; c++) {
+            lam[c] = x.ld(f.sg, c);
+            sgt[c] = x.ld(f.sgt, c);
+        }
+        for (int c = 0; c < 9; c++) L[c] = x.ld(f.L, c);
+        to6(L, D);
+        for (int c = 0; c < 6; c++) dsgn += (sg[c] - lam[c]) * (sg[c] - lam[c]);
+        double sev[6];
+        for (int c = 0; c < 6; c++) sev[c] = parked ? x.ld(f.edotp, c) : sgold[c];
+        for (int i = 0; i < 6; i++) {
+            double s = 0.0;
+            for (int k = 0; k < 6; k++) s += x.ld(f.sg66, 6 * i + k) * ((sev[k] - sgt[k]) / tdot);
+            const double e = (i < 5 ? ep[i] : 0.0) + s - D[i];
+            dd2 += e * e;
+        }
+    }
+    const double w = x.ld(f.w, 0);
+    acc[R_ERRS] = dsgn * w;
+    acc[R_ERRE] = dd2 * w;
+    for (int c = 0; c < 6; c++) sgf[c] = sg[c];
+    for (int c = 0; c < 5; c++) x.st(f.edotp, c, ep[c]);
+    x.st(f.edotp, 5, 0.0);
+    acc[R_NR] = (double)nnr;
+    acc[R_EVAL] = (double)nev;
+    acc[R_OUTER] = (double)nout;
+}
+
 template <bool MODAL, bool INTPOW>
-__global__ __launch_bounds__(BS) void k_correct_solve(Fields f, PhaseTables pt, SolveArgs a) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 2))) void k_correct_solve(Fields f, PhaseTables pt, SolveArgs a) {
     const long V = f.V;
     const long v = (long)blockIdx.x * BS + threadIdx.x;
     const bool act = v < V;
     const PassConst& pc = a.pc;
+    const Vox x(v, V);
     // ---- phase A: correction_to_current + dL_iter/mixing; its 31 sums are reduced
     // before the constitutive solve so their accumulators are not live across it
-    double D[6];
     double w = 0.0;
     bool gas = false;
     {
@@ -220,19 +383,19 @@ __global__ __launch_bounds__(BS) void k_correct_solve(Fields f, PhaseTables pt, 
             }
             double L[9], Lo[9], Fi[9], t[9];
             for (int c = 0; c < 9; c++) {
-                L[c] = f.L[c * V + v];
-                Fi[c] = f.Fi[c * V + v];
-                t[c] = f.dLref[c * V + v] + dud[c];
+                L[c] = x.ld(f.L, c);
+                Fi[c] = x.ld(f.Fi, c);
+                t[c] = x.ld(f.dLref, c) + dud[c];
             }
             if (pc.lold_same) {
                 for (int c = 0; c < 9; c++) Lo[c] = L[c];
             } else {
-                for (int c = 0; c < 9; c++) Lo[c] = f.Lold[c * V + v];
+                for (int c = 0; c < 9; c++) Lo[c] = x.ld(f.Lold, c);
             }
             double q[9];
             mm3(t, Fi, q);
             for (int c = 0; c < 9; c++) L[c] += q[c];
-            w = f.w[v];
+            w = x.ld(f.w, 0);
             gas = f.gas[v] != 0;
             for (int c = 0; c < 9; c++) {
                 accA[R_LALL - R_NPHASEB + c] = L[c] * w;
@@ -251,8 +414,7 @@ __global__ __launch_bounds__(BS) void k_correct_solve(Fields f, PhaseTables pt, 
             } else {
                 for (int c = 0; c < 9; c++) L[c] = pc.mix * L[c];
             }
-            for (int c = 0; c < 9; c++) f.L[c * V + v] = L[c];
-            to6(L, D);
+            for (int c = 0; c < 9; c++) x.st(f.L, c, L[c]);
         }
         block_reduce_store_at<R_NSOLVE - R_NPHASEB, BS>(accA, f.part, R_NSOLVE, R_NPHASEB);
     }
@@ -261,22 +423,29 @@ __global__ __launch_bounds__(BS) void k_correct_solve(Fields f, PhaseTables pt, 
 #pragma unroll
     for (int i = 0; i < R_NPHASEB; i++) acc[i] = 0.0;
     if (act) {
-        double lam[6], sgt[6];
-        for (int c = 0; c < 6; c++) {
-            lam[c] = f.sg[c * V + v];
-            sgt[c] = f.sgt[c * V + v];
-        }
         const double tdot = pc.tdot;
         double sgf[6];
-        if (!gas) {
+        if (!gas && !MODAL) {
+            solve_solid_sym<INTPOW>(f, pt, pc, x, sgf, acc);
+        } else if (!gas) {
+            double lam[6], sgt[6], D[6];
+            {
+                double L[9];
+                for (int c = 0; c < 9; c++) L[c] = x.ld(f.L, c);
+                to6(L, D);
+            }
+            for (int c = 0; c < 6; c++) {
+                lam[c] = x.ld(f.sg, c);
+                sgt[c] = x.ld(f.sgt, c);
+            }
             const PhaseDev& P = pt.ph[f.phase[v]];
             const double* tab = pt.data;
             double cm[36], Jb[36], r0[6];
-            for (int i = 0; i < 36; i++) cm[i] = f.c66m[(long)i * V + v];
+            for (int i = 0; i < 36; i++) cm[i] = x.ld(f.c66m, (long)i);
             {
                 // Jb = I + cm s66 / tdot [- cGm];  r0 = -lambda - (cm s66/tdot) sgt - cm D [+ cGm lambda]
                 double s66[36];
-                for (int i = 0; i < 36; i++) s66[i] = f.sg66[(long)i * V + v] / tdot;
+                for (int i = 0; i < 36; i++) s66[i] = x.ld(f.sg66, (long)i) / tdot;
                 mm6(cm, s66, Jb);
                 for (int i = 0; i < 6; i++) {
                     double s = -lam[i];
@@ -288,7 +457,7 @@ __global__ __launch_bounds__(BS) void k_correct_solve(Fields f, PhaseTables pt, 
                     for (int i = 0; i < 6; i++) {
                         double s = 0.0;
                         for (int k = 0; k < 6; k++) {
-                            const double g = f.cgm[(long)(6 * i + k) * V + v];
+                            const double g = x.ld(f.cgm, (long)(6 * i + k));
                             Jb[6 * i + k] -= g;
                             s += g * lam[k];
                         }
@@ -302,8 +471,8 @@ __global__ __launch_bounds__(BS) void k_correct_solve(Fields f, PhaseTables pt, 
             for (int c = 0; c < 6; c++) sgnorm += sg[c] * sg[c];
             sgnorm = sqrt(sgnorm);
             for (int k = 0; k < 2 * P.nsyst; k++) {
-                const double x = f.ttau[(long)k * V + v];
-                taun += x * x;
+                const double tv = x.ld(f.ttau, (long)k);
+                taun += tv * tv;
             }
             taun = sqrt(taun);
             const double tol = fmax(pc.error, pc.erroral);
@@ -325,7 +494,7 @@ __global__ __launch_bounds__(BS) void k_correct_solve(Fields f, PhaseTables pt, 
                     while (resn > resoldn && icut < 10) {
                         nev++;
                         for (int c = 0; c < 6; c++) sev[c] = sg[c];
-                        slip_eval<INTPOW, false>(f, tab, P, v, sg, ep, dd);
+                        slip_eval<INTPOW, false>(f, tab, P, x, sg, ep, dd);
                         double rn = 0.0;
 #pragma unroll
                         for (int i = 0; i < 6; i++) {
@@ -365,10 +534,10 @@ __global__ __launch_bounds__(BS) void k_correct_solve(Fields f, PhaseTables pt, 
                     erral = sqrt(d1) / sgnorm;
                 }
                 // harden with the slip rates of the last evaluation
-                slip_rates<INTPOW>(f, tab, P, v, sev);
+                slip_rates<INTPOW>(f, tab, P, x, sev);
                 if (pc.iuphard == 1) {
                     double g1;
-                    const double e2 = harden_voxel(f, tab, P, v, tdot, &g1, false);
+                    const double e2 = harden_voxel(f, tab, P, x, tdot, &g1, false);
                     errtau = sqrt(e2) / taun;
                 } else {
                     errtau = 0.0;
@@ -381,7 +550,7 @@ __global__ __launch_bounds__(BS) void k_correct_solve(Fields f, PhaseTables pt, 
             }
             for (int i = 0; i < 6; i++) {
                 double s = 0.0;
-                for (int k = 0; k < 6; k++) s += f.sg66[(long)(6 * i + k) * V + v] * ((sev[k] - sgt[k]) / tdot);
+                for (int k = 0; k < 6; k++) s += x.ld(f.sg66, (long)(6 * i + k)) * ((sev[k] - sgt[k]) / tdot);
                 s66sev[i] = s;
             }
             for (int i = 0; i < 6; i++) {
@@ -391,31 +560,41 @@ __global__ __launch_bounds__(BS) void k_correct_solve(Fields f, PhaseTables pt, 
             acc[R_ERRS] = dsgn * w;
             acc[R_ERRE] = dd2 * w;
             for (int c = 0; c < 6; c++) sgf[c] = sg[c];
-            for (int c = 0; c < 5; c++) f.edotp[c * V + v] = ep[c];
-            f.edotp[5 * V + v] = 0.0;
+            for (int c = 0; c < 5; c++) x.st(f.edotp, c, ep[c]);
+            x.st(f.edotp, 5, 0.0);
             acc[R_NR] = (double)nnr;
             acc[R_EVAL] = (double)nev;
             acc[R_OUTER] = (double)nout;
         } else {
             // gas voxel (various_functions.f90:1326-1345)
+            double lam[6], sgt[6], D[6];
+            {
+                double L[9];
+                for (int c = 0; c < 9; c++) L[c] = x.ld(f.L, c);
+                to6(L, D);
+            }
+            for (int c = 0; c < 6; c++) {
+                lam[c] = x.ld(f.sg, c);
+                sgt[c] = x.ld(f.sgt, c);
+            }
             double rr[6], tt[6];
             for (int i = 0; i < 6; i++) tt[i] = D[i] + sgt[i] / tdot * pc.complgas;
             for (int i = 0; i < 6; i++) {
                 double s = lam[i];
-                for (int k = 0; k < 6; k++) s += f.c66m[(long)(6 * i + k) * V + v] * tt[k];
+                for (int k = 0; k < 6; k++) s += x.ld(f.c66m, (long)(6 * i + k)) * tt[k];
                 if (MODAL)
-                    for (int k = 0; k < 6; k++) s -= f.cgm[(long)(6 * i + k) * V + v] * lam[k];
+                    for (int k = 0; k < 6; k++) s -= x.ld(f.cgm, (long)(6 * i + k)) * lam[k];
                 rr[i] = s;
             }
             for (int i = 0; i < 6; i++) {
                 double s = 0.0;
-                for (int k = 0; k < 6; k++) s += f.cgas[(long)(6 * i + k) * V + v] * rr[k];
+                for (int k = 0; k < 6; k++) s += x.ld(f.cgas, (long)(6 * i + k)) * rr[k];
                 sgf[i] = s;
             }
-            for (int c = 0; c < 6; c++) f.edotp[c * V + v] = 0.0;
-            for (int is = 0; is < f.ns; is++) f.gdot[is * V + v] = 0.0;
+            for (int c = 0; c < 6; c++) x.st(f.edotp, c, 0.0);
+            for (int is = 0; is < f.ns; is++) x.st(f.gdot, is, 0.0);
         }
-        for (int c = 0; c < 6; c++) f.sg[c * V + v] = sgf[c];
+        for (int c = 0; c < 6; c++) x.st(f.sg, c, sgf[c]);
         double s3[9];
         from6(sgf, s3);
         for (int c = 0; c < 9; c++) {
@@ -854,7 +1033,7 @@ __global__ __launch_bounds__(BS) void k_hardening(Fields f, PhaseTables pt, doub
     if (v >= V || f.gas[v]) return;
     const PhaseDev& P = pt.ph[f.phase[v]];
     double g1;
-    harden_voxel(f, pt.data, P, v, tdot, &g1, true);
+    harden_voxel(f, pt.data, P, Vox(v, V), tdot, &g1, true);
     f.gacc[v] = g1;
 }
 void launch_hardening(const Fields& f, const PhaseTables& pt, double tdot, hipStream_t s) {
@@ -918,7 +1097,7 @@ __global__ __launch_bounds__(BS) void k_calc_c0(Fields f, PhaseTables pt, double
         double sg[6], ep[5], dd[15], T[36];
         for (int c = 0; c < 6; c++) sg[c] = f.sg[c * V + v];
         // plastic_strain_rate on a local copy (gdot is not modified: STORE=false)
-        slip_eval<INTPOW, false>(f, pt.data, P, v, sg, ep, dd);
+        slip_eval<INTPOW, false>(f, pt.data, P, Vox(v, V), sg, ep, dd);
         for (int i = 0; i < 6; i++)
             for (int j = 0; j < 6; j++) {
                 double e = f.sg66[(long)(6 * i + j) * V + v] / tdot * tdot / tdotref;

@@ -150,6 +150,69 @@ CG_HD bool solve6(double* A, double* b) {
     }
     return ok;
 }
+// ---- symmetric 6x6, packed upper triangle (21 entries, row by row) ----------
+CG_HD constexpr int sp6(int i, int j) {
+    return i <= j ? i * 6 - (i * (i - 1)) / 2 + (j - i) : j * 6 - (j * (j - 1)) / 2 + (i - j);
+}
+// packed symmetric part of a full row-major 6x6: (A + A^T)/2
+CG_HD void sym_pack6(const double* A, double* P) {
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+        for (int j = i; j < 6; j++) P[sp6(i, j)] = i == j ? A[7 * i] : 0.5 * (A[6 * i + j] + A[6 * j + i]);
+}
+// y = P x
+CG_HD void spmv6(const double* P, const double* x, double* y) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; j++) s += P[sp6(i, j)] * x[j];
+        y[i] = s;
+    }
+}
+// In-place L D L^T factorisation of a packed symmetric 6x6 (no pivoting; the
+// matrices factored here are positive definite).  Afterwards P[sp6(i,i)] = d_i and
+// P[sp6(i,j)] = L_ij (i > j).  Returns false if a pivot is not positive.
+CG_HD bool ldl6(double* P) {
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        double d = P[sp6(j, j)];
+#pragma unroll
+        for (int k = 0; k < j; k++) d -= P[sp6(j, k)] * P[sp6(j, k)] * P[sp6(k, k)];
+        if (!(d > 0.0)) ok = false;
+        P[sp6(j, j)] = d;
+        const double r = 1.0 / d;
+#pragma unroll
+        for (int i = j + 1; i < 6; i++) {
+            double s = P[sp6(i, j)];
+#pragma unroll
+            for (int k = 0; k < j; k++) s -= P[sp6(i, k)] * P[sp6(j, k)] * P[sp6(k, k)];
+            P[sp6(i, j)] = s * r;
+        }
+    }
+    return ok;
+}
+// solve (L D L^T) x = b in place with the factors of ldl6
+CG_HD void ldl6_solve(const double* P, double* b) {
+#pragma unroll
+    for (int i = 1; i < 6; i++) {
+        double s = b[i];
+#pragma unroll
+        for (int k = 0; k < i; k++) s -= P[sp6(i, k)] * b[k];
+        b[i] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) b[i] /= P[sp6(i, i)];
+#pragma unroll
+    for (int i = 4; i >= 0; i--) {
+        double s = b[i];
+#pragma unroll
+        for (int k = i + 1; k < 6; k++) s -= P[sp6(k, i)] * b[k];
+        b[i] = s;
+    }
+}
 // LU factorisation with partial pivoting of a 6x6 (in place; unit lower L)
 CG_HD bool lu6(double* A, int* piv) {
     bool ok = true;

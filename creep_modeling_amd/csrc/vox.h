@@ -1,0 +1,38 @@
+// vox.h -- per-voxel access to the component-major (SoA) fields through buffer
+// resources.
+//
+// A field component column is a[c*V + v].  Addressed as a plain pointer, every
+// column of every field costs a 64-bit per-lane address, and the compiler keeps
+// (or spills) one address pair per column it loads.  A buffer resource built from
+// the column base is wave-uniform (scalar registers), so each lane needs only its
+// 32-bit byte offset v*8, shared by every column of every field.  The resource's
+// record count is the column length: an access outside the column reads 0 and
+// a store there is dropped.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cg {
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+struct Vox {
+    uint32_t off;  // v * 8
+    uint32_t nb;   // V * 8 (bytes of one column)
+    long V;
+
+    __device__ __forceinline__ Vox(long v, long V_) : off((uint32_t)v * 8u), nb((uint32_t)(V_ * 8)), V(V_) {}
+
+    __device__ __forceinline__ static __amdgpu_buffer_rsrc_t rs(const void* p, uint32_t nbytes) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)nbytes, 0x00020000);
+    }
+    // component c of field a at this voxel
+    __device__ __forceinline__ double ld(const double* a, long c) const {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs(a + c * V, nb), (int)off, 0, 0));
+    }
+    __device__ __forceinline__ void st(double* a, long c, double x) const {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), rs(a + c * V, nb), (int)off, 0, 0);
+    }
+};
+
+}  // namespace cg
