@@ -220,7 +220,7 @@ int cg_create(int n1, int n2, int n3, int nph, double complgas, int modal, int d
 static void free_fields(cg_ctx* c) {
     Fields& f = c->f;
     void* ptrs[] = {f.phase, f.gas, f.w, f.sg, f.sgt, f.L, f.Lold, f.dLref, f.F, f.Fi, f.J, f.Fe, f.Fp, f.ag,
-                    f.sg66, f.c66m, f.cgm, f.cgas, f.sch, f.crss, f.ttau, f.gdot, f.gacc, f.edotp, f.ept, f.dgs,
+                    f.sg66, f.c66m, f.cgm, f.cgas, f.sch, f.schA, f.crss, f.ttau, f.gdot, f.gacc, f.edotp, f.ept, f.dgs,
                     f.epvm, f.Finc, f.Feinc, f.s_L, f.s_Lold, f.s_sg, f.s_edotp, f.s_gdot, f.s_ttau, f.spec,
                     f.specf, f.xnode, f.part, f.red};
     for (void* p : ptrs)
@@ -357,7 +357,7 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     rc |= dalloc(c, &f.sg66, 36 * V); rc |= dalloc(c, &f.c66m, 36 * V);
     if (c->modal) rc |= dalloc(c, &f.cgm, 36 * V);
     if (c->ngas) rc |= dalloc(c, &f.cgas, 36 * V);
-    rc |= dalloc(c, &f.sch, 5 * ns * V); rc |= dalloc(c, &f.crss, 2 * ns * V); rc |= dalloc(c, &f.ttau, 2 * ns * V);
+    rc |= dalloc(c, &f.sch, 5 * ns * V); rc |= dalloc(c, &f.schA, 18 * V); rc |= dalloc(c, &f.crss, 2 * ns * V); rc |= dalloc(c, &f.ttau, 2 * ns * V);
     rc |= dalloc(c, &f.gdot, ns * V); rc |= dalloc(c, &f.gacc, V);
     rc |= dalloc(c, &f.edotp, 6 * V); rc |= dalloc(c, &f.ept, 6 * V); rc |= dalloc(c, &f.dgs, 6 * V);
     rc |= dalloc(c, &f.epvm, V); rc |= dalloc(c, &f.Finc, 9 * V); rc |= dalloc(c, &f.Feinc, 9 * V);

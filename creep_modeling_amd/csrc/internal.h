@@ -38,6 +38,7 @@ struct Fields {
     double *F, *Fi, *J, *Fe, *Fp, *ag;  // 9,9,1,9,9,9
     double *sg66, *c66m, *cgm, *cgas;   // 36 each (cgm only if modal, cgas only if gas voxels)
     double *sch, *crss, *ttau, *gdot, *gacc;  // 5ns, 2ns, 2ns, ns, 1
+    double *schA;         // 18: A = Fe ag and A^-1 of update_schmid (sch_s = sym(A b_s x A^-T n_s))
     double *edotp, *ept, *dgs, *epvm;   // 6,6,6,1
     double *Finc, *Feinc;               // 9,9
     double *s_L, *s_Lold, *s_sg, *s_edotp, *s_gdot, *s_ttau;  // snapshots

@@ -15,6 +15,17 @@ namespace cg {
 
 constexpr int BS = 256;
 
+// Phase tables seen through the constant address space: with a wave-uniform phase id
+// and uniform indices (readfirstlane) the compiler reads them with scalar loads, which
+// neither occupy vector registers nor join the in-order vector-memory counter.
+typedef __attribute__((address_space(4))) const double cdbl;
+typedef __attribute__((address_space(4))) const PhaseDev cphase;
+template <bool UNI>
+__device__ __forceinline__ int uidx(int i) {
+    if constexpr (UNI) return __builtin_amdgcn_readfirstlane(i);
+    else return i;
+}
+
 static inline int nblk(long V) { return (int)((V + BS - 1) / BS); }
 
 // |x|^(n-1) for an integer n (exponentiation by squaring) or through pow().
@@ -106,6 +117,26 @@ struct SolveArgs {
     const double* fluct;  // 9 reduced sums of w dLref Fi
 };
 
+// Deviatoric b-basis components of sym(A (b x n) A^-1) = sym((A b) x (A^-T n)) for one
+// slip system (update_schmid, various_functions.f90:472-539); A = Fe ag.
+__device__ __forceinline__ void schmid5(const double* A, const double* Ai, const double* b3, const double* n3,
+                                        double* s5) {
+    double ab[3], an[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        ab[i] = A[3 * i] * b3[0] + A[3 * i + 1] * b3[1] + A[3 * i + 2] * b3[2];
+        an[i] = Ai[i] * n3[0] + Ai[3 + i] * n3[1] + Ai[6 + i] * n3[2];
+    }
+    double m[9], s6[6];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) m[3 * i + j] = ab[i] * an[j];
+    to6(m, s6);
+#pragma unroll
+    for (int c = 0; c < 5; c++) s5[c] = s6[c];
+}
+
 // Power-law slip rates at sigma (b-basis) -> edotp (5), dedotp (sym 5x5, 15), optional gdot store.
 template <bool INTPOW, bool STORE>
 __device__ __forceinline__ void slip_eval(const Fields& f, const double* __restrict__ tab, const PhaseDev& P,
@@ -127,6 +158,59 @@ __device__ __forceinline__ void slip_eval(const Fields& f, const double* __restr
         const double r1 = n * dum / tau;
         const double r2 = rss * dum;
         if (STORE) x.st(f.gdot, is, r2);
+        ep[0] += s0 * r2; ep[1] += s1 * r2; ep[2] += s2 * r2; ep[3] += s3 * r2; ep[4] += s4 * r2;
+        const double t0 = s0 * r1, t1 = s1 * r1, t2 = s2 * r1, t3 = s3 * r1, t4 = s4 * r1;
+        dd[0] += s0 * t0; dd[1] += s0 * t1; dd[2] += s0 * t2; dd[3] += s0 * t3; dd[4] += s0 * t4;
+        dd[5] += s1 * t1; dd[6] += s1 * t2; dd[7] += s1 * t3; dd[8] += s1 * t4;
+        dd[9] += s2 * t2; dd[10] += s2 * t3; dd[11] += s2 * t4;
+        dd[12] += s3 * t3; dd[13] += s3 * t4;
+        dd[14] += s4 * t4;
+    }
+}
+// slip_eval for the symmetric Newton solve: the Schmid tensors are rebuilt from A, A^-1
+// (this thread's column of an LDS array [18][BS]) and the wave-uniform crystal b, n, so
+// an evaluation reads 2 n_s trial strengths from memory instead of 5 n_s Schmid
+// components; the next system's strengths are loaded while the current one is computed.
+template <bool INTPOW, class PH, class TB>
+__device__ __forceinline__ void slip_eval_ab(const Fields& f, TB tab, const PH& P, const Vox& x, const double* sal,
+                                             const double* sg, double* ep, double* dd) {
+    double A[9], Ai[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        A[k] = sal[k * BS];
+        Ai[k] = sal[(9 + k) * BS];
+    }
+    for (int i = 0; i < 5; i++) ep[i] = 0.0;
+    for (int i = 0; i < 15; i++) dd[i] = 0.0;
+    const TB nrs = tab + P.off_nrs;
+    const TB g0 = tab + P.off_gamd0;
+    const TB db = tab + P.off_dbca;
+    const TB dn = tab + P.off_dnca;
+    const int ns = P.nsyst;
+    double tpn = x.ld(f.ttau, 0), tnn = x.ld(f.ttau, 1);
+#pragma unroll 1
+    for (int is0 = 0; is0 < ns; is0++) {
+        const int is = __builtin_amdgcn_readfirstlane(is0);
+        const double tp = tpn, tn = tnn;
+        if (is + 1 < ns) {
+            tpn = x.ld(f.ttau, 2 * is + 2);
+            tnn = x.ld(f.ttau, 2 * is + 3);
+        }
+        double sv[5], b3[3], n3[3];
+        for (int k = 0; k < 3; k++) {
+            b3[k] = db[3 * is + k];
+            n3[k] = dn[3 * is + k];
+        }
+        schmid5(A, Ai, b3, n3, sv);
+        const double s0 = sv[0], s1 = sv[1], s2 = sv[2], s3 = sv[3], s4 = sv[4];
+        double rss = s0 * sg[0] + s1 * sg[1] + s2 * sg[2] + s3 * sg[3] + s4 * sg[4];
+        const double tau = rss < 0.0 ? tn : tp;
+        rss = rss / tau;
+        const double n = nrs[is];
+        const double dum = g0[is] * powm1<INTPOW>(fabs(rss), n);
+        const double r1 = n * dum / tau;
+        const double r2 = rss * dum;
+        x.st(f.gdot, is, r2);
         ep[0] += s0 * r2; ep[1] += s1 * r2; ep[2] += s2 * r2; ep[3] += s3 * r2; ep[4] += s4 * r2;
         const double t0 = s0 * r1, t1 = s1 * r1, t2 = s2 * r1, t3 = s3 * r1, t4 = s4 * r1;
         dd[0] += s0 * t0; dd[1] += s0 * t1; dd[2] += s0 * t2; dd[3] += s0 * t3; dd[4] += s0 * t4;
@@ -160,25 +244,46 @@ __device__ __forceinline__ int sym5(int i, int j) {  // packed index of the symm
 
 // Voce hardening (harden, various_functions.f90:1439-1491) from f.gdot; writes the
 // new trial strengths to ttau; returns ||ttau_old - ttau_new||^2.
-__device__ __forceinline__ double harden_voxel(const Fields& f, const double* __restrict__ tab, const PhaseDev& P,
-                                               const Vox& x, double tdot, double* gamma1_out, bool store_crss) {
+template <bool UNI = false, class PH = PhaseDev, class TB = const double*>
+__device__ __forceinline__ double harden_voxel(const Fields& f, TB tab, const PH& P, const Vox& x, double tdot,
+                                               double* gamma1_out, bool store_crss) {
+    const int ns = P.nsyst;
+    constexpr int NR = 12;  // up to this many systems the |gdot| stay in registers
+    double gr[NR];
+    const bool inreg = ns <= NR;
     double deltgam = 0.0;
-    for (int is = 0; is < P.nsyst; is++) deltgam += fabs(x.ld(f.gdot, is)) * tdot;
+    if (inreg) {
+#pragma unroll
+        for (int js = 0; js < NR; js++) gr[js] = js < ns ? fabs(x.ld(f.gdot, js)) : 0.0;
+#pragma unroll
+        for (int js = 0; js < NR; js++)
+            if (js < ns) deltgam += gr[js] * tdot;
+    } else {
+        for (int is = 0; is < ns; is++) deltgam += fabs(x.ld(f.gdot, is)) * tdot;
+    }
     const double gamma0 = x.ld(f.gacc, 0);
     const double gamma1 = gamma0 + deltgam;
     *gamma1_out = gamma1;
-    const double* hard = tab + P.off_hard;
+    const TB hard = tab + P.off_hard;
     double e2 = 0.0;
     int is = 0;
-    for (int m = 0; m < P.nmodes; m++) {
+    for (int m0 = 0; m0 < P.nmodes; m0++) {
+        const int m = uidx<UNI>(m0);
         const double tau1 = P.mode_tau1[m], th0 = P.mode_thet0[m], th1 = P.mode_thet1[m];
         const double fact = fabs(th0 / tau1);
         const double ex0 = exp(-fact * gamma0), ex1 = exp(-fact * gamma1);
         const double voce = deltgam * th1 + tau1 * (ex0 - ex1) + th1 * (gamma0 * ex0 - gamma1 * ex1);
         const double sc = voce * tdot / deltgam;
         for (int k = 0; k < P.nsm[m]; k++, is++) {
+            is = uidx<UNI>(is);
             double dtau = 0.0;
-            for (int js = 0; js < P.nsyst; js++) dtau += hard[is * P.nsyst + js] * fabs(x.ld(f.gdot, js));
+            if (inreg) {
+#pragma unroll
+                for (int js = 0; js < NR; js++)
+                    if (js < ns) dtau += hard[is * ns + js] * gr[js];
+            } else {
+                for (int js = 0; js < ns; js++) dtau += hard[is * ns + js] * fabs(x.ld(f.gdot, js));
+            }
             const double t0 = x.ld(f.crss, 2 * is) + dtau * sc;
             const double t1 = x.ld(f.crss, 2 * is + 1) + dtau * sc;
             const double o0 = x.ld(f.ttau, 2 * is), o1 = x.ld(f.ttau, 2 * is + 1);
@@ -204,12 +309,21 @@ __device__ __forceinline__ double harden_voxel(const Fields& f, const double* __
 // each) are kept, and K is factored as L D L^T: the live state of the Newton loop is
 // about half of the J/LU form, which doubles the waves each SIMD can hold.
 template <bool INTPOW>
+// p0 is wave-uniform (the caller iterates over the phases present in the wave), so the
+// phase tables are read through scalar loads.
+// cml: this thread's column of a block-shared LDS array [21][BS] holding the packed
+// symmetric cm, read once per evaluation for the residual norm (keeping it out of the
+// registers lets three waves share a SIMD).
+// sal: this thread's column of an LDS array [18][BS] that receives A, A^-1 (schA).
 __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTables& pt, const PassConst& pc,
-                                                const Vox& x, double* sgf, double* acc) {
-    const PhaseDev& P = pt.ph[f.phase[x.off / 8u]];
-    const double* tab = pt.data;
+                                                const Vox& x, int p0, double* cml, double* sal, double* sgf,
+                                                double* acc) {
+    const cphase& P = ((cphase*)pt.ph)[p0];
+    cdbl* tab = (cdbl*)pt.data;
     const double tdot = pc.tdot;
-    double cmS[21], A[21], b[6];
+#pragma unroll
+    for (int k = 0; k < 18; k++) sal[k * BS] = x.ld(f.schA, k);
+    double A[21], b[6];
     {
         double lam[6], sgt[6], D[6], L[9];
         for (int c = 0; c < 6; c++) {
@@ -218,14 +332,16 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
         }
         for (int c = 0; c < 9; c++) L[c] = x.ld(f.L, c);
         to6(L, D);
-#pragma unroll
-        for (int i = 0; i < 6; i++)
-#pragma unroll
-            for (int j = i; j < 6; j++)
-                cmS[sp6(i, j)] = i == j ? x.ld(f.c66m, 7 * i) : 0.5 * (x.ld(f.c66m, 6 * i + j) + x.ld(f.c66m, 6 * j + i));
         {
             double Lf[21];
-            for (int i = 0; i < 21; i++) Lf[i] = cmS[i];
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+#pragma unroll
+                for (int j = i; j < 6; j++)
+                    Lf[sp6(i, j)] = i == j ? x.ld(f.c66m, 7 * i)
+                                           : 0.5 * (x.ld(f.c66m, 6 * i + j) + x.ld(f.c66m, 6 * j + i));
+#pragma unroll
+            for (int i = 0; i < 21; i++) cml[i * BS] = Lf[i];
             ldl6(Lf);
 #pragma unroll
             for (int j = 0; j < 6; j++) {  // columns of cm^-1 (upper part kept)
@@ -281,7 +397,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
             while (resn > resoldn && icut < 10) {
                 nev++;
                 parked = false;
-                slip_eval<INTPOW, true>(f, tab, P, x, sg, ep, dd);
+                slip_eval_ab<INTPOW>(f, tab, P, x, sal, sg, ep, dd);
                 spmv6(A, sg, rp);
 #pragma unroll
                 for (int i = 0; i < 6; i++) rp[i] += (i < 5 ? ep[i] : 0.0) - b[i];
@@ -290,7 +406,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
                 for (int i = 0; i < 6; i++) {
                     double s = 0.0;
 #pragma unroll
-                    for (int k = 0; k < 6; k++) s += cmS[sp6(i, k)] * rp[k];
+                    for (int k = 0; k < 6; k++) s += cml[sp6(i, k) * BS] * rp[k];
                     rn += s * s;
                 }
                 resn = sqrt(rn);
@@ -321,7 +437,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
         // harden with the slip rates of the last evaluation (stored by slip_eval)
         if (pc.iuphard == 1) {
             double g1;
-            const double e2 = harden_voxel(f, tab, P, x, tdot, &g1, false);
+            const double e2 = harden_voxel<true>(f, tab, P, x, tdot, &g1, false);
             errtau = sqrt(e2) / taun;
         } else {
             errtau = 0.0;
@@ -365,6 +481,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 
     const bool act = v < V;
     const PassConst& pc = a.pc;
     const Vox x(v, V);
+    // LDS of the symmetric solve: per thread 21 (packed cm) + 18 (A, A^-1) doubles;
+    // the block reductions reuse it when no solve is using it
+    __shared__ double pool[MODAL ? 1 : 39 * BS];
+    double* const cm_lds = pool;
+    double* const sa_lds = pool + 21 * BS;
     // ---- phase A: correction_to_current + dL_iter/mixing; its 31 sums are reduced
     // before the constitutive solve so their accumulators are not live across it
     double w = 0.0;
@@ -416,7 +537,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 
             }
             for (int c = 0; c < 9; c++) x.st(f.L, c, L[c]);
         }
-        block_reduce_store_at<R_NSOLVE - R_NPHASEB, BS>(accA, f.part, R_NSOLVE, R_NPHASEB);
+        if (MODAL) {
+            block_reduce_store_at<R_NSOLVE - R_NPHASEB, BS>(accA, f.part, R_NSOLVE, R_NPHASEB);
+        } else {  // scratch aliased onto the solve's LDS columns, free until phase B
+            block_reduce_lds<R_NSOLVE - R_NPHASEB, BS>(accA, f.part, R_NSOLVE, R_NPHASEB, pool);
+            __syncthreads();
+        }
     }
     // ---- phase B: solve_res (various_functions.f90:1142-1364)
     double acc[R_NPHASEB];
@@ -426,7 +552,16 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 
         const double tdot = pc.tdot;
         double sgf[6];
         if (!gas && !MODAL) {
-            solve_solid_sym<INTPOW>(f, pt, pc, x, sgf, acc);
+            // one pass per distinct phase among the wave's solid lanes (one for a
+            // single-phase RVE); the phase id is then uniform inside the solve
+            const int ph = f.phase[v];
+            for (;;) {
+                const int p0 = __builtin_amdgcn_readfirstlane(ph);
+                if (ph == p0) {
+                    solve_solid_sym<INTPOW>(f, pt, pc, x, p0, cm_lds + threadIdx.x, sa_lds + threadIdx.x, sgf, acc);
+                    break;
+                }
+            }
         } else if (!gas) {
             double lam[6], sgt[6], D[6];
             {
@@ -603,7 +738,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 
         }
     }
     __syncthreads();
-    block_reduce_store_at<R_NPHASEB, BS>(acc, f.part, R_NSOLVE, 0);
+    if (MODAL) block_reduce_store_at<R_NPHASEB, BS>(acc, f.part, R_NSOLVE, 0);
+    else block_reduce_lds<R_NPHASEB, BS>(acc, f.part, R_NSOLVE, 0, pool);  // every solve has finished
 }
 
 // Public launcher with the full argument set.
@@ -638,18 +774,14 @@ __global__ __launch_bounds__(BS) void k_update_schmid(Fields f, PhaseTables pt) 
     for (int c = 0; c < 9; c++) { Fe[c] = f.Fe[c * V + v]; ag[c] = f.ag[c * V + v]; }
     mm3(Fe, ag, A);
     inv3(A, Ai);
+    for (int c = 0; c < 9; c++) {
+        f.schA[c * V + v] = A[c];
+        f.schA[(9 + c) * V + v] = Ai[c];
+    }
     for (int is = 0; is < P.nsyst; is++) {
-        // A (b x n) A^-1 = (A b) x (A^-T n)
-        double ab[3], an[3];
-        for (int i = 0; i < 3; i++) {
-            ab[i] = A[3 * i] * db[3 * is] + A[3 * i + 1] * db[3 * is + 1] + A[3 * i + 2] * db[3 * is + 2];
-            an[i] = Ai[i] * dn[3 * is] + Ai[3 + i] * dn[3 * is + 1] + Ai[6 + i] * dn[3 * is + 2];
-        }
-        double m[9], s6[6];
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) m[3 * i + j] = ab[i] * an[j];
-        to6(m, s6);
-        for (int c = 0; c < 5; c++) f.sch[(5 * is + c) * V + v] = s6[c];
+        double s5[5];
+        schmid5(A, Ai, db + 3 * is, dn + 3 * is, s5);
+        for (int c = 0; c < 5; c++) f.sch[(5 * is + c) * V + v] = s5[c];
     }
 }
 void launch_update_schmid(const Fields& f, const PhaseTables& pt, hipStream_t s) {
@@ -1373,6 +1505,7 @@ __global__ __launch_bounds__(BS) void k_init_voxels(Fields f, PhaseTables pt, co
         for (int k = 0; k < f.ns; k++) f.gdot[(long)k * V + v] = 0.0;
         for (int k = 0; k < 2 * f.ns; k++) { f.crss[(long)k * V + v] = 0.0; f.ttau[(long)k * V + v] = 0.0; }
         for (int k = 0; k < 5 * f.ns; k++) f.sch[(long)k * V + v] = 0.0;
+        for (int k = 0; k < 18; k++) f.schA[(long)k * V + v] = 0.0;
         if (!f.gas[v]) {
             const PhaseDev& P = pt.ph[f.phase[v]];
             // Bunge angles -> a (sample -> crystal), tensor_functions.f90:525-569; ag = a^T

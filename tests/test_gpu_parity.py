@@ -63,6 +63,22 @@ def test_fcc_increments_match_oracle():
         assert relerr(fo[k], fg[k]) < FIELD_TOL, k
 
 
+def test_xnode_matches_oracle():
+    # nodal coordinates after two increments (update_grid_velgrad_node, various_functions.f90:1970-2056):
+    # the displacement from the initial grid agrees to the field tolerance
+    p = problem(0, n=8, ngrains=4, nsteps=2)
+    rows, fo, fg = run_pair(p, 2, fields=("xnode",))
+    check_rows(rows)
+    n1, n2, n3 = p.n
+    # node (i, j, k) at xnode[3*(i + (n1+1)*(j + (n2+1)*k)) + c], starting at (i, j, k) + 1/2
+    kk, jj, ii = np.meshgrid(np.arange(n3 + 1), np.arange(n2 + 1), np.arange(n1 + 1), indexing="ij")
+    x0 = np.stack([ii, jj, kk], axis=-1).astype(float) + 0.5
+    xo = fo["xnode"].reshape(n3 + 1, n2 + 1, n1 + 1, 3)
+    xg = fg["xnode"].reshape(n3 + 1, n2 + 1, n1 + 1, 3)
+    assert np.abs(xo - x0).max() > 1e-6  # the grid has moved
+    assert relerr(xo - x0, xg - x0) < FIELD_TOL
+
+
 def test_diagnostics_match_oracle():
     rows, _, _ = run_pair(problem(0, n=8, ngrains=4, nsteps=2), 2, fields=("sg",))
     for ro, rg in rows:

@@ -403,3 +403,42 @@ def test_loop_determinism_and_bc_satisfaction():
     last = rows[0][-1]
     assert last["errsbc"] <= 1e-6 or last["iter"] == 200
     assert abs(last["s33"] / 1.0 - 300.0 * 1.0) / 300.0 < 1e-3 or last["iter"] == 200
+
+
+# --------------------------------------------------------------------------- hardening
+def voce_tau(G, tau1, th0, th1):
+    """Extended Voce law: tau(G) - tau0 = (tau1 + th1 G)(1 - exp(-th0 G / tau1))."""
+    return (tau1 + th1 * G) * (1.0 - np.exp(-th0 * G / tau1))
+
+
+def test_harden_matches_voce_closed_form_and_is_additive():
+    # harden (various_functions.f90:1439-1491) integrates the Voce law exactly over one
+    # increment: with equal |gamdot| and an all-ones hardening matrix,
+    # tau_new - crss = tau_V(G0 + dG) - tau_V(G0).  Three increments chained equal one.
+    p = make_problem(n=4, ngrains=2, nsteps=1)
+    o = bind.Oracle(p)
+    ns, V, v = 12, p.nvox, 3
+    tdot = p.processes[0].tdot
+    tau1, th0, th1 = 40.0, 500.0, 50.0
+    G0, g = 0.013, 2.5e-4
+    gd = np.zeros((V, ns)); gd[v] = g * np.where(np.arange(ns) % 2, 1.0, -1.0)
+    cr = np.zeros((V, ns, 2)); cr[v] = 123.0
+    ga = np.zeros(V); ga[v] = G0
+    o.set_field("gamdot", gd.ravel()); o.set_field("crss", cr.ravel()); o.set_field("gacumgr", ga)
+    tau = np.zeros(2 * ns); g1 = np.zeros(1)
+    o.L.orc_harden(o.h, v, bind.dp(tau), bind.dp(g1))
+    dG = ns * g * tdot
+    assert g1[0] == pytest.approx(G0 + dG, rel=1e-15)
+    want = 123.0 + voce_tau(G0 + dG, tau1, th0, th1) - voce_tau(G0, tau1, th0, th1)
+    np.testing.assert_allclose(tau, want, rtol=1e-12)
+    # additivity: three steps of dG from G0 reach the same strength as tau_V at G0 + 3 dG
+    crss = 123.0
+    G = G0
+    for _ in range(3):
+        cr[v] = crss; ga[v] = G
+        o.set_field("crss", cr.ravel()); o.set_field("gacumgr", ga)
+        o.L.orc_harden(o.h, v, bind.dp(tau), bind.dp(g1))
+        crss, G = tau[0], g1[0]
+    want3 = 123.0 + voce_tau(G0 + 3 * dG, tau1, th0, th1) - voce_tau(G0, tau1, th0, th1)
+    assert crss == pytest.approx(want3, rel=1e-12)
+    o.close()
