@@ -19,6 +19,7 @@
 #include "internal.h"
 #include "reduce.h"
 #include "tensor.h"
+#include "fft_reg.h"
 
 namespace cg {
 
@@ -471,6 +472,7 @@ struct FftCtx {
     double2* tw64;
     float2* tw32;
     int rows, ty, tz;
+    int use_reg;  // register-FFT kernels for the per-pass stage (all axis lengths supported)
 };
 
 FftCtx* fft_create(const Fields& f, char* err, int errlen) {
@@ -513,6 +515,7 @@ FftCtx* fft_create(const Fields& f, char* err, int errlen) {
     while ((long)(f.n2) * f.n3 % c->rows) c->rows /= 2;
     c->ty = 2;
     c->tz = 2;
+    c->use_reg = 1;
     return c;
 }
 void fft_destroy(FftCtx* c) {
@@ -543,11 +546,24 @@ static int run_green(const Fields& f, FftCtx* c, const double* sgsum, const Gree
     return nrow;
 }
 
+bool fft_reg_supported(const Fields& f);
+template <typename R>
+int run_green_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
+                  const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, double scale, hipStream_t s);
+
 int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
               int mixed, double scale, hipStream_t s) {
     GreenDev g;
     for (int i = 0; i < 81; i++) { g.c0[i] = c0[i]; g.s0[i] = s0[i]; }
     g.igamma = igamma;
+    if (c->use_reg && fft_reg_supported(f)) {
+        const Plan3& pl = c->plan;
+        if (mixed)
+            return run_green_reg<float>(f, c->tw32 + pl.ax[0].tw_off, c->tw32 + pl.ax[1].tw_off,
+                                        c->tw32 + pl.ax[2].tw_off, sgsum_dev, g, scale, s);
+        return run_green_reg<double>(f, c->tw64 + pl.ax[0].tw_off, c->tw64 + pl.ax[1].tw_off,
+                                     c->tw64 + pl.ax[2].tw_off, sgsum_dev, g, 1.0, s);
+    }
     if (mixed) return run_green<float>(f, c, sgsum_dev, g, scale, s);
     return run_green<double>(f, c, sgsum_dev, g, 1.0, s);
 }
@@ -718,5 +734,326 @@ int fft_integrate(const Fields& f, FftCtx* c, int ideriv, double* dv, hipStream_
     hipLaunchKernelGGL(k_x_inv3, dim3(nrow), dim3(FBS), shx, s, f, c->plan, rows, dv);
     return nrow;
 }
+
+}  // namespace cg
+
+// ===========================================================================
+// Register-FFT form of the FFT + Green stage (axis lengths with a Shape<N>):
+// every transform is level-1 DFT in registers -> one LDS exchange -> level-2 DFT.
+// Same five passes and half-spectrum layout as above; two real components share one
+// complex transform in the x passes.
+// ===========================================================================
+
+namespace cg {
+
+constexpr int RBS = 256;  // threads per block: x and y passes
+constexpr int ZBS = 288;  // threads per block: z pass (whole kx columns of 9 components)
+
+template <int N> struct RegDims {
+    static constexpr int T = Shape<N>::T, E = N / T, U = (E + T - 1) / T;
+    static constexpr int AN = U * T > E ? U * T : E;  // register slots per thread
+    static constexpr int NS = RBS / T;                 // signals per x / y block
+    static constexpr int RW = NS / 5;                  // rows per x block (5 signals a row)
+    static constexpr int NSZ = ZBS / T;                // signals per z block
+    static constexpr int KT = NSZ / 9;                 // kx columns per z block
+};
+
+template <typename R> __device__ __forceinline__ typename Cx<R>::T* spec_of(const Fields& f) {
+    return (typename Cx<R>::T*)(sizeof(R) == 8 ? (void*)f.spec : (void*)f.specf);
+}
+
+// x forward: PK1 = J (sigma - <sigma>) F^-T (LS-EVPFFT.f90:256-269), r2c along x.
+// Components (2p, 2p+1) of a row form one complex signal; the two half spectra are
+// separated with X[N-k] afterwards.
+template <int N, typename R>
+__global__ __launch_bounds__(RBS) void k_x_fwd_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
+                                                   const double* __restrict__ sgsum, double inv_scale) {
+    using T2 = typename Cx<R>::T;
+    using D = RegDims<N>;
+    constexpr int T = D::T, E = D::E, U = D::U, NS = D::NS, RW = D::RW;
+    __shared__ T2 ex[N * NS];
+    R* stage = (R*)ex;  // RW*9*N reals, dead before the exchange
+    static_assert(RW * 9 * N * sizeof(R) <= N * NS * sizeof(T2), "stage fits the exchange");
+    const long V = f.V;
+    const long nrows = (long)f.n2 * f.n3;
+    const long row0 = (long)blockIdx.x * RW;
+    const int h1 = f.h1;
+    double sa[9];
+    for (int c = 0; c < 9; c++) sa[c] = sgsum[c];
+    for (int e = threadIdx.x; e < RW * N; e += RBS) {
+        const int r = e / N, i = e - r * N;
+        const long row = row0 + r;
+        double pk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double J = 0.0;
+        if (row < nrows) {
+            const long v = row * N + i;
+            double s6[6], s[9], Fi[9];
+            for (int c = 0; c < 6; c++) s6[c] = f.sg[c * V + v];
+            for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * V + v];
+            from6(s6, s);
+            J = f.J[v];
+            for (int c = 0; c < 9; c++) s[c] -= sa[c];
+            mmt3(s, Fi, pk);  // (sigma - <sigma>) F^-T
+        }
+        for (int c = 0; c < 9; c++) stage[(r * 9 + c) * N + i] = (R)(pk[c] * J * inv_scale);
+    }
+    __syncthreads();
+    const int s = threadIdx.x % NS, t = threadIdx.x / NS;
+    const int r = s / 5, p = s - 5 * r;
+    const bool act = r < RW;
+    T2 a[D::AN];
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        const int i = t + T * m;
+        R re = 0, im = 0;
+        if (act) {
+            re = stage[(r * 9 + 2 * p) * N + i];
+            if (p < 4) im = stage[(r * 9 + 2 * p + 1) * N + i];
+        }
+        a[m] = T2{re, im};
+    }
+    __syncthreads();
+    fft_dist<N, T, false>(a, s, t, NS, ex, tw);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int q = t + T * u;
+        if (U == 1 || q < E)
+#pragma unroll
+            for (int j = 0; j < T; j++) ex[s * N + q + E * j] = a[u * T + j];
+    }
+    __syncthreads();
+    T2* spec = spec_of<R>(f);
+    const int w = h1 * 9;
+    for (int e = threadIdx.x; e < RW * w; e += RBS) {
+        const int rr = e / w, rem = e - rr * w;
+        const int kx = rem / 9, c = rem - kx * 9;
+        const long row = row0 + rr;
+        if (row >= nrows) continue;
+        const int sp = rr * 5 + c / 2;
+        const T2 Z = ex[sp * N + kx], Zm = ex[sp * N + (N - kx) % N];
+        T2 o;
+        if ((c & 1) == 0) o = T2{(R)0.5 * (Z.x + Zm.x), (R)0.5 * (Z.y - Zm.y)};  // (Z + conj Zm)/2
+        else o = T2{(R)0.5 * (Z.y + Zm.y), (R)-0.5 * (Z.x - Zm.x)};           // (Z - conj Zm)/(2i)
+        spec[row * w + rem] = o;
+    }
+}
+
+// y pass over every (kx, c) column of one z plane
+template <int N, bool INV, typename R>
+__global__ __launch_bounds__(RBS) void k_y_reg(Fields f, const typename Cx<R>::T* __restrict__ tw, int ntile) {
+    using T2 = typename Cx<R>::T;
+    using D = RegDims<N>;
+    constexpr int T = D::T, E = D::E, U = D::U, NS = D::NS;
+    __shared__ T2 ex[N * NS];
+    const int k = blockIdx.x / ntile, tile = blockIdx.x - k * ntile;
+    const int ncol = f.h1 * 9;
+    const int s = threadIdx.x % NS, t = threadIdx.x / NS;
+    const int col = tile * NS + s;
+    const bool act = col < ncol;
+    T2* base = spec_of<R>(f) + (long)k * N * ncol + col;
+    T2 a[D::AN];
+#pragma unroll
+    for (int m = 0; m < E; m++) a[m] = act ? base[(long)(t + T * m) * ncol] : T2{0, 0};
+    fft_dist<N, T, INV>(a, s, t, NS, ex, tw);
+    if (act)
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int q = t + T * u;
+            if (U == 1 || q < E)
+#pragma unroll
+                for (int j = 0; j < T; j++) base[(long)(q + E * j) * ncol] = a[u * T + j];
+        }
+}
+
+// z pass with the Green operator: KT whole kx columns (9 components each) of one y row
+template <int N, typename R, bool GENERAL>
+__global__ __launch_bounds__(ZBS) void k_z_green_reg(Fields f, const typename Cx<R>::T* __restrict__ tw, GreenDev g,
+                                                     int ntile) {
+    using T2 = typename Cx<R>::T;
+    using D = RegDims<N>;
+    constexpr int T = D::T, E = D::E, U = D::U, NS = D::NSZ, KT = D::KT;
+    __shared__ T2 ex[N * NS];
+    const int h1 = f.h1;
+    const int j = blockIdx.x / ntile, tile = blockIdx.x - j * ntile;
+    const int kx0 = tile * KT;
+    const int KTe = min(KT, h1 - kx0);
+    const int s = threadIdx.x % NS, t = threadIdx.x / NS;
+    const bool act = s < 9 * KTe;
+    const long ps = (long)f.n2 * h1 * 9;
+    T2* base = spec_of<R>(f) + ((long)j * h1 + kx0) * 9 + s;
+    T2 a[D::AN];
+#pragma unroll
+    for (int m = 0; m < E; m++) a[m] = act ? base[(long)(t + T * m) * ps] : T2{0, 0};
+    fft_dist<N, T, false>(a, s, t, NS, ex, tw);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int q = t + T * u;
+        if (U == 1 || q < E)
+#pragma unroll
+            for (int jj = 0; jj < T; jj++) ex[(q + E * jj) * NS + s] = a[u * T + jj];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < KTe * N; e += ZBS) {
+        const int kz = e / KTe, kxl = e - kz * KTe;
+        green_apply<R, GENERAL>(g, kx0 + kxl, j, kz, f.n1, f.n2, N, ex + kz * NS + kxl * 9, 1);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; m++) a[m] = ex[(t + T * m) * NS + s];
+    __syncthreads();
+    fft_dist<N, T, true>(a, s, t, NS, ex, tw);
+    if (act)
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int q = t + T * u;
+            if (U == 1 || q < E)
+#pragma unroll
+                for (int jj = 0; jj < T; jj++) base[(long)(q + E * jj) * ps] = a[u * T + jj];
+        }
+}
+
+// x inverse (c2r): dLref = Re(IFFT)/V with the Hermitian continuation of the half
+// spectrum (imaginary parts at kx = 0 and n1/2 dropped, as Re() of the reference's
+// c2c inverse does); block partial sums of w dLref F^-1.
+template <int N, typename R>
+__global__ __launch_bounds__(RBS) void k_x_inv_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
+                                                   double out_scale) {
+    using T2 = typename Cx<R>::T;
+    using D = RegDims<N>;
+    constexpr int T = D::T, E = D::E, U = D::U, NS = D::NS, RW = D::RW;
+    constexpr int H = N / 2 + 1;
+    __shared__ T2 ex[N * NS];
+    static_assert(RW * H * 9 <= N * NS, "spectrum rows fit the exchange");
+    static_assert(RW * 9 * N * sizeof(R) <= N * NS * sizeof(T2), "output rows fit the exchange");
+    const long V = f.V;
+    const long nrows = (long)f.n2 * f.n3;
+    const long row0 = (long)blockIdx.x * RW;
+    const int w = H * 9;
+    const T2* spec = spec_of<R>(f);
+    for (int e = threadIdx.x; e < RW * w; e += RBS) {
+        const long row = row0 + e / w;
+        ex[e] = row < nrows ? spec[row0 * w + e] : T2{0, 0};
+    }
+    __syncthreads();
+    const int s = threadIdx.x % NS, t = threadIdx.x / NS;
+    const int r = s / 5, p = s - 5 * r;
+    const bool act = r < RW;
+    T2 a[D::AN];
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        const int k = t + T * m;
+        const int kk = k <= N / 2 ? k : N - k;
+        T2 A = T2{0, 0}, B = T2{0, 0};
+        if (act) {
+            A = ex[(r * H + kk) * 9 + 2 * p];
+            if (p < 4) B = ex[(r * H + kk) * 9 + 2 * p + 1];
+            if (kk == 0 || 2 * kk == N) { A.y = 0; B.y = 0; }
+            if (k > N / 2) { A.y = -A.y; B.y = -B.y; }
+        }
+        a[m] = T2{A.x - B.y, A.y + B.x};  // A + i B
+    }
+    __syncthreads();
+    fft_dist<N, T, true>(a, s, t, NS, ex, tw);
+    __syncthreads();
+    R* out = (R*)ex;
+    if (act)
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int q = t + T * u;
+            if (U == 1 || q < E)
+#pragma unroll
+                for (int jj = 0; jj < T; jj++) {
+                    const T2 z = a[u * T + jj];
+                    const int n = q + E * jj;
+                    out[(r * 9 + 2 * p) * N + n] = z.x;
+                    if (p < 4) out[(r * 9 + 2 * p + 1) * N + n] = z.y;
+                }
+        }
+    __syncthreads();
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int e = threadIdx.x; e < RW * N; e += RBS) {
+        const int rr = e / N, i = e - rr * N;
+        const long row = row0 + rr;
+        if (row >= nrows) continue;
+        const long v = row * N + i;
+        double d[9], Fi[9], q9[9];
+        for (int c = 0; c < 9; c++) {
+            d[c] = (double)out[(rr * 9 + c) * N + i] * out_scale;
+            f.dLref[c * V + v] = d[c];
+            Fi[c] = f.Fi[c * V + v];
+        }
+        mm3(d, Fi, q9);
+        const double wv = f.w[v];
+        for (int c = 0; c < 9; c++) acc[c] += q9[c] * wv;
+    }
+    block_reduce_store<9, RBS>(acc, f.part);
+}
+
+static bool reg_size(int n) { return n == 16 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128 || n == 256; }
+bool fft_reg_supported(const Fields& f) { return reg_size(f.n1) && reg_size(f.n2) && reg_size(f.n3); }
+
+template <int N, typename R> static void launch_x_fwd_reg(const Fields& f, const typename Cx<R>::T* tw, const double* sgsum,
+                                                          double inv_scale, hipStream_t s) {
+    const long nrows = (long)f.n2 * f.n3;
+    const int nb = (int)((nrows + RegDims<N>::RW - 1) / RegDims<N>::RW);
+    hipLaunchKernelGGL((k_x_fwd_reg<N, R>), dim3(nb), dim3(RBS), 0, s, f, tw, sgsum, inv_scale);
+}
+template <int N, typename R> static int launch_x_inv_reg(const Fields& f, const typename Cx<R>::T* tw, double out_scale,
+                                                         hipStream_t s) {
+    const long nrows = (long)f.n2 * f.n3;
+    const int nb = (int)((nrows + RegDims<N>::RW - 1) / RegDims<N>::RW);
+    hipLaunchKernelGGL((k_x_inv_reg<N, R>), dim3(nb), dim3(RBS), 0, s, f, tw, out_scale);
+    return nb;
+}
+template <int N, bool INV, typename R> static void launch_y_reg(const Fields& f, const typename Cx<R>::T* tw, hipStream_t s) {
+    const int ntile = (f.h1 * 9 + RegDims<N>::NS - 1) / RegDims<N>::NS;
+    hipLaunchKernelGGL((k_y_reg<N, INV, R>), dim3(f.n3 * ntile), dim3(RBS), 0, s, f, tw, ntile);
+}
+template <int N, typename R> static void launch_z_reg(const Fields& f, const typename Cx<R>::T* tw, const GreenDev& g,
+                                                      hipStream_t s) {
+    const int ntile = (f.h1 + RegDims<N>::KT - 1) / RegDims<N>::KT;
+    if (g.igamma == 0) hipLaunchKernelGGL((k_z_green_reg<N, R, false>), dim3(f.n2 * ntile), dim3(ZBS), 0, s, f, tw, g, ntile);
+    else hipLaunchKernelGGL((k_z_green_reg<N, R, true>), dim3(f.n2 * ntile), dim3(ZBS), 0, s, f, tw, g, ntile);
+}
+
+#define CG_SWITCH_N(n, CALL)                                             \
+    switch (n) {                                                         \
+        case 16: CALL(16); break;                                        \
+        case 32: CALL(32); break;                                        \
+        case 48: CALL(48); break;                                        \
+        case 64: CALL(64); break;                                        \
+        case 96: CALL(96); break;                                        \
+        case 128: CALL(128); break;                                      \
+        case 256: CALL(256); break;                                      \
+        default: break;                                                  \
+    }
+
+template <typename R>
+int run_green_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
+                  const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, double scale, hipStream_t s) {
+    int nb = -1;
+#define CG_XF(N) launch_x_fwd_reg<N, R>(f, tw1, sgsum, 1.0 / scale, s)
+#define CG_YF(N) launch_y_reg<N, false, R>(f, tw2, s)
+#define CG_Z(N) launch_z_reg<N, R>(f, tw3, g, s)
+#define CG_YI(N) launch_y_reg<N, true, R>(f, tw2, s)
+#define CG_XI(N) nb = launch_x_inv_reg<N, R>(f, tw1, scale / (double)f.V, s)
+    CG_SWITCH_N(f.n1, CG_XF)
+    CG_SWITCH_N(f.n2, CG_YF)
+    CG_SWITCH_N(f.n3, CG_Z)
+    CG_SWITCH_N(f.n2, CG_YI)
+    CG_SWITCH_N(f.n1, CG_XI)
+#undef CG_XF
+#undef CG_YF
+#undef CG_Z
+#undef CG_YI
+#undef CG_XI
+    return nb;
+}
+template int run_green_reg<double>(const Fields&, const double2*, const double2*, const double2*, const double*,
+                                   const GreenDev&, double, hipStream_t);
+template int run_green_reg<float>(const Fields&, const float2*, const float2*, const float2*, const double*,
+                                  const GreenDev&, double, hipStream_t);
 
 }  // namespace cg

@@ -89,6 +89,14 @@ def test_diagnostics_match_oracle():
         assert abs(ro["cond_c066mod_max_sol"] - rg["cond_c066mod_max_sol"]) <= 1e-6 * ro["cond_c066mod_max_sol"]
 
 
+def test_anisotropic_grid_matches_oracle():
+    # a different transform length on each axis (x 32, y 16, z 48: radix-2 and radix-3 sizes)
+    rows, fo, fg = run_pair(problem(0, n=(32, 16, 48), ngrains=6, nsteps=1), 1)
+    check_rows(rows)
+    for k in fo:
+        assert relerr(fo[k], fg[k]) < FIELD_TOL, k
+
+
 def test_bcc48_increment_matches_oracle():
     p = problem(3, n=8, ngrains=4, nsteps=2)
     rows, fo, fg = run_pair(p, 2)
