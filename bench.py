@@ -27,6 +27,23 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by what the
+# same counters read for a 1 GiB device copy); the file records the commands and raw counts.
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_fcc128.json"
+
+
+def measured_traffic(stage, n, grains):
+    """HBM bytes per launch of `stage` for this workload from the committed counter runs, or None."""
+    try:
+        d = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    e = d.get("stages", {}).get(stage)
+    if not e or d.get("n") != n or d.get("grains") != grains:
+        return None
+    return e["bytes_per_launch"]
+
+
 # algorithmic bytes per voxel-iteration of the FFT + Green stage: read the 9-component
 # FP64 PK1 field once, write the 9-component FP64 correction once, plus 2P-2 = 4
 # half-spectrum passes of 9 complex values (P = 3 when one component's complex xy slab fits
@@ -89,8 +106,9 @@ def cpu_baseline(problem, max_passes=3):
 
 
 def copy_calibration(gib=1.0, reps=5):
-    """A plain device copy of known size (torch's elementwise copy kernel), to scale the
-    FETCH_SIZE / WRITE_SIZE counters: each launch reads and writes exactly `gib` GiB."""
+    """A plain device copy of known size (torch's device-to-device copy, which runs the HIP
+    runtime's copy kernel), to scale the FETCH_SIZE / WRITE_SIZE counters: each launch
+    reads and writes exactly `gib` GiB."""
     import torch
 
     n = int(gib * (1 << 30)) // 8
@@ -148,10 +166,11 @@ def main():
     if dominant_solve:
         roof = dict(bound="mfma", kernel="k_correct_solve (constitutive Newton, FP64 vector ALU)",
                     achieved=solve_tflops, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=solve_tflops / FP64_PEAK_TFLOPS,
-                    traffic=None, avg_ms=t_solve)
+                    traffic=measured_traffic("k_correct_solve", args.n, args.grains), avg_ms=t_solve)
     else:
         roof = dict(bound="hbm", kernel="FFT + Green stage", achieved=fft_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=fft_gbs / HBM_PEAK_GBS, traffic=None, avg_ms=t_fft)
+                    frac=fft_gbs / HBM_PEAK_GBS, traffic=measured_traffic("fft_green", args.n, args.grains),
+                    avg_ms=t_fft)
     out = {
         "metric": "voxel-iters/s at 128^3 creep RVE (mixed vs FP64 GPU) + % HBM roofline",
         "value": value,
@@ -173,7 +192,8 @@ def main():
                                "reduce_copy": st.t_ms[2] / max(st.passes, 1),
                                "per_increment_updates_ms": st.t_ms[3] / max(args.steps, 1)},
         "fft_green_roofline": {"bound": "hbm", "achieved": fft_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                               "frac": fft_gbs / HBM_PEAK_GBS, "alg_bytes_per_voxel": fft_alg_bytes(args.n)},
+                               "frac": fft_gbs / HBM_PEAK_GBS, "alg_bytes_per_voxel": fft_alg_bytes(args.n),
+                               "traffic": measured_traffic("fft_green", args.n, args.grains)},
         "constitutive": {"tflops": solve_tflops, "frac_fp64_peak": solve_tflops / FP64_PEAK_TFLOPS,
                          "nr_steps_per_voxel_iter": st.nr_steps / max(st.voxel_iters, 1),
                          "evals_per_voxel_iter": st.evals / max(st.voxel_iters, 1),

@@ -202,6 +202,8 @@ int cg_create(int n1, int n2, int n3, int nph, double complgas, int modal, int d
     if (!out) return -1;
     *out = nullptr;
     if (n1 < 2 || n2 < 1 || n3 < 1 || nph < 1 || nph > MAXPH) return -1;
+    // field columns are addressed with 32-bit byte offsets (v * 8)
+    if ((long)n1 * n2 * n3 >= (1L << 29)) return -1;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return -4;
     cg_ctx* c = new cg_ctx();
@@ -255,7 +257,7 @@ int cg_set_phase(cg_ctx* c, int iph, int igas, int nsyst, int nmodes, const int*
     P.igas = igas;
     P.set = true;
     if (igas) return 0;
-    if (nsyst < 1 || nmodes < 1 || nmodes > MAXMODES) return fail(c, -1, "bad phase sizes");
+    if (nsyst < 1 || nsyst > MAXSYS || nmodes < 1 || nmodes > MAXMODES) return fail(c, -1, "bad phase sizes");
     int tot = 0;
     for (int m = 0; m < nmodes; m++) { P.nsm[m] = nsm[m]; tot += nsm[m]; }
     if (tot != nsyst) return fail(c, -1, "sum(nsm) != nsyst");

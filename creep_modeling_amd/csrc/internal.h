@@ -8,6 +8,7 @@ namespace cg {
 
 constexpr int MAXPH = 4;
 constexpr int MAXMODES = 8;
+constexpr int MAXSYS = 48;  // slip systems per phase
 
 // Per-phase constants, resident in device memory (read-only in kernels).
 struct PhaseDev {

@@ -182,32 +182,29 @@ __device__ __forceinline__ void slip_eval_ab(const Fields& f, TB tab, const PH& 
     }
     for (int i = 0; i < 5; i++) ep[i] = 0.0;
     for (int i = 0; i < 15; i++) dd[i] = 0.0;
-    const TB nrs = tab + P.off_nrs;
-    const TB g0 = tab + P.off_gamd0;
-    const TB db = tab + P.off_dbca;
-    const TB dn = tab + P.off_dnca;
     const int ns = P.nsyst;
     double tpn = x.ld(f.ttau, 0), tnn = x.ld(f.ttau, 1);
 #pragma unroll 1
     for (int is0 = 0; is0 < ns; is0++) {
         const int is = __builtin_amdgcn_readfirstlane(is0);
         const double tp = tpn, tn = tnn;
-        if (is + 1 < ns) {
-            tpn = x.ld(f.ttau, 2 * is + 2);
-            tnn = x.ld(f.ttau, 2 * is + 3);
+        {  // unconditional prefetch (clamped to the last system) keeps the wait counts exact
+            const int nx = __builtin_amdgcn_readfirstlane(is + 1 < ns ? is + 1 : is);
+            tpn = x.ld(f.ttau, 2 * nx);
+            tnn = x.ld(f.ttau, 2 * nx + 1);
         }
         double sv[5], b3[3], n3[3];
         for (int k = 0; k < 3; k++) {
-            b3[k] = db[3 * is + k];
-            n3[k] = dn[3 * is + k];
+            b3[k] = tab[__builtin_amdgcn_readfirstlane(P.off_dbca + 3 * is + k)];
+            n3[k] = tab[__builtin_amdgcn_readfirstlane(P.off_dnca + 3 * is + k)];
         }
         schmid5(A, Ai, b3, n3, sv);
         const double s0 = sv[0], s1 = sv[1], s2 = sv[2], s3 = sv[3], s4 = sv[4];
         double rss = s0 * sg[0] + s1 * sg[1] + s2 * sg[2] + s3 * sg[3] + s4 * sg[4];
         const double tau = rss < 0.0 ? tn : tp;
         rss = rss / tau;
-        const double n = nrs[is];
-        const double dum = g0[is] * powm1<INTPOW>(fabs(rss), n);
+        const double n = tab[__builtin_amdgcn_readfirstlane(P.off_nrs + is)];
+        const double dum = tab[__builtin_amdgcn_readfirstlane(P.off_gamd0 + is)] * powm1<INTPOW>(fabs(rss), n);
         const double r1 = n * dum / tau;
         const double r2 = rss * dum;
         x.st(f.gdot, is, r2);
@@ -1435,18 +1432,35 @@ __global__ __launch_bounds__(BS) void k_diagnostics(Fields f, double lfloor, int
         part[19 * (size_t)nb + 4 * (size_t)blockIdx.x + 3] = (double)si[1][0];
     }
 }
-__global__ void k_argmin_final(const double* p, int nb, double* out) {
-    if (threadIdx.x != 0) return;
+// (minimum value, then largest index on ties) over the block candidates; the order is
+// total, so the strided scan and the tree merge give the serial scan's answer
+__device__ __forceinline__ bool argmin_better(double val, double idx, double best, double bi) {
+    return idx >= 0.0 && (val < best || (val == best && idx > bi));
+}
+__global__ __launch_bounds__(BS) void k_argmin_final(const double* p, int nb, double* out) {
+    __shared__ double sv[BS], si[BS];
+    const int t = threadIdx.x;
     for (int k = 0; k < 2; k++) {
-        double best = 1.7976931348623157e308;
-        double bi = -1.0;
-        for (int b = 0; b < nb; b++) {
+        double best = 1.7976931348623157e308, bi = -1.0;
+        for (int b = t; b < nb; b += BS) {
             const double val = p[4 * b + 2 * k], idx = p[4 * b + 2 * k + 1];
-            if (idx < 0) continue;
-            if (val < best || (val == best && idx > bi)) { best = val; bi = idx; }
+            if (argmin_better(val, idx, best, bi)) { best = val; bi = idx; }
         }
-        out[2 * k] = best;
-        out[2 * k + 1] = bi;
+        sv[t] = best;
+        si[t] = bi;
+        __syncthreads();
+        for (int h = BS / 2; h > 0; h >>= 1) {
+            if (t < h && argmin_better(sv[t + h], si[t + h], sv[t], si[t])) {
+                sv[t] = sv[t + h];
+                si[t] = si[t + h];
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            out[2 * k] = sv[0];
+            out[2 * k + 1] = si[0];
+        }
+        __syncthreads();
     }
 }
 void launch_diagnostics(const Fields& f, double lfloor, int scan_cond, hipStream_t s, int* nb) {
@@ -1456,7 +1470,7 @@ void launch_diagnostics(const Fields& f, double lfloor, int scan_cond, hipStream
     hipLaunchKernelGGL(k_reduce_final, dim3(6), dim3(BS), 0, s, f.part, n, 6, f.red);
     hipLaunchKernelGGL(k_reduce_final_op<OpMax>, dim3(12), dim3(BS), 0, s, f.part + 6 * (size_t)n, n, 12, f.red + 6);
     hipLaunchKernelGGL(k_reduce_final_op<OpMin>, dim3(1), dim3(BS), 0, s, f.part + 18 * (size_t)n, n, 1, f.red + 18);
-    hipLaunchKernelGGL(k_argmin_final, dim3(1), dim3(64), 0, s, f.part + 19 * (size_t)n, n, f.red + 19);
+    hipLaunchKernelGGL(k_argmin_final, dim3(1), dim3(BS), 0, s, f.part + 19 * (size_t)n, n, f.red + 19);
 }
 
 // take_snapshot / rollback (:2839-2895)

@@ -270,9 +270,13 @@ class CsvWriter:
         self.res.write_text(RESULTS_HEADER + "\n")
         self.diag.write_text(DIAG_HEADER + "\n")
         zero = [0] * 3 + [0.0] * 3
-        dvm0 = solver.ctx  # initial row: micro averages at init (zeros except dvm)
-        del dvm0
-        res0 = [0, 0, 0] + [0.0] * 4 + [0.0] * 36 + [0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0]
+        # initial row (LS-EVPFFT.f90:176-178): the averages of init_micro are zero except
+        # dvmavg = vm_strain(dsim of process 1) (initial_guess, various_functions.f90:50)
+        d = np.asarray(solver.procs[0].dsim[:], dtype=np.float64).reshape(3, 3)
+        h = (d[0, 0] + d[1, 1] + d[2, 2]) / 3.0
+        dvm0 = math.sqrt(2.0 / 3.0 * ((d[0, 0] - h) ** 2 + (d[1, 1] - h) ** 2 + (d[2, 2] - h) ** 2 + d[0, 1] ** 2
+                                      + d[0, 2] ** 2 + d[1, 2] ** 2 + d[1, 0] ** 2 + d[2, 0] ** 2 + d[2, 1] ** 2))
+        res0 = [0, 0, 0] + [0.0] * 4 + [0.0] * 36 + [0.0, 0.0, dvm0, 0.0, 0.0, 0.0, 0.0, 0.0]
         with open(self.res, "a") as f:
             f.write(",".join(_g0(x) for x in res0) + "\n")
         with open(self.diag, "a") as f:
