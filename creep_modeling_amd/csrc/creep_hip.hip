@@ -137,7 +137,7 @@ struct cg_ctx {
     double* d_sgsum = nullptr;
     double* d_fluct = nullptr;
     size_t part_cap = 0;
-    int sgsum_valid = 0, lold_same = 0, int_pow = 0, ngas = 0, n3node = 1;
+    int sgsum_valid = 0, lold_same = 0, int_pow = 0, sym_tau = 0, ngas = 0, n3node = 1;
     long last_solid = -1;
     double nsys_solid = 0.0;
     // macroscopic state (micro_type, types.f90:55-94)
@@ -202,8 +202,8 @@ int cg_create(int n1, int n2, int n3, int nph, double complgas, int modal, int d
     if (!out) return -1;
     *out = nullptr;
     if (n1 < 2 || n2 < 1 || n3 < 1 || nph < 1 || nph > MAXPH) return -1;
-    // field columns are addressed with 32-bit byte offsets (v * 8)
-    if ((long)n1 * n2 * n3 >= (1L << 29)) return -1;
+    // field columns are addressed through buffer resources over 8 columns: 64 V < 2^32
+    if ((long)n1 * n2 * n3 >= (1L << 26)) return -1;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return -4;
     cg_ctx* c = new cg_ctx();
@@ -301,7 +301,7 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     PhaseDev hp[MAXPH];
     memset(hp, 0, sizeof hp);
     std::vector<double> tab;
-    int all_int = 1;
+    int all_int = 1, all_sym = 1;
     for (int i = 0; i < c->nph; i++) {
         HostPhase& P = c->ph[i];
         PhaseDev& D = hp[i];
@@ -329,8 +329,11 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
         D.off_hpfac = put(P.hpfac);
         for (double n : P.nrs)
             if (!(n == floor(n) && n >= 1.0 && n <= 64.0)) all_int = 0;
+        for (int is = 0; is < P.nsyst; is++)
+            if (P.tau[3 * is] != P.tau[3 * is + 1]) all_sym = 0;
     }
     c->int_pow = all_int;
+    c->sym_tau = all_sym;
     HIPCHK(c, hipMalloc(&c->d_ph, sizeof hp));
     HIPCHK(c, hipMemcpy(c->d_ph, hp, sizeof hp, hipMemcpyHostToDevice));
     if (dalloc(c, &c->d_tab, std::max<size_t>(tab.size(), 1))) return -3;
@@ -524,7 +527,7 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     inv3(c->defgradinvavgc, finv_inv);
     PhaseTables pt{c->d_ph, c->d_tab};
     tstart(c, 1);
-    launch_correct_solve_full(f, pt, pc, c->dvelgradavg, finv_inv, c->d_fluct, c->s, &nb, c->int_pow);
+    launch_correct_solve_full(f, pt, pc, c->dvelgradavg, finv_inv, c->d_fluct, c->s, &nb, c->int_pow, c->sym_tau);
     tstop(c, 1);
     launch_reduce_final(f.part, nb, R_NSOLVE, f.red, c->s);
     // next pass' <sigma> = sum_all w sigma

@@ -171,7 +171,11 @@ __device__ __forceinline__ void slip_eval(const Fields& f, const double* __restr
 // (this thread's column of an LDS array [18][BS]) and the wave-uniform crystal b, n, so
 // an evaluation reads 2 n_s trial strengths from memory instead of 5 n_s Schmid
 // components; the next system's strengths are loaded while the current one is computed.
-template <bool INTPOW, class PH, class TB>
+// SYMTAU: the forward and backward trial strengths are equal (tau0f = tau0b for every
+// system: init and harden then keep the two columns bitwise equal), so one is read.
+// ACC: accumulate edotp and its derivative; STORE: store gdot (the last evaluation's
+// rates feed harden and the per-increment updates).
+template <bool INTPOW, bool SYMTAU, bool ACC, bool STORE, class PH, class TB>
 __device__ __forceinline__ void slip_eval_ab(const Fields& f, TB tab, const PH& P, const Vox& x, const double* sal,
                                              const double* sg, double* ep, double* dd) {
     double A[9], Ai[9];
@@ -180,41 +184,64 @@ __device__ __forceinline__ void slip_eval_ab(const Fields& f, TB tab, const PH& 
         A[k] = sal[k * BS];
         Ai[k] = sal[(9 + k) * BS];
     }
-    for (int i = 0; i < 5; i++) ep[i] = 0.0;
-    for (int i = 0; i < 15; i++) dd[i] = 0.0;
+    if (ACC) {
+        for (int i = 0; i < 5; i++) ep[i] = 0.0;
+        for (int i = 0; i < 15; i++) dd[i] = 0.0;
+    }
     const int ns = P.nsyst;
-    double tpn = x.ld(f.ttau, 0), tnn = x.ld(f.ttau, 1);
+    // Trial strengths are read for a group of G systems at a time, one group ahead: the
+    // loads of group g+1 are in flight while group g computes (they come from HBM: the
+    // per-XCD working set of the solve exceeds L2).  Group indices are clamped to the last
+    // system so that every load is unconditional and the wait counts stay exact.
+    constexpr int G = SYMTAU ? 4 : 2;
+    double tpn[G], tnn[G];
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+        const int c = __builtin_amdgcn_readfirstlane(k < ns ? k : ns - 1);
+        tpn[k] = x.ld(f.ttau, 2 * c);
+        if (!SYMTAU) tnn[k] = x.ld(f.ttau, 2 * c + 1);
+    }
 #pragma unroll 1
-    for (int is0 = 0; is0 < ns; is0++) {
-        const int is = __builtin_amdgcn_readfirstlane(is0);
-        const double tp = tpn, tn = tnn;
-        {  // unconditional prefetch (clamped to the last system) keeps the wait counts exact
-            const int nx = __builtin_amdgcn_readfirstlane(is + 1 < ns ? is + 1 : is);
-            tpn = x.ld(f.ttau, 2 * nx);
-            tnn = x.ld(f.ttau, 2 * nx + 1);
+    for (int g0 = 0; g0 < ns; g0 += G) {
+        const int gb = __builtin_amdgcn_readfirstlane(g0);
+        double tp[G], tn[G];
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            tp[k] = tpn[k];
+            tn[k] = SYMTAU ? tpn[k] : tnn[k];
+            const int c = __builtin_amdgcn_readfirstlane(gb + G + k < ns ? gb + G + k : ns - 1);
+            tpn[k] = x.ld(f.ttau, 2 * c);
+            if (!SYMTAU) tnn[k] = x.ld(f.ttau, 2 * c + 1);
         }
-        double sv[5], b3[3], n3[3];
-        for (int k = 0; k < 3; k++) {
-            b3[k] = tab[__builtin_amdgcn_readfirstlane(P.off_dbca + 3 * is + k)];
-            n3[k] = tab[__builtin_amdgcn_readfirstlane(P.off_dnca + 3 * is + k)];
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            const int is = gb + k;
+            if (is >= ns) break;
+            double sv[5], b3[3], n3[3];
+            for (int q = 0; q < 3; q++) {
+                b3[q] = tab[__builtin_amdgcn_readfirstlane(P.off_dbca + 3 * is + q)];
+                n3[q] = tab[__builtin_amdgcn_readfirstlane(P.off_dnca + 3 * is + q)];
+            }
+            schmid5(A, Ai, b3, n3, sv);
+            const double s0 = sv[0], s1 = sv[1], s2 = sv[2], s3 = sv[3], s4 = sv[4];
+            double rss = s0 * sg[0] + s1 * sg[1] + s2 * sg[2] + s3 * sg[3] + s4 * sg[4];
+            const double tau = rss < 0.0 ? tn[k] : tp[k];
+            rss = rss / tau;
+            const double n = tab[__builtin_amdgcn_readfirstlane(P.off_nrs + is)];
+            const double dum = tab[__builtin_amdgcn_readfirstlane(P.off_gamd0 + is)] * powm1<INTPOW>(fabs(rss), n);
+            const double r2 = rss * dum;
+            if (STORE) x.st(f.gdot, is, r2);
+            if (ACC) {
+                const double r1 = n * dum / tau;
+                ep[0] += s0 * r2; ep[1] += s1 * r2; ep[2] += s2 * r2; ep[3] += s3 * r2; ep[4] += s4 * r2;
+                const double t0 = s0 * r1, t1 = s1 * r1, t2 = s2 * r1, t3 = s3 * r1, t4 = s4 * r1;
+                dd[0] += s0 * t0; dd[1] += s0 * t1; dd[2] += s0 * t2; dd[3] += s0 * t3; dd[4] += s0 * t4;
+                dd[5] += s1 * t1; dd[6] += s1 * t2; dd[7] += s1 * t3; dd[8] += s1 * t4;
+                dd[9] += s2 * t2; dd[10] += s2 * t3; dd[11] += s2 * t4;
+                dd[12] += s3 * t3; dd[13] += s3 * t4;
+                dd[14] += s4 * t4;
+            }
         }
-        schmid5(A, Ai, b3, n3, sv);
-        const double s0 = sv[0], s1 = sv[1], s2 = sv[2], s3 = sv[3], s4 = sv[4];
-        double rss = s0 * sg[0] + s1 * sg[1] + s2 * sg[2] + s3 * sg[3] + s4 * sg[4];
-        const double tau = rss < 0.0 ? tn : tp;
-        rss = rss / tau;
-        const double n = tab[__builtin_amdgcn_readfirstlane(P.off_nrs + is)];
-        const double dum = tab[__builtin_amdgcn_readfirstlane(P.off_gamd0 + is)] * powm1<INTPOW>(fabs(rss), n);
-        const double r1 = n * dum / tau;
-        const double r2 = rss * dum;
-        x.st(f.gdot, is, r2);
-        ep[0] += s0 * r2; ep[1] += s1 * r2; ep[2] += s2 * r2; ep[3] += s3 * r2; ep[4] += s4 * r2;
-        const double t0 = s0 * r1, t1 = s1 * r1, t2 = s2 * r1, t3 = s3 * r1, t4 = s4 * r1;
-        dd[0] += s0 * t0; dd[1] += s0 * t1; dd[2] += s0 * t2; dd[3] += s0 * t3; dd[4] += s0 * t4;
-        dd[5] += s1 * t1; dd[6] += s1 * t2; dd[7] += s1 * t3; dd[8] += s1 * t4;
-        dd[9] += s2 * t2; dd[10] += s2 * t3; dd[11] += s2 * t4;
-        dd[12] += s3 * t3; dd[13] += s3 * t4;
-        dd[14] += s4 * t4;
     }
 }
 // gdot only (for harden and the final store): same arithmetic as slip_eval
@@ -296,6 +323,11 @@ __device__ __forceinline__ double harden_voxel(const Fields& f, TB tab, const PH
     return e2;
 }
 
+// kRatesAfter: instead of storing gdot at every evaluation, evaluate the rates once more
+// after each Newton loop at the last evaluation's stress (the same arithmetic, so the
+// same values) and store them then.
+constexpr bool kRatesAfter = false;
+
 // solve_res for a solid voxel without the modified-AL term (various_functions.f90:1142-1324).
 // The reference solves J d = R with J = I + cm (dedotp + s66/tdot) and
 // R = sg - lambda + cm (edotp + s66 (sg - sgt)/tdot - D).  Both factor through cm:
@@ -305,7 +337,7 @@ __device__ __forceinline__ double harden_voxel(const Fields& f, TB tab, const PH
 // the same line search.  Only the packed symmetric parts of cm and s66 (21 entries
 // each) are kept, and K is factored as L D L^T: the live state of the Newton loop is
 // about half of the J/LU form, which doubles the waves each SIMD can hold.
-template <bool INTPOW>
+template <bool INTPOW, bool SYMTAU>
 // p0 is wave-uniform (the caller iterates over the phases present in the wave), so the
 // phase tables are read through scalar loads.
 // cml: this thread's column of a block-shared LDS array [21][BS] holding the packed
@@ -394,7 +426,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
             while (resn > resoldn && icut < 10) {
                 nev++;
                 parked = false;
-                slip_eval_ab<INTPOW>(f, tab, P, x, sal, sg, ep, dd);
+                slip_eval_ab<INTPOW, SYMTAU, true, !kRatesAfter>(f, tab, P, x, sal, sg, ep, dd);
                 spmv6(A, sg, rp);
 #pragma unroll
                 for (int i = 0; i < 6; i++) rp[i] += (i < 5 ? ep[i] : 0.0) - b[i];
@@ -431,7 +463,12 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
             }
             erral = sqrt(d1) / sgnorm;
         }
-        // harden with the slip rates of the last evaluation (stored by slip_eval)
+        if (kRatesAfter) {  // the rates of the last evaluation, at its stress, stored once
+            double sev[6];
+            for (int c = 0; c < 6; c++) sev[c] = parked ? x.ld(f.edotp, c) : sgold[c];
+            slip_eval_ab<INTPOW, SYMTAU, false, true>(f, tab, P, x, sal, sev, nullptr, nullptr);
+        }
+        // harden with the slip rates of the last evaluation
         if (pc.iuphard == 1) {
             double g1;
             const double e2 = harden_voxel<true>(f, tab, P, x, tdot, &g1, false);
@@ -471,7 +508,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
     acc[R_OUTER] = (double)nout;
 }
 
-template <bool MODAL, bool INTPOW>
+template <bool MODAL, bool INTPOW, bool SYMTAU>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 2))) void k_correct_solve(Fields f, PhaseTables pt, SolveArgs a) {
     const long V = f.V;
     const long v = (long)blockIdx.x * BS + threadIdx.x;
@@ -555,7 +592,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 
             for (;;) {
                 const int p0 = __builtin_amdgcn_readfirstlane(ph);
                 if (ph == p0) {
-                    solve_solid_sym<INTPOW>(f, pt, pc, x, p0, cm_lds + threadIdx.x, sa_lds + threadIdx.x, sgf, acc);
+                    solve_solid_sym<INTPOW, SYMTAU>(f, pt, pc, x, p0, cm_lds + threadIdx.x, sa_lds + threadIdx.x, sgf,
+                                                    acc);
                     break;
                 }
             }
@@ -742,18 +780,21 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 
 // Public launcher with the full argument set.
 void launch_correct_solve_full(const Fields& f, const PhaseTables& pt, const PassConst& pc,
                                const double* dvelgradavg, const double* finv_inv, const double* fluct_dev,
-                               hipStream_t s, int* nb, int int_pow) {
+                               hipStream_t s, int* nb, int int_pow, int sym_tau) {
     SolveArgs a;
     a.pc = pc;
     for (int i = 0; i < 9; i++) { a.dvelgradavg[i] = dvelgradavg[i]; a.finv_inv[i] = finv_inv[i]; }
     a.fluct = fluct_dev;
     *nb = nblk(f.V);
     if (pc.modal) {
-        if (int_pow) hipLaunchKernelGGL((k_correct_solve<true, true>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
-        else hipLaunchKernelGGL((k_correct_solve<true, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
+        if (int_pow) hipLaunchKernelGGL((k_correct_solve<true, true, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
+        else hipLaunchKernelGGL((k_correct_solve<true, false, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
+    } else if (sym_tau) {
+        if (int_pow) hipLaunchKernelGGL((k_correct_solve<false, true, true>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
+        else hipLaunchKernelGGL((k_correct_solve<false, false, true>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
     } else {
-        if (int_pow) hipLaunchKernelGGL((k_correct_solve<false, true>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
-        else hipLaunchKernelGGL((k_correct_solve<false, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
+        if (int_pow) hipLaunchKernelGGL((k_correct_solve<false, true, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
+        else hipLaunchKernelGGL((k_correct_solve<false, false, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
     }
 }
 

@@ -3,11 +3,13 @@
 //
 // A field component column is a[c*V + v].  Addressed as a plain pointer, every
 // column of every field costs a 64-bit per-lane address, and the compiler keeps
-// (or spills) one address pair per column it loads.  A buffer resource built from
-// the column base is wave-uniform (scalar registers), so each lane needs only its
-// 32-bit byte offset v*8, shared by every column of every field.  The resource's
-// record count is the column length: an access outside the column reads 0 and
-// a store there is dropped.
+// (or spills) one address pair per column it loads.  A buffer resource is
+// wave-uniform (scalar registers), so each lane needs only its 32-bit byte offset
+// v*8, shared by every column of every field.  One resource covers a group of 8
+// columns (the column within the group goes into the scalar offset), which keeps
+// the number of live 4-register resources, and their spills, down; its record
+// count is the group's length, so an access past the group reads 0 and a store
+// there is dropped.  Needs 64 V < 2^32 (V < 2^26 voxels).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -20,6 +22,7 @@ struct Vox {
     uint32_t off;  // v * 8
     uint32_t nb;   // V * 8 (bytes of one column)
     long V;
+    static constexpr int GC = 8;  // columns per resource
 
     __device__ __forceinline__ Vox(long v, long V_) : off((uint32_t)v * 8u), nb((uint32_t)(V_ * 8)), V(V_) {}
 
@@ -28,10 +31,16 @@ struct Vox {
     }
     // component c of field a at this voxel
     __device__ __forceinline__ double ld(const double* a, long c) const {
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs(a + c * V, nb), (int)off, 0, 0));
+        const long g = c & ~(long)(GC - 1);
+        const uint32_t so = (uint32_t)(c - g) * nb;
+        return __builtin_bit_cast(double,
+                                  __builtin_amdgcn_raw_buffer_load_b64(rs(a + g * V, nb * GC), (int)off, (int)so, 0));
     }
     __device__ __forceinline__ void st(double* a, long c, double x) const {
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), rs(a + c * V, nb), (int)off, 0, 0);
+        const long g = c & ~(long)(GC - 1);
+        const uint32_t so = (uint32_t)(c - g) * nb;
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), rs(a + g * V, nb * GC), (int)off, (int)so,
+                                              0);
     }
 };
 

@@ -97,6 +97,19 @@ def test_anisotropic_grid_matches_oracle():
         assert relerr(fo[k], fg[k]) < FIELD_TOL, k
 
 
+def test_asymmetric_strengths_match_oracle():
+    # tau0b != tau0f: forward and backward trial strengths differ and the slip law picks
+    # them by the sign of the resolved shear stress (plastic_strain_rate :1387-1392)
+    p = problem(0, n=8, ngrains=4, nsteps=2)
+    p.phases[0].tau[:, 1] *= 1.25
+    rows, fo, fg = run_pair(p, 2)
+    check_rows(rows)
+    tt = fo["trialtau"].reshape(p.nvox, -1, 2)
+    assert np.abs(tt[..., 0] - tt[..., 1]).max() > 1.0
+    for k in fo:
+        assert relerr(fo[k], fg[k]) < FIELD_TOL, k
+
+
 def test_bcc48_increment_matches_oracle():
     p = problem(3, n=8, ngrains=4, nsteps=2)
     rows, fo, fg = run_pair(p, 2)
