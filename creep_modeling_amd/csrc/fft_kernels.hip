@@ -746,16 +746,19 @@ int fft_integrate(const Fields& f, FftCtx* c, int ideriv, double* dv, hipStream_
 
 namespace cg {
 
-constexpr int RBS = 256;  // threads per block: x and y passes
-constexpr int ZBS = 288;  // threads per block: z pass (whole kx columns of 9 components)
+constexpr int XBS = 128;  // threads per block: x passes (a few rows of 9 components)
+constexpr int YBS = 256;  // threads per block: y passes
+constexpr int ZBS = 144;  // threads per block: z pass (whole kx columns of 9 components)
 
 template <int N> struct RegDims {
     static constexpr int T = Shape<N>::T, E = N / T, U = (E + T - 1) / T;
     static constexpr int AN = U * T > E ? U * T : E;  // register slots per thread
-    static constexpr int NS = RBS / T;                 // signals per x / y block
-    static constexpr int RW = NS / 5;                  // rows per x block (5 signals a row)
+    static constexpr int NS = YBS / T;                 // signals per y block
+    static constexpr int NSX = XBS / T;                // signals per x block
+    static constexpr int RW = NSX / 5;                 // rows per x block (5 signals a row)
     static constexpr int NSZ = ZBS / T;                // signals per z block
     static constexpr int KT = NSZ / 9;                 // kx columns per z block
+    static_assert(RW >= 1 && KT >= 1, "block shapes");
 };
 
 template <typename R> __device__ __forceinline__ typename Cx<R>::T* spec_of(const Fields& f) {
@@ -766,11 +769,11 @@ template <typename R> __device__ __forceinline__ typename Cx<R>::T* spec_of(cons
 // Components (2p, 2p+1) of a row form one complex signal; the two half spectra are
 // separated with X[N-k] afterwards.
 template <int N, typename R>
-__global__ __launch_bounds__(RBS) void k_x_fwd_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
+__global__ __launch_bounds__(XBS) void k_x_fwd_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
                                                    const double* __restrict__ sgsum, double inv_scale) {
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
-    constexpr int T = D::T, E = D::E, U = D::U, NS = D::NS, RW = D::RW;
+    constexpr int T = D::T, E = D::E, U = D::U, NS = D::NSX, RW = D::RW;
     __shared__ T2 ex[N * NS];
     R* stage = (R*)ex;  // RW*9*N reals, dead before the exchange
     static_assert(RW * 9 * N * sizeof(R) <= N * NS * sizeof(T2), "stage fits the exchange");
@@ -780,7 +783,7 @@ __global__ __launch_bounds__(RBS) void k_x_fwd_reg(Fields f, const typename Cx<R
     const int h1 = f.h1;
     double sa[9];
     for (int c = 0; c < 9; c++) sa[c] = sgsum[c];
-    for (int e = threadIdx.x; e < RW * N; e += RBS) {
+    for (int e = threadIdx.x; e < RW * N; e += XBS) {
         const int r = e / N, i = e - r * N;
         const long row = row0 + r;
         double pk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -825,7 +828,7 @@ __global__ __launch_bounds__(RBS) void k_x_fwd_reg(Fields f, const typename Cx<R
     __syncthreads();
     T2* spec = spec_of<R>(f);
     const int w = h1 * 9;
-    for (int e = threadIdx.x; e < RW * w; e += RBS) {
+    for (int e = threadIdx.x; e < RW * w; e += XBS) {
         const int rr = e / w, rem = e - rr * w;
         const int kx = rem / 9, c = rem - kx * 9;
         const long row = row0 + rr;
@@ -841,7 +844,7 @@ __global__ __launch_bounds__(RBS) void k_x_fwd_reg(Fields f, const typename Cx<R
 
 // y pass over every (kx, c) column of one z plane
 template <int N, bool INV, typename R>
-__global__ __launch_bounds__(RBS) void k_y_reg(Fields f, const typename Cx<R>::T* __restrict__ tw, int ntile) {
+__global__ __launch_bounds__(YBS) void k_y_reg(Fields f, const typename Cx<R>::T* __restrict__ tw, int ntile) {
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
     constexpr int T = D::T, E = D::E, U = D::U, NS = D::NS;
@@ -918,11 +921,11 @@ __global__ __launch_bounds__(ZBS) void k_z_green_reg(Fields f, const typename Cx
 // spectrum (imaginary parts at kx = 0 and n1/2 dropped, as Re() of the reference's
 // c2c inverse does); block partial sums of w dLref F^-1.
 template <int N, typename R>
-__global__ __launch_bounds__(RBS) void k_x_inv_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
+__global__ __launch_bounds__(XBS) void k_x_inv_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
                                                    double out_scale) {
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
-    constexpr int T = D::T, E = D::E, U = D::U, NS = D::NS, RW = D::RW;
+    constexpr int T = D::T, E = D::E, U = D::U, NS = D::NSX, RW = D::RW;
     constexpr int H = N / 2 + 1;
     __shared__ T2 ex[N * NS];
     static_assert(RW * H * 9 <= N * NS, "spectrum rows fit the exchange");
@@ -932,7 +935,7 @@ __global__ __launch_bounds__(RBS) void k_x_inv_reg(Fields f, const typename Cx<R
     const long row0 = (long)blockIdx.x * RW;
     const int w = H * 9;
     const T2* spec = spec_of<R>(f);
-    for (int e = threadIdx.x; e < RW * w; e += RBS) {
+    for (int e = threadIdx.x; e < RW * w; e += XBS) {
         const long row = row0 + e / w;
         ex[e] = row < nrows ? spec[row0 * w + e] : T2{0, 0};
     }
@@ -973,7 +976,7 @@ __global__ __launch_bounds__(RBS) void k_x_inv_reg(Fields f, const typename Cx<R
         }
     __syncthreads();
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int e = threadIdx.x; e < RW * N; e += RBS) {
+    for (int e = threadIdx.x; e < RW * N; e += XBS) {
         const int rr = e / N, i = e - rr * N;
         const long row = row0 + rr;
         if (row >= nrows) continue;
@@ -988,7 +991,7 @@ __global__ __launch_bounds__(RBS) void k_x_inv_reg(Fields f, const typename Cx<R
         const double wv = f.w[v];
         for (int c = 0; c < 9; c++) acc[c] += q9[c] * wv;
     }
-    block_reduce_store<9, RBS>(acc, f.part);
+    block_reduce_store<9, XBS>(acc, f.part);
 }
 
 static bool reg_size(int n) { return n == 16 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128 || n == 256; }
@@ -998,18 +1001,18 @@ template <int N, typename R> static void launch_x_fwd_reg(const Fields& f, const
                                                           double inv_scale, hipStream_t s) {
     const long nrows = (long)f.n2 * f.n3;
     const int nb = (int)((nrows + RegDims<N>::RW - 1) / RegDims<N>::RW);
-    hipLaunchKernelGGL((k_x_fwd_reg<N, R>), dim3(nb), dim3(RBS), 0, s, f, tw, sgsum, inv_scale);
+    hipLaunchKernelGGL((k_x_fwd_reg<N, R>), dim3(nb), dim3(XBS), 0, s, f, tw, sgsum, inv_scale);
 }
 template <int N, typename R> static int launch_x_inv_reg(const Fields& f, const typename Cx<R>::T* tw, double out_scale,
                                                          hipStream_t s) {
     const long nrows = (long)f.n2 * f.n3;
     const int nb = (int)((nrows + RegDims<N>::RW - 1) / RegDims<N>::RW);
-    hipLaunchKernelGGL((k_x_inv_reg<N, R>), dim3(nb), dim3(RBS), 0, s, f, tw, out_scale);
+    hipLaunchKernelGGL((k_x_inv_reg<N, R>), dim3(nb), dim3(XBS), 0, s, f, tw, out_scale);
     return nb;
 }
 template <int N, bool INV, typename R> static void launch_y_reg(const Fields& f, const typename Cx<R>::T* tw, hipStream_t s) {
     const int ntile = (f.h1 * 9 + RegDims<N>::NS - 1) / RegDims<N>::NS;
-    hipLaunchKernelGGL((k_y_reg<N, INV, R>), dim3(f.n3 * ntile), dim3(RBS), 0, s, f, tw, ntile);
+    hipLaunchKernelGGL((k_y_reg<N, INV, R>), dim3(f.n3 * ntile), dim3(YBS), 0, s, f, tw, ntile);
 }
 template <int N, typename R> static void launch_z_reg(const Fields& f, const typename Cx<R>::T* tw, const GreenDev& g,
                                                       hipStream_t s) {

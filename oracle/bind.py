@@ -110,6 +110,8 @@ def lib():
         L.orc_harden.argtypes = [C.c_void_p, C.c_long, _DBL, _DBL]
         L.orc_set_c066.argtypes = [C.c_void_p, _DBL]
         L.orc_set_ipr.argtypes = [C.c_void_p, C.c_int]
+        L.orc_calc_c0.argtypes = [C.c_void_p]
+        L.orc_kinematic.argtypes = [C.c_void_p, C.c_double, _DBL]
         _lib = L
     return _lib
 

@@ -2177,3 +2177,10 @@ int orc_set_c066(orc_ctx *c, const double *c066) {
 }
 /* set the iteration-time process index (for hooks that read process data) */
 int orc_set_ipr(orc_ctx *c, int ipr0) { c->ipr = ipr0; return 0; }
+/* calc_c0 with the current process and fields */
+int orc_calc_c0(orc_ctx *c) { calc_c0(c); return 0; }
+/* kinematic_increments at step tdot: out4 = max volinc, max rotang (deg), max stretchinc,
+ * min detF_next; returns nsub */
+int orc_kinematic(orc_ctx *c, double tdot, double *out4) {
+    return kinematic_substep(c, tdot, out4, out4 + 1, out4 + 2, out4 + 3);
+}

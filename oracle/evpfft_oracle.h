@@ -111,6 +111,8 @@ int orc_convolve(orc_ctx *c, const double *in, double *out);
 int orc_harden(orc_ctx *c, long v, double *tau_out, double *gamma1);
 int orc_set_c066(orc_ctx *c, const double *c066);
 int orc_set_ipr(orc_ctx *c, int ipr0);
+int orc_calc_c0(orc_ctx *c);
+int orc_kinematic(orc_ctx *c, double tdot, double *out4);
 
 #ifdef __cplusplus
 }

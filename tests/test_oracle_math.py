@@ -442,3 +442,76 @@ def test_harden_matches_voce_closed_form_and_is_additive():
     want3 = 123.0 + voce_tau(G0 + 3 * dG, tau1, th0, th1) - voce_tau(G0, tau1, th0, th1)
     assert crss == pytest.approx(want3, rel=1e-12)
     o.close()
+
+
+# --------------------------------------------------------------------------- calc_c0, kinematics, averages
+MACRO_C066 = slice(3 * 81, 3 * 81 + 36)
+MACRO_WPHCSOL = 3 * 81 + 36 + 81 + 45
+
+
+def test_calc_c0_elastic_limit_is_mean_stiffness():
+    # calc_c0 (various_functions.f90:2537-2649) with vanishing slip rates and F = I: the tangent is
+    # T = (S_e / tdotref)^-1 = C tdotref (both tangents are built from the tdotref-scaled
+    # compliance, :2604-2605), dP/dF = T, so c066 = xc0 tdotref <C>; a tdot-scaled tangent
+    # (the other reading of :2604-2605) would give xc0 tdot <C> instead.
+    p = make_problem(n=4, ngrains=3, nsteps=1)
+    p.phases[0].gamd0[:] = 1e-300
+    pr = p.processes[0]
+    pr.tdot, pr.tdotref = 0.5, 2.0
+    o = bind.Oracle(p)
+    o.L.orc_set_ipr(o.h, 0)
+    o.L.orc_calc_c0(o.h)
+    c066 = o.field("macro")[MACRO_C066]
+    cg = o.field("cg66").reshape(p.nvox, 36)
+    want = pr.xc0 * pr.tdotref * cg.mean(axis=0)
+    np.testing.assert_allclose(c066, want, rtol=1e-11, atol=1e-11 * np.abs(want).max())
+    o.close()
+
+
+def test_kinematic_increments_of_a_uniform_velocity_gradient():
+    # kinematic_increments (various_functions.f90:2766-2837): F_inc = expm(L dt), volume
+    # increment |log det F_inc|, rotation angle of its polar factor, |log U|, next det F,
+    # and nsub = ceil of the largest ratio to (0.05, 2 deg, 0.05)
+    p = make_problem(n=4, ngrains=2, nsteps=1)
+    o = bind.Oracle(p)
+    rng = np.random.default_rng(12)
+    L = rng.standard_normal((3, 3)) * 0.03
+    o.set_field("velgrad", np.tile(L.ravel(), p.nvox))
+    o.set_field("detF", np.full(p.nvox, 1.1))
+    dt = 1.7
+    out = np.zeros(4)
+    nsub = o.L.orc_kinematic(o.h, dt, bind.dp(out))
+    Fi = sla.expm(L * dt)
+    R, U = sla.polar(Fi, side="right")
+    vol = abs(np.log(np.linalg.det(Fi)))
+    rot = np.degrees(np.arccos(np.clip((np.trace(R) - 1.0) / 2.0, -1.0, 1.0)))
+    stretch = np.sqrt(np.sum(np.log(np.linalg.eigvalsh(U)) ** 2))
+    np.testing.assert_allclose(out, [vol, rot, stretch, 1.1 * np.linalg.det(Fi)], rtol=1e-10)
+    assert nsub == max(1, int(np.ceil(max(vol / 0.05, rot / 2.0, stretch / 0.05))))
+    o.close()
+
+
+def v6(m):
+    return np.array([m[0, 0], m[1, 1], m[2, 2], m[1, 2], m[0, 2], m[0, 1]])
+
+
+def test_macro_averages_from_fields_with_the_eelavg_quirk():
+    # LS-EVPFFT.f90:460-523: solid w-weighted averages / wphcsol; eelavg is overwritten in the
+    # voxel loop as eavg(running) + (disgradsym - ept) w of the current voxel (:482), so it ends
+    # as the full eavg sum plus the last solid voxel's elastic part
+    p = make_problem(n=4, ngrains=3, nsteps=1)
+    o = bind.Oracle(p)
+    row = o.step().as_dict()
+    V = p.nvox
+    w = o.field("wgtc")
+    wph = o.field("macro")[MACRO_WPHCSOL]
+    f = {k: o.field(k).reshape(V, 3, 3) for k in ("sgt", "edotp", "ept", "disgradsym")}
+    avg = {k: np.einsum("v,vij->ij", w, f[k]) / wph for k in f}
+    np.testing.assert_allclose(row["sgsolavg"], v6(avg["sgt"]), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(row["edotpavg"], v6(avg["edotp"]), rtol=1e-12, atol=1e-20)
+    np.testing.assert_allclose(row["epavg"], v6(avg["ept"]), rtol=1e-12, atol=1e-20)
+    np.testing.assert_allclose(row["eavg"], v6(avg["disgradsym"]), rtol=1e-12, atol=1e-20)
+    last = V - 1  # voxel order i fastest; every voxel is solid here
+    eel = (np.einsum("v,vij->ij", w, f["disgradsym"]) + (f["disgradsym"][last] - f["ept"][last]) * w[last]) / wph
+    np.testing.assert_allclose(row["eelavg"], v6(eel), rtol=1e-12, atol=1e-20)
+    o.close()
