@@ -32,14 +32,14 @@ FP64_PEAK_TFLOPS = 78.6
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_fcc128.json"
 
 
-def measured_traffic(stage, n, grains):
+def measured_traffic(stage, config, n, grains):
     """HBM bytes per launch of `stage` for this workload from the committed counter runs, or None."""
     try:
         d = json.loads(TRAFFIC_FILE.read_text())
     except (OSError, ValueError):
         return None
     e = d.get("stages", {}).get(stage)
-    if not e or d.get("n") != n or d.get("grains") != grains:
+    if not e or d.get("config") != config or d.get("n") != n or d.get("grains") != grains:
         return None
     return e["bytes_per_launch"]
 
@@ -71,13 +71,17 @@ def run_gpu(problem, precision, steps, warmup):
     torch.cuda.init()
     s = driver.Solver(problem, precision=precision, scan_cond=True)
     lib = s.L
-    for _ in range(warmup):
-        s.step()
+    for i in range(warmup):
+        r = s.step()
+        print("warm-up increment %d: %d passes" % (i + 1, r["passes"]), file=sys.stderr, flush=True)
     lib.cg_set_timing(s.h, 1)
     s.ctx.stats(reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    rows = [s.step() for _ in range(steps)]
+    rows = []
+    for i in range(steps):
+        rows.append(s.step())
+        print("timed increment %d: %d passes" % (i + 1, rows[-1]["passes"]), file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     st = s.ctx.stats()
@@ -129,8 +133,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--grains", type=int, default=512)
+    ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3),
+                    help="1: FCC n=10 (the headline workload); 2: FCC n=15; 3: BCC 48 systems, biaxial, dt=0.5 s")
+    ap.add_argument("--n", type=int, default=None, help="grid edge (default 128 for configs 1-2, 256 for 3)")
+    ap.add_argument("--grains", type=int, default=None, help="Voronoi grains (default 512, 2048 for config 3)")
+    ap.add_argument("--s33", type=float, default=None, help="creep stress sigma33 in MPa (config default 300)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-passes", type=int, default=2)
@@ -147,8 +154,14 @@ def main():
 
     from creep_modeling_amd import inputs, parse
 
-    ms, mat, procs = inputs.config_case(1, n=args.n, ngrains=args.grains, nsteps=args.warmup + args.steps + 1)
+    if args.n is None:
+        args.n = 256 if args.config == 3 else 128
+    if args.grains is None:
+        args.grains = 2048 if args.config == 3 else 512
+    ms, mat, procs = inputs.config_case(args.config, n=args.n, ngrains=args.grains,
+                                        nsteps=args.warmup + args.steps + 1, s33=args.s33)
     problem = parse.problem_from_case(ms, mat, procs)
+    pr0 = procs[0]
     V = problem.nvox
     ns = problem.phases[0].nsyst
 
@@ -166,10 +179,10 @@ def main():
     if dominant_solve:
         roof = dict(bound="mfma", kernel="k_correct_solve (constitutive Newton, FP64 vector ALU)",
                     achieved=solve_tflops, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=solve_tflops / FP64_PEAK_TFLOPS,
-                    traffic=measured_traffic("k_correct_solve", args.n, args.grains), avg_ms=t_solve)
+                    traffic=measured_traffic("k_correct_solve", args.config, args.n, args.grains), avg_ms=t_solve)
     else:
         roof = dict(bound="hbm", kernel="FFT + Green stage", achieved=fft_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=fft_gbs / HBM_PEAK_GBS, traffic=measured_traffic("fft_green", args.n, args.grains),
+                    frac=fft_gbs / HBM_PEAK_GBS, traffic=measured_traffic("fft_green", args.config, args.n, args.grains),
                     avg_ms=t_fft)
     out = {
         "metric": "voxel-iters/s at 128^3 creep RVE (mixed vs FP64 GPU) + % HBM roofline",
@@ -184,8 +197,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded periodic Voronoi RVE, uniform SO(3) orientations)",
-        "config": {"workload": "%d^3 RVE, %d grains, FCC 12 systems, sigma33=300 MPa creep, FP64 form; step = one "
-                               "accepted increment" % (args.n, args.grains),
+        "config": {"workload": "%d^3 RVE, %d grains, %s %d systems, n=%g, %screep, FP64 form; step = one "
+                               "accepted increment" % (args.n, args.grains, "BCC" if args.config == 3 else "FCC", ns,
+                                                       problem.phases[0].nrs[0],
+                                                       ("sigma11=%g " % pr0.s11 if pr0.s11 else "") +
+                                                       "sigma33=%g MPa " % pr0.s33),
                    "passes": passes, "voxels": V, "rve_increments_per_s": args.steps / T},
         "roofline": roof,
         "stages_ms_per_pass": {"fft_green": t_fft, "correct_solve": t_solve,
@@ -193,7 +209,7 @@ def main():
                                "per_increment_updates_ms": st.t_ms[3] / max(args.steps, 1)},
         "fft_green_roofline": {"bound": "hbm", "achieved": fft_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": fft_gbs / HBM_PEAK_GBS, "alg_bytes_per_voxel": fft_alg_bytes(args.n),
-                               "traffic": measured_traffic("fft_green", args.n, args.grains)},
+                               "traffic": measured_traffic("fft_green", args.config, args.n, args.grains)},
         "constitutive": {"tflops": solve_tflops, "frac_fp64_peak": solve_tflops / FP64_PEAK_TFLOPS,
                          "nr_steps_per_voxel_iter": st.nr_steps / max(st.voxel_iters, 1),
                          "evals_per_voxel_iter": st.evals / max(st.voxel_iters, 1),

@@ -137,7 +137,7 @@ struct cg_ctx {
     double* d_sgsum = nullptr;
     double* d_fluct = nullptr;
     size_t part_cap = 0;
-    int sgsum_valid = 0, lold_same = 0, int_pow = 0, sym_tau = 0, ngas = 0, n3node = 1;
+    int sgsum_valid = 0, lold_same = 0, int_pow = 0, sym_tau = 0, small_ns = 0, ngas = 0, n3node = 1;
     long last_solid = -1;
     double nsys_solid = 0.0;
     // macroscopic state (micro_type, types.f90:55-94)
@@ -301,7 +301,7 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     PhaseDev hp[MAXPH];
     memset(hp, 0, sizeof hp);
     std::vector<double> tab;
-    int all_int = 1, all_sym = 1;
+    int all_int = 1, all_sym = 1, small_ns = 1;
     for (int i = 0; i < c->nph; i++) {
         HostPhase& P = c->ph[i];
         PhaseDev& D = hp[i];
@@ -331,9 +331,11 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
             if (!(n == floor(n) && n >= 1.0 && n <= 64.0)) all_int = 0;
         for (int is = 0; is < P.nsyst; is++)
             if (P.tau[3 * is] != P.tau[3 * is + 1]) all_sym = 0;
+        if (P.nsyst > 12) small_ns = 0;
     }
     c->int_pow = all_int;
     c->sym_tau = all_sym;
+    c->small_ns = small_ns;
     HIPCHK(c, hipMalloc(&c->d_ph, sizeof hp));
     HIPCHK(c, hipMemcpy(c->d_ph, hp, sizeof hp, hipMemcpyHostToDevice));
     if (dalloc(c, &c->d_tab, std::max<size_t>(tab.size(), 1))) return -3;
@@ -527,7 +529,7 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     inv3(c->defgradinvavgc, finv_inv);
     PhaseTables pt{c->d_ph, c->d_tab};
     tstart(c, 1);
-    launch_correct_solve_full(f, pt, pc, c->dvelgradavg, finv_inv, c->d_fluct, c->s, &nb, c->int_pow, c->sym_tau);
+    launch_correct_solve_full(f, pt, pc, c->dvelgradavg, finv_inv, c->d_fluct, c->s, &nb, c->int_pow, c->sym_tau, c->small_ns);
     tstop(c, 1);
     launch_reduce_final(f.part, nb, R_NSOLVE, f.red, c->s);
     // next pass' <sigma> = sum_all w sigma

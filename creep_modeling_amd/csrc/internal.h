@@ -76,7 +76,7 @@ void launch_reduce_final(const double* part, int nblocks, int nvals, double* out
 void launch_sigma_sum(const Fields& f, hipStream_t s, int* nblocks_out);
 void launch_correct_solve_full(const Fields& f, const PhaseTables& pt, const PassConst& pc,
                                const double* dvelgradavg, const double* finv_inv, const double* fluct_dev,
-                               hipStream_t s, int* nb, int int_pow, int sym_tau);
+                               hipStream_t s, int* nb, int int_pow, int sym_tau, int small_ns);
 void launch_update_schmid(const Fields& f, const PhaseTables& pt, hipStream_t s);
 void launch_c066mod(const Fields& f, const double* c066, const double* goperr0, int modal, hipStream_t s);
 void launch_gas_compl(const Fields& f, double tdot, double complgas, int modal, hipStream_t s);

@@ -175,9 +175,29 @@ __device__ __forceinline__ void slip_eval(const Fields& f, const double* __restr
 // system: init and harden then keep the two columns bitwise equal), so one is read.
 // ACC: accumulate edotp and its derivative; STORE: store gdot (the last evaluation's
 // rates feed harden and the per-increment updates).
+// Trial strengths are read for a group of TauGroup systems at a time, one group ahead:
+// the loads of group g+1 are in flight while group g computes.  Group indices are
+// clamped to the last system so that every load is unconditional and the wait counts
+// stay exact.  tpn/tnn carry group 0 in and out: the caller loads it once after each
+// change of the strengths, and the last group of an evaluation prefetches group 0 for
+// the next evaluation.
+template <bool SYMTAU>
+struct TauGroup {
+    static constexpr int G = SYMTAU ? 4 : 2;
+};
+template <bool SYMTAU>
+__device__ __forceinline__ void load_tau_group(const Fields& f, const Vox& x, int ns, int g0, double* tpn,
+                                               double* tnn) {
+#pragma unroll
+    for (int k = 0; k < TauGroup<SYMTAU>::G; k++) {
+        const int c = __builtin_amdgcn_readfirstlane(g0 + k < ns ? g0 + k : ns - 1);
+        tpn[k] = x.ld(f.ttau, 2 * c);
+        if (!SYMTAU) tnn[k] = x.ld(f.ttau, 2 * c + 1);
+    }
+}
 template <bool INTPOW, bool SYMTAU, bool ACC, bool STORE, class PH, class TB>
 __device__ __forceinline__ void slip_eval_ab(const Fields& f, TB tab, const PH& P, const Vox& x, const double* sal,
-                                             const double* sg, double* ep, double* dd) {
+                                             const double* sg, double* ep, double* dd, double* tpn, double* tnn) {
     double A[9], Ai[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
@@ -189,30 +209,18 @@ __device__ __forceinline__ void slip_eval_ab(const Fields& f, TB tab, const PH& 
         for (int i = 0; i < 15; i++) dd[i] = 0.0;
     }
     const int ns = P.nsyst;
-    // Trial strengths are read for a group of G systems at a time, one group ahead: the
-    // loads of group g+1 are in flight while group g computes (they come from HBM: the
-    // per-XCD working set of the solve exceeds L2).  Group indices are clamped to the last
-    // system so that every load is unconditional and the wait counts stay exact.
-    constexpr int G = SYMTAU ? 4 : 2;
-    double tpn[G], tnn[G];
-#pragma unroll
-    for (int k = 0; k < G; k++) {
-        const int c = __builtin_amdgcn_readfirstlane(k < ns ? k : ns - 1);
-        tpn[k] = x.ld(f.ttau, 2 * c);
-        if (!SYMTAU) tnn[k] = x.ld(f.ttau, 2 * c + 1);
-    }
+    constexpr int G = TauGroup<SYMTAU>::G;
 #pragma unroll 1
     for (int g0 = 0; g0 < ns; g0 += G) {
         const int gb = __builtin_amdgcn_readfirstlane(g0);
+        const int nx = __builtin_amdgcn_readfirstlane(gb + G < ns ? gb + G : 0);  // next group (wraps to 0)
         double tp[G], tn[G];
 #pragma unroll
         for (int k = 0; k < G; k++) {
             tp[k] = tpn[k];
             tn[k] = SYMTAU ? tpn[k] : tnn[k];
-            const int c = __builtin_amdgcn_readfirstlane(gb + G + k < ns ? gb + G + k : ns - 1);
-            tpn[k] = x.ld(f.ttau, 2 * c);
-            if (!SYMTAU) tnn[k] = x.ld(f.ttau, 2 * c + 1);
         }
+        load_tau_group<SYMTAU>(f, x, ns, nx, tpn, tnn);
 #pragma unroll
         for (int k = 0; k < G; k++) {
             const int is = gb + k;
@@ -225,14 +233,14 @@ __device__ __forceinline__ void slip_eval_ab(const Fields& f, TB tab, const PH& 
             schmid5(A, Ai, b3, n3, sv);
             const double s0 = sv[0], s1 = sv[1], s2 = sv[2], s3 = sv[3], s4 = sv[4];
             double rss = s0 * sg[0] + s1 * sg[1] + s2 * sg[2] + s3 * sg[3] + s4 * sg[4];
-            const double tau = rss < 0.0 ? tn[k] : tp[k];
-            rss = rss / tau;
+            const double ti = 1.0 / (rss < 0.0 ? tn[k] : tp[k]);
+            rss = rss * ti;
             const double n = tab[__builtin_amdgcn_readfirstlane(P.off_nrs + is)];
             const double dum = tab[__builtin_amdgcn_readfirstlane(P.off_gamd0 + is)] * powm1<INTPOW>(fabs(rss), n);
             const double r2 = rss * dum;
             if (STORE) x.st(f.gdot, is, r2);
             if (ACC) {
-                const double r1 = n * dum / tau;
+                const double r1 = n * dum * ti;
                 ep[0] += s0 * r2; ep[1] += s1 * r2; ep[2] += s2 * r2; ep[3] += s3 * r2; ep[4] += s4 * r2;
                 const double t0 = s0 * r1, t1 = s1 * r1, t2 = s2 * r1, t3 = s3 * r1, t4 = s4 * r1;
                 dd[0] += s0 * t0; dd[1] += s0 * t1; dd[2] += s0 * t2; dd[3] += s0 * t3; dd[4] += s0 * t4;
@@ -323,10 +331,127 @@ __device__ __forceinline__ double harden_voxel(const Fields& f, TB tab, const PH
     return e2;
 }
 
-// kRatesAfter: instead of storing gdot at every evaluation, evaluate the rates once more
-// after each Newton loop at the last evaluation's stress (the same arithmetic, so the
-// same values) and store them then.
-constexpr bool kRatesAfter = false;
+// harden_voxel inside the solve for phases with at most 12 systems: the old strengths
+// and crss of a chunk of HC systems are read one chunk ahead (the first chunk together
+// with the rates), so the chunks do not each wait a memory round trip.
+template <class PH, class TB>
+__device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH& P, const Vox& x, double tdot) {
+    const int ns = P.nsyst;
+    constexpr int NR = 12, HC = 4;
+    double cr[2 * HC], ot[2 * HC];
+    auto load_chunk = [&](int c0, double* c2, double* o2) {
+#pragma unroll
+        for (int k = 0; k < HC; k++) {
+            const int i = __builtin_amdgcn_readfirstlane(c0 + k < ns ? c0 + k : ns - 1);
+            c2[2 * k] = x.ld(f.crss, 2 * i);
+            c2[2 * k + 1] = x.ld(f.crss, 2 * i + 1);
+            o2[2 * k] = x.ld(f.ttau, 2 * i);
+            o2[2 * k + 1] = x.ld(f.ttau, 2 * i + 1);
+        }
+    };
+    load_chunk(0, cr, ot);
+    double gr[NR];
+#pragma unroll
+    for (int js = 0; js < NR; js++) gr[js] = js < ns ? fabs(x.ld(f.gdot, js)) : 0.0;
+    const double gamma0 = x.ld(f.gacc, 0);
+    double deltgam = 0.0;
+#pragma unroll
+    for (int js = 0; js < NR; js++)
+        if (js < ns) deltgam += gr[js] * tdot;
+    const double gamma1 = gamma0 + deltgam;
+    const TB hard = tab + P.off_hard;
+    double e2 = 0.0;
+    int m = -1, end = 0;
+    double sc = 0.0;
+#pragma unroll 1
+    for (int c00 = 0; c00 < ns; c00 += HC) {
+        const int c0 = __builtin_amdgcn_readfirstlane(c00);
+        double crn[2 * HC], otn[2 * HC];
+        load_chunk(c0 + HC < ns ? c0 + HC : c0, crn, otn);
+#pragma unroll
+        for (int k = 0; k < HC; k++) {
+            const int is = c0 + k;
+            if (is >= ns) break;
+            while (is >= end) {  // the Voce factor of the mode this system belongs to
+                m = __builtin_amdgcn_readfirstlane(m + 1);
+                end += P.nsm[m];
+                const double tau1 = P.mode_tau1[m], th0 = P.mode_thet0[m], th1 = P.mode_thet1[m];
+                const double fact = fabs(th0 / tau1);
+                const double ex0 = exp(-fact * gamma0), ex1 = exp(-fact * gamma1);
+                const double voce = deltgam * th1 + tau1 * (ex0 - ex1) + th1 * (gamma0 * ex0 - gamma1 * ex1);
+                sc = voce * tdot / deltgam;
+            }
+            double dtau = 0.0;
+#pragma unroll
+            for (int js = 0; js < NR; js++)
+                if (js < ns) dtau += hard[is * ns + js] * gr[js];
+            const double t0 = cr[2 * k] + dtau * sc;
+            const double t1 = cr[2 * k + 1] + dtau * sc;
+            const double o0 = ot[2 * k], o1 = ot[2 * k + 1];
+            e2 += (o0 - t0) * (o0 - t0) + (o1 - t1) * (o1 - t1);
+            x.st(f.ttau, 2 * is, t0);
+            x.st(f.ttau, 2 * is + 1, t1);
+        }
+#pragma unroll
+        for (int k = 0; k < 2 * HC; k++) {
+            cr[k] = crn[k];
+            ot[k] = otn[k];
+        }
+    }
+    return e2;
+}
+
+// harden_voxel for phases with more than 12 systems: dtau of 12 systems of a mode at a
+// time (js ascending), so each |gdot_js| is read once per chunk instead of once per system
+template <class PH, class TB>
+__device__ __forceinline__ double harden_voxel_chunked(const Fields& f, TB tab, const PH& P, const Vox& x,
+                                                       double tdot) {
+    const int ns = P.nsyst;
+    double deltgam = 0.0;
+    for (int js = 0; js < ns; js++) deltgam += fabs(x.ld(f.gdot, uidx<true>(js))) * tdot;
+    const double gamma0 = x.ld(f.gacc, 0);
+    const double gamma1 = gamma0 + deltgam;
+    constexpr int C = 12;
+    const TB hard = tab + P.off_hard;
+    double e2 = 0.0;
+    int is = 0;
+    for (int m0 = 0; m0 < P.nmodes; m0++) {
+        const int m = uidx<true>(m0);
+        const double tau1 = P.mode_tau1[m], th0 = P.mode_thet0[m], th1 = P.mode_thet1[m];
+        const double fact = fabs(th0 / tau1);
+        const double ex0 = exp(-fact * gamma0), ex1 = exp(-fact * gamma1);
+        const double voce = deltgam * th1 + tau1 * (ex0 - ex1) + th1 * (gamma0 * ex0 - gamma1 * ex1);
+        const double sc = voce * tdot / deltgam;
+        const int nm = P.nsm[m];
+        for (int k0 = 0; k0 < nm; k0 += C) {
+            const int i0 = uidx<true>(is + k0);
+            const int nk = uidx<true>(nm - k0 < C ? nm - k0 : C);
+            double dt[C];
+#pragma unroll
+            for (int k = 0; k < C; k++) dt[k] = 0.0;
+            for (int js = 0; js < ns; js++) {
+                const int j = uidx<true>(js);
+                const double g = fabs(x.ld(f.gdot, j));
+#pragma unroll
+                for (int k = 0; k < C; k++)
+                    if (k < nk) dt[k] += hard[(i0 + k) * ns + j] * g;
+            }
+#pragma unroll
+            for (int k = 0; k < C; k++) {
+                if (k >= nk) break;
+                const int i = i0 + k;
+                const double t0 = x.ld(f.crss, 2 * i) + dt[k] * sc;
+                const double t1 = x.ld(f.crss, 2 * i + 1) + dt[k] * sc;
+                const double o0 = x.ld(f.ttau, 2 * i), o1 = x.ld(f.ttau, 2 * i + 1);
+                e2 += (o0 - t0) * (o0 - t0) + (o1 - t1) * (o1 - t1);
+                x.st(f.ttau, 2 * i, t0);
+                x.st(f.ttau, 2 * i + 1, t1);
+            }
+        }
+        is += nm;
+    }
+    return e2;
+}
 
 // solve_res for a solid voxel without the modified-AL term (various_functions.f90:1142-1324).
 // The reference solves J d = R with J = I + cm (dedotp + s66/tdot) and
@@ -337,7 +462,7 @@ constexpr bool kRatesAfter = false;
 // the same line search.  Only the packed symmetric parts of cm and s66 (21 entries
 // each) are kept, and K is factored as L D L^T: the live state of the Newton loop is
 // about half of the J/LU form, which doubles the waves each SIMD can hold.
-template <bool INTPOW, bool SYMTAU>
+template <bool INTPOW, bool SYMTAU, bool SMALLNS>
 // p0 is wave-uniform (the caller iterates over the phases present in the wave), so the
 // phase tables are read through scalar loads.
 // cml: this thread's column of a block-shared LDS array [21][BS] holding the packed
@@ -413,9 +538,11 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
     int itertau = 0;
     double errtau = 2.0 * tol;
     int nnr = 0, nev = 0, nout = 0;
+    double tpn[TauGroup<SYMTAU>::G], tnn[TauGroup<SYMTAU>::G];
     while (itertau < 10 && errtau > tol) {
         itertau++;
         nout++;
+        load_tau_group<SYMTAU>(f, x, P.nsyst, 0, tpn, tnn);  // the strengths changed (harden) or are new
         int iternr = 0;
         double erral = 2.0 * pc.erroral, resoldn = 1.0;
         while (iternr < 100 && erral > pc.erroral) {
@@ -426,7 +553,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
             while (resn > resoldn && icut < 10) {
                 nev++;
                 parked = false;
-                slip_eval_ab<INTPOW, SYMTAU, true, !kRatesAfter>(f, tab, P, x, sal, sg, ep, dd);
+                slip_eval_ab<INTPOW, SYMTAU, true, true>(f, tab, P, x, sal, sg, ep, dd, tpn, tnn);
                 spmv6(A, sg, rp);
 #pragma unroll
                 for (int i = 0; i < 6; i++) rp[i] += (i < 5 ? ep[i] : 0.0) - b[i];
@@ -463,15 +590,10 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
             }
             erral = sqrt(d1) / sgnorm;
         }
-        if (kRatesAfter) {  // the rates of the last evaluation, at its stress, stored once
-            double sev[6];
-            for (int c = 0; c < 6; c++) sev[c] = parked ? x.ld(f.edotp, c) : sgold[c];
-            slip_eval_ab<INTPOW, SYMTAU, false, true>(f, tab, P, x, sal, sev, nullptr, nullptr);
-        }
         // harden with the slip rates of the last evaluation
         if (pc.iuphard == 1) {
             double g1;
-            const double e2 = harden_voxel<true>(f, tab, P, x, tdot, &g1, false);
+            const double e2 = SMALLNS ? harden_small(f, tab, P, x, tdot) : harden_voxel_chunked(f, tab, P, x, tdot);
             errtau = sqrt(e2) / taun;
         } else {
             errtau = 0.0;
@@ -508,7 +630,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
     acc[R_OUTER] = (double)nout;
 }
 
-template <bool MODAL, bool INTPOW, bool SYMTAU>
+template <bool MODAL, bool INTPOW, bool SYMTAU, bool SMALLNS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 2))) void k_correct_solve(Fields f, PhaseTables pt, SolveArgs a) {
     const long V = f.V;
     const long v = (long)blockIdx.x * BS + threadIdx.x;
@@ -592,7 +714,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 
             for (;;) {
                 const int p0 = __builtin_amdgcn_readfirstlane(ph);
                 if (ph == p0) {
-                    solve_solid_sym<INTPOW, SYMTAU>(f, pt, pc, x, p0, cm_lds + threadIdx.x, sa_lds + threadIdx.x, sgf,
+                    solve_solid_sym<INTPOW, SYMTAU, SMALLNS>(f, pt, pc, x, p0, cm_lds + threadIdx.x, sa_lds + threadIdx.x, sgf,
                                                     acc);
                     break;
                 }
@@ -780,22 +902,31 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 
 // Public launcher with the full argument set.
 void launch_correct_solve_full(const Fields& f, const PhaseTables& pt, const PassConst& pc,
                                const double* dvelgradavg, const double* finv_inv, const double* fluct_dev,
-                               hipStream_t s, int* nb, int int_pow, int sym_tau) {
+                               hipStream_t s, int* nb, int int_pow, int sym_tau, int small_ns) {
     SolveArgs a;
     a.pc = pc;
     for (int i = 0; i < 9; i++) { a.dvelgradavg[i] = dvelgradavg[i]; a.finv_inv[i] = finv_inv[i]; }
     a.fluct = fluct_dev;
     *nb = nblk(f.V);
     if (pc.modal) {
-        if (int_pow) hipLaunchKernelGGL((k_correct_solve<true, true, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
-        else hipLaunchKernelGGL((k_correct_solve<true, false, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
-    } else if (sym_tau) {
-        if (int_pow) hipLaunchKernelGGL((k_correct_solve<false, true, true>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
-        else hipLaunchKernelGGL((k_correct_solve<false, false, true>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
-    } else {
-        if (int_pow) hipLaunchKernelGGL((k_correct_solve<false, true, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
-        else hipLaunchKernelGGL((k_correct_solve<false, false, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
+        if (int_pow) hipLaunchKernelGGL((k_correct_solve<true, true, false, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
+        else hipLaunchKernelGGL((k_correct_solve<true, false, false, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
+        return;
     }
+    // the symmetric-solve kernel is instantiated per (integer exponents, equal forward and
+    // backward strengths, at most 12 systems per phase)
+#define CG_SOLVE(IP, ST, SM) hipLaunchKernelGGL((k_correct_solve<false, IP, ST, SM>), dim3(*nb), dim3(BS), 0, s, f, pt, a)
+    switch ((int_pow ? 4 : 0) + (sym_tau ? 2 : 0) + (small_ns ? 1 : 0)) {
+        case 0: CG_SOLVE(false, false, false); break;
+        case 1: CG_SOLVE(false, false, true); break;
+        case 2: CG_SOLVE(false, true, false); break;
+        case 3: CG_SOLVE(false, true, true); break;
+        case 4: CG_SOLVE(true, false, false); break;
+        case 5: CG_SOLVE(true, false, true); break;
+        case 6: CG_SOLVE(true, true, false); break;
+        default: CG_SOLVE(true, true, true); break;
+    }
+#undef CG_SOLVE
 }
 
 // ---------------------------------------------------------------------------

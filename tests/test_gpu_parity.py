@@ -97,6 +97,19 @@ def test_anisotropic_grid_matches_oracle():
         assert relerr(fo[k], fg[k]) < FIELD_TOL, k
 
 
+@pytest.mark.parametrize("n", [(256, 16, 16), (16, 256, 16), (16, 16, 256)])
+def test_transform_length_256_matches_oracle(n):
+    # each pass kernel at length 256 (16 threads x 16 points per transform); 20 fixed-point
+    # passes of the first increment, accepted at itmax
+    p = problem(0, n=n, ngrains=6, nsteps=1)
+    pr = p.processes[0]
+    pr.itmax, pr.tdotmin = 20, pr.tdot
+    rows, fo, fg = run_pair(p, 1)
+    check_rows(rows)
+    for k in fo:
+        assert relerr(fo[k], fg[k]) < FIELD_TOL, k
+
+
 def test_asymmetric_strengths_match_oracle():
     # tau0b != tau0f: forward and backward trial strengths differ and the slip law picks
     # them by the sign of the resolved shear stress (plastic_strain_rate :1387-1392)
