@@ -327,15 +327,29 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
         D.off_hard = put(P.hard);
         D.off_tau = put(P.tau);
         D.off_hpfac = put(P.hpfac);
+        {
+            std::vector<double> rec(8 * (size_t)P.nsyst);
+            for (int is = 0; is < P.nsyst; is++) {
+                for (int q = 0; q < 3; q++) {
+                    rec[8 * is + q] = P.dbca[3 * is + q];
+                    rec[8 * is + 3 + q] = P.dnca[3 * is + q];
+                }
+                rec[8 * is + 6] = P.nrs[is];
+                rec[8 * is + 7] = P.gamd0[is];
+            }
+            if (tab.size() % 2) tab.push_back(0.0);  // records on 16-byte boundaries
+            D.off_sys = put(rec);
+        }
         for (double n : P.nrs)
             if (!(n == floor(n) && n >= 1.0 && n <= 64.0)) all_int = 0;
         for (int is = 0; is < P.nsyst; is++)
             if (P.tau[3 * is] != P.tau[3 * is + 1]) all_sym = 0;
-        if (P.nsyst > 12) small_ns = 0;
+        if (P.nsyst > 12 || P.nmodes != 1) small_ns = 0;  // the one-mode, at most 12 systems solve
     }
     c->int_pow = all_int;
     c->sym_tau = all_sym;
     c->small_ns = small_ns;
+    tab.insert(tab.end(), 16, 0.0);  // a row of the interaction matrix may be read 12 wide
     HIPCHK(c, hipMalloc(&c->d_ph, sizeof hp));
     HIPCHK(c, hipMemcpy(c->d_ph, hp, sizeof hp, hipMemcpyHostToDevice));
     if (dalloc(c, &c->d_tab, std::max<size_t>(tab.size(), 1))) return -3;

@@ -18,6 +18,7 @@ struct PhaseDev {
     double mode_tau1[MAXMODES], mode_thet0[MAXMODES], mode_thet1[MAXMODES];
     // offsets (in doubles) into PhaseTables::data
     int off_dnca, off_dbca, off_nrs, off_gamd0, off_hard, off_tau, off_hpfac;
+    int off_sys;          // per system 8 doubles: b (3), n (3), nrs, gamd0 (one scalar load per system)
 };
 
 struct PhaseTables {

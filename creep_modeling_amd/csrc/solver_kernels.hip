@@ -226,17 +226,18 @@ __device__ __forceinline__ void slip_eval_ab(const Fields& f, TB tab, const PH& 
             const int is = gb + k;
             if (is >= ns) break;
             double sv[5], b3[3], n3[3];
+            const int rb = __builtin_amdgcn_readfirstlane(P.off_sys + 8 * is);  // b, n, nrs, gamd0
             for (int q = 0; q < 3; q++) {
-                b3[q] = tab[__builtin_amdgcn_readfirstlane(P.off_dbca + 3 * is + q)];
-                n3[q] = tab[__builtin_amdgcn_readfirstlane(P.off_dnca + 3 * is + q)];
+                b3[q] = tab[rb + q];
+                n3[q] = tab[rb + 3 + q];
             }
             schmid5(A, Ai, b3, n3, sv);
             const double s0 = sv[0], s1 = sv[1], s2 = sv[2], s3 = sv[3], s4 = sv[4];
             double rss = s0 * sg[0] + s1 * sg[1] + s2 * sg[2] + s3 * sg[3] + s4 * sg[4];
             const double ti = 1.0 / (rss < 0.0 ? tn[k] : tp[k]);
             rss = rss * ti;
-            const double n = tab[__builtin_amdgcn_readfirstlane(P.off_nrs + is)];
-            const double dum = tab[__builtin_amdgcn_readfirstlane(P.off_gamd0 + is)] * powm1<INTPOW>(fabs(rss), n);
+            const double n = tab[rb + 6];
+            const double dum = tab[rb + 7] * powm1<INTPOW>(fabs(rss), n);
             const double r2 = rss * dum;
             if (STORE) x.st(f.gdot, is, r2);
             if (ACC) {
@@ -331,18 +332,23 @@ __device__ __forceinline__ double harden_voxel(const Fields& f, TB tab, const PH
     return e2;
 }
 
-// harden_voxel inside the solve for phases with at most 12 systems: the old strengths
-// and crss of a chunk of HC systems are read one chunk ahead (the first chunk together
-// with the rates), so the chunks do not each wait a memory round trip.
+// harden_voxel inside the solve for phases of one slip mode with at most 12 systems, written without
+// branches on the system count: every load is unconditional (indices clamped to the last
+// system, unused rates weighted by zero), so the rates, gacc and the first chunk's crss
+// and old strengths are in flight together, and each later chunk of HC systems is read
+// while the previous one computes.  The sums run over the same terms in the same order
+// as harden_voxel (the added terms are exact zeros).
 template <class PH, class TB>
 __device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH& P, const Vox& x, double tdot) {
-    const int ns = P.nsyst;
+    const int ns = __builtin_amdgcn_readfirstlane(P.nsyst);
+    const int oh = __builtin_amdgcn_readfirstlane(P.off_hard);
     constexpr int NR = 12, HC = 4;
+    auto clampi = [&](int i) { return __builtin_amdgcn_readfirstlane(i < ns ? i : ns - 1); };
     double cr[2 * HC], ot[2 * HC];
     auto load_chunk = [&](int c0, double* c2, double* o2) {
 #pragma unroll
         for (int k = 0; k < HC; k++) {
-            const int i = __builtin_amdgcn_readfirstlane(c0 + k < ns ? c0 + k : ns - 1);
+            const int i = clampi(c0 + k);
             c2[2 * k] = x.ld(f.crss, 2 * i);
             c2[2 * k + 1] = x.ld(f.crss, 2 * i + 1);
             o2[2 * k] = x.ld(f.ttau, 2 * i);
@@ -352,39 +358,37 @@ __device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH
     load_chunk(0, cr, ot);
     double gr[NR];
 #pragma unroll
-    for (int js = 0; js < NR; js++) gr[js] = js < ns ? fabs(x.ld(f.gdot, js)) : 0.0;
+    for (int js = 0; js < NR; js++) gr[js] = fabs(x.ld(f.gdot, clampi(js)));
+#pragma unroll
+    for (int js = 0; js < NR; js++) gr[js] = js < ns ? gr[js] : 0.0;
     const double gamma0 = x.ld(f.gacc, 0);
     double deltgam = 0.0;
 #pragma unroll
-    for (int js = 0; js < NR; js++)
-        if (js < ns) deltgam += gr[js] * tdot;
+    for (int js = 0; js < NR; js++) deltgam += gr[js] * tdot;
     const double gamma1 = gamma0 + deltgam;
-    const TB hard = tab + P.off_hard;
+    // Voce factor of the (single) mode
+    double sc;
+    {
+        const double tau1 = P.mode_tau1[0], th0 = P.mode_thet0[0], th1 = P.mode_thet1[0];
+        const double fact = fabs(th0 / tau1);
+        const double ex0 = exp(-fact * gamma0), ex1 = exp(-fact * gamma1);
+        const double voce = deltgam * th1 + tau1 * (ex0 - ex1) + th1 * (gamma0 * ex0 - gamma1 * ex1);
+        sc = voce * tdot / deltgam;
+    }
     double e2 = 0.0;
-    int m = -1, end = 0;
-    double sc = 0.0;
-#pragma unroll 1
-    for (int c00 = 0; c00 < ns; c00 += HC) {
-        const int c0 = __builtin_amdgcn_readfirstlane(c00);
+#pragma unroll
+    for (int c0 = 0; c0 < NR; c0 += HC) {
+        if (c0 >= ns) break;
         double crn[2 * HC], otn[2 * HC];
         load_chunk(c0 + HC < ns ? c0 + HC : c0, crn, otn);
 #pragma unroll
         for (int k = 0; k < HC; k++) {
             const int is = c0 + k;
             if (is >= ns) break;
-            while (is >= end) {  // the Voce factor of the mode this system belongs to
-                m = __builtin_amdgcn_readfirstlane(m + 1);
-                end += P.nsm[m];
-                const double tau1 = P.mode_tau1[m], th0 = P.mode_thet0[m], th1 = P.mode_thet1[m];
-                const double fact = fabs(th0 / tau1);
-                const double ex0 = exp(-fact * gamma0), ex1 = exp(-fact * gamma1);
-                const double voce = deltgam * th1 + tau1 * (ex0 - ex1) + th1 * (gamma0 * ex0 - gamma1 * ex1);
-                sc = voce * tdot / deltgam;
-            }
             double dtau = 0.0;
 #pragma unroll
             for (int js = 0; js < NR; js++)
-                if (js < ns) dtau += hard[is * ns + js] * gr[js];
+                dtau += tab[__builtin_amdgcn_readfirstlane(oh + is * ns) + js] * gr[js];  // table padded
             const double t0 = cr[2 * k] + dtau * sc;
             const double t1 = cr[2 * k + 1] + dtau * sc;
             const double o0 = ot[2 * k], o1 = ot[2 * k + 1];
@@ -401,51 +405,86 @@ __device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH
     return e2;
 }
 
-// harden_voxel for phases with more than 12 systems: dtau of 12 systems of a mode at a
-// time (js ascending), so each |gdot_js| is read once per chunk instead of once per system
+// harden_voxel for phases with more than 12 systems (or several modes): dtau of 12 systems
+// of a mode at a time, the rates read 4 at a time (in flight together) and each |gdot_js|
+// once per chunk; the rows of the interaction matrix are read 4 wide (the table is padded;
+// rates past the last system weigh zero).  dtau sums the same terms in the same order as
+// harden_voxel.
 template <class PH, class TB>
 __device__ __forceinline__ double harden_voxel_chunked(const Fields& f, TB tab, const PH& P, const Vox& x,
                                                        double tdot) {
-    const int ns = P.nsyst;
+    const int ns = __builtin_amdgcn_readfirstlane(P.nsyst);
+    const int oh = __builtin_amdgcn_readfirstlane(P.off_hard);
+    auto clampi = [&](int i) { return __builtin_amdgcn_readfirstlane(i < ns ? i : ns - 1); };
+    constexpr int JB = 4, C = 12, SB = 4;
+    auto rates = [&](int j0, double* g) {
+#pragma unroll
+        for (int jj = 0; jj < JB; jj++) g[jj] = fabs(x.ld(f.gdot, clampi(j0 + jj)));
+#pragma unroll
+        for (int jj = 0; jj < JB; jj++) g[jj] = j0 + jj < ns ? g[jj] : 0.0;
+    };
     double deltgam = 0.0;
-    for (int js = 0; js < ns; js++) deltgam += fabs(x.ld(f.gdot, uidx<true>(js))) * tdot;
+    for (int j00 = 0; j00 < ns; j00 += JB) {
+        const int j0 = __builtin_amdgcn_readfirstlane(j00);
+        double g[JB];
+        rates(j0, g);
+#pragma unroll
+        for (int jj = 0; jj < JB; jj++) deltgam += g[jj] * tdot;
+    }
     const double gamma0 = x.ld(f.gacc, 0);
     const double gamma1 = gamma0 + deltgam;
-    constexpr int C = 12;
-    const TB hard = tab + P.off_hard;
     double e2 = 0.0;
     int is = 0;
     for (int m0 = 0; m0 < P.nmodes; m0++) {
-        const int m = uidx<true>(m0);
+        const int m = __builtin_amdgcn_readfirstlane(m0);
         const double tau1 = P.mode_tau1[m], th0 = P.mode_thet0[m], th1 = P.mode_thet1[m];
         const double fact = fabs(th0 / tau1);
         const double ex0 = exp(-fact * gamma0), ex1 = exp(-fact * gamma1);
         const double voce = deltgam * th1 + tau1 * (ex0 - ex1) + th1 * (gamma0 * ex0 - gamma1 * ex1);
         const double sc = voce * tdot / deltgam;
-        const int nm = P.nsm[m];
+        const int nm = __builtin_amdgcn_readfirstlane(P.nsm[m]);
         for (int k0 = 0; k0 < nm; k0 += C) {
-            const int i0 = uidx<true>(is + k0);
-            const int nk = uidx<true>(nm - k0 < C ? nm - k0 : C);
+            const int i0 = __builtin_amdgcn_readfirstlane(is + k0);
+            const int nk = __builtin_amdgcn_readfirstlane(nm - k0 < C ? nm - k0 : C);
             double dt[C];
 #pragma unroll
             for (int k = 0; k < C; k++) dt[k] = 0.0;
-            for (int js = 0; js < ns; js++) {
-                const int j = uidx<true>(js);
-                const double g = fabs(x.ld(f.gdot, j));
+            for (int j00 = 0; j00 < ns; j00 += JB) {
+                const int j0 = __builtin_amdgcn_readfirstlane(j00);
+                double g[JB];
+                rates(j0, g);
 #pragma unroll
                 for (int k = 0; k < C; k++)
-                    if (k < nk) dt[k] += hard[(i0 + k) * ns + j] * g;
-            }
+                    if (k < nk) {
+                        const int row = __builtin_amdgcn_readfirstlane(oh + (i0 + k) * ns + j0);
 #pragma unroll
-            for (int k = 0; k < C; k++) {
-                if (k >= nk) break;
-                const int i = i0 + k;
-                const double t0 = x.ld(f.crss, 2 * i) + dt[k] * sc;
-                const double t1 = x.ld(f.crss, 2 * i + 1) + dt[k] * sc;
-                const double o0 = x.ld(f.ttau, 2 * i), o1 = x.ld(f.ttau, 2 * i + 1);
-                e2 += (o0 - t0) * (o0 - t0) + (o1 - t1) * (o1 - t1);
-                x.st(f.ttau, 2 * i, t0);
-                x.st(f.ttau, 2 * i + 1, t1);
+                        for (int jj = 0; jj < JB; jj++) dt[k] += tab[row + jj] * g[jj];
+                    }
+            }
+            // new strengths, SB systems at a time (their crss and old strengths in flight together)
+#pragma unroll
+            for (int s0 = 0; s0 < C; s0 += SB) {
+                if (s0 >= nk) break;
+                double cr[2 * SB], ot[2 * SB];
+#pragma unroll
+                for (int k = 0; k < SB; k++) {
+                    const int i = __builtin_amdgcn_readfirstlane(i0 + (s0 + k < nk ? s0 + k : nk - 1));
+                    cr[2 * k] = x.ld(f.crss, 2 * i);
+                    cr[2 * k + 1] = x.ld(f.crss, 2 * i + 1);
+                    ot[2 * k] = x.ld(f.ttau, 2 * i);
+                    ot[2 * k + 1] = x.ld(f.ttau, 2 * i + 1);
+                }
+#pragma unroll
+                for (int k = 0; k < SB; k++) {
+                    if (s0 + k >= nk) break;
+                    const int i = i0 + s0 + k;
+                    const double t0 = cr[2 * k] + dt[s0 + k] * sc;
+                    const double t1 = cr[2 * k + 1] + dt[s0 + k] * sc;
+                    const double o0 = ot[2 * k], o1 = ot[2 * k + 1];
+                    e2 += (o0 - t0) * (o0 - t0) + (o1 - t1) * (o1 - t1);
+                    x.st(f.ttau, 2 * i, t0);
+                    x.st(f.ttau, 2 * i + 1, t1);
+                }
             }
         }
         is += nm;
@@ -453,15 +492,6 @@ __device__ __forceinline__ double harden_voxel_chunked(const Fields& f, TB tab, 
     return e2;
 }
 
-// solve_res for a solid voxel without the modified-AL term (various_functions.f90:1142-1324).
-// The reference solves J d = R with J = I + cm (dedotp + s66/tdot) and
-// R = sg - lambda + cm (edotp + s66 (sg - sgt)/tdot - D).  Both factor through cm:
-// J = cm K, R = cm R' with the symmetric positive definite K = cm^-1 + s66/tdot + dedotp
-// and R' = A sg - b + edotp, A = cm^-1 + s66/tdot, b = cm^-1 lambda + s66 sgt/tdot + D,
-// so the Newton step d = K^-1 R' is the reference's step, and |R| = |cm R'| drives
-// the same line search.  Only the packed symmetric parts of cm and s66 (21 entries
-// each) are kept, and K is factored as L D L^T: the live state of the Newton loop is
-// about half of the J/LU form, which doubles the waves each SIMD can hold.
 template <bool INTPOW, bool SYMTAU, bool SMALLNS>
 // p0 is wave-uniform (the caller iterates over the phases present in the wave), so the
 // phase tables are read through scalar loads.
@@ -529,9 +559,16 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
     double sgnorm = 0.0, taun = 0.0;
     for (int c = 0; c < 6; c++) sgnorm += sg[c] * sg[c];
     sgnorm = sqrt(sgnorm);
-    for (int k = 0; k < 2 * P.nsyst; k++) {
-        const double t = x.ld(f.ttau, k);
-        taun += t * t;
+    {  // |ttau|: 8 loads in flight at a time (clamped indices, the extra terms are exact zeros)
+        const int nt = __builtin_amdgcn_readfirstlane(2 * P.nsyst);
+        for (int k00 = 0; k00 < nt; k00 += 8) {
+            const int k0 = __builtin_amdgcn_readfirstlane(k00);
+            double t[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) t[q] = x.ld(f.ttau, __builtin_amdgcn_readfirstlane(k0 + q < nt ? k0 + q : nt - 1));
+#pragma unroll
+            for (int q = 0; q < 8; q++) taun += k0 + q < nt ? t[q] * t[q] : 0.0;
+        }
     }
     taun = sqrt(taun);
     const double tol = fmax(pc.error, pc.erroral);
