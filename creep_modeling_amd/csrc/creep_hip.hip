@@ -137,6 +137,8 @@ struct cg_ctx {
     double* d_sgsum = nullptr;
     double* d_fluct = nullptr;
     size_t part_cap = 0;
+    int sop_valid = 0;     // f.sop holds the operators for sop_tdot and the current c066mod, sg66
+    double sop_tdot = 0.0;
     int sgsum_valid = 0, lold_same = 0, int_pow = 0, sym_tau = 0, small_ns = 0, ngas = 0, n3node = 1;
     long last_solid = -1;
     double nsys_solid = 0.0;
@@ -222,7 +224,7 @@ int cg_create(int n1, int n2, int n3, int nph, double complgas, int modal, int d
 static void free_fields(cg_ctx* c) {
     Fields& f = c->f;
     void* ptrs[] = {f.phase, f.gas, f.w, f.sg, f.sgt, f.L, f.Lold, f.dLref, f.F, f.Fi, f.J, f.Fe, f.Fp, f.ag,
-                    f.sg66, f.c66m, f.cgm, f.cgas, f.sch, f.schA, f.crss, f.ttau, f.gdot, f.gacc, f.edotp, f.ept, f.dgs,
+                    f.sg66, f.c66m, f.cgm, f.cgas, f.sop, f.sch, f.schA, f.crss, f.ttau, f.gdot, f.gacc, f.edotp, f.ept, f.dgs,
                     f.epvm, f.Finc, f.Feinc, f.s_L, f.s_Lold, f.s_sg, f.s_edotp, f.s_gdot, f.s_ttau, f.spec,
                     f.specf, f.xnode, f.part, f.red};
     for (void* p : ptrs)
@@ -377,6 +379,7 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     rc |= dalloc(c, &f.Fe, 9 * V); rc |= dalloc(c, &f.Fp, 9 * V); rc |= dalloc(c, &f.ag, 9 * V);
     rc |= dalloc(c, &f.sg66, 36 * V); rc |= dalloc(c, &f.c66m, 36 * V);
     if (c->modal) rc |= dalloc(c, &f.cgm, 36 * V);
+    else rc |= dalloc(c, &f.sop, 63 * V);
     if (c->ngas) rc |= dalloc(c, &f.cgas, 36 * V);
     rc |= dalloc(c, &f.sch, 5 * ns * V); rc |= dalloc(c, &f.schA, 18 * V); rc |= dalloc(c, &f.crss, 2 * ns * V); rc |= dalloc(c, &f.ttau, 2 * ns * V);
     rc |= dalloc(c, &f.gdot, ns * V); rc |= dalloc(c, &f.gacc, V);
@@ -502,6 +505,7 @@ int cg_begin_increment(cg_ctx* c, const cg_process* p, int do_schmid) {
         for (int i = 0; i < 81; i++) c->goperr0[i] = g[i] / (double)c->V;
     }
     launch_c066mod(c->f, c->c066, c->goperr0, c->modal, c->s);
+    c->sop_valid = 0;
     tstop(c, 3);
     HIPCHK(c, hipGetLastError());
     tcollect(c, 3);
@@ -543,6 +547,11 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     inv3(c->defgradinvavgc, finv_inv);
     PhaseTables pt{c->d_ph, c->d_tab};
     tstart(c, 1);
+    if (!c->modal && (!c->sop_valid || c->sop_tdot != p->tdot)) {
+        launch_solve_ops(f, p->tdot, c->s);
+        c->sop_valid = 1;
+        c->sop_tdot = p->tdot;
+    }
     launch_correct_solve_full(f, pt, pc, c->dvelgradavg, finv_inv, c->d_fluct, c->s, &nb, c->int_pow, c->sym_tau, c->small_ns);
     tstop(c, 1);
     launch_reduce_final(f.part, nb, R_NSOLVE, f.red, c->s);
@@ -707,6 +716,7 @@ int cg_end_increment(cg_ctx* c, const cg_process* p, int scan_diagnostics, cg_in
         }
         launch_div_w(f, detfavg, c->s);
         launch_elastic_stiff(f, pt, c->s);
+        c->sop_valid = 0;
         launch_tensor_config_a(f, c->s, &nb);
         if (reduce_to_host(c, nb, 9, r)) return -2;
         double inv[9], dm[9], dud[9];
@@ -899,6 +909,7 @@ int cg_get_field(cg_ctx* c, const char* name, double* out, size_t n) {
 
 int cg_set_field(cg_ctx* c, const char* name, const double* in, size_t n) {
     if (!c || !c->initialised || !name || !in) return -1;
+    c->sop_valid = 0;
     FieldDesc d;
     if (!field_desc(c, name, d) || !d.p) return fail(c, -1, "unknown or absent field %s", name);
     const long V = c->V;

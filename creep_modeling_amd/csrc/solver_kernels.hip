@@ -516,35 +516,19 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
         }
         for (int c = 0; c < 9; c++) L[c] = x.ld(f.L, c);
         to6(L, D);
-        {
-            double Lf[21];
+        // the operators of this increment (k_solve_ops): packed cm -> LDS, A = cm^-1 + S,
+        // b = cm^-1 lambda + D + S sgt with S = sym(s66)/tdot
 #pragma unroll
-            for (int i = 0; i < 6; i++)
+        for (int i = 0; i < 21; i++) cml[i * BS] = x.ld(f.sop, i);
 #pragma unroll
-                for (int j = i; j < 6; j++)
-                    Lf[sp6(i, j)] = i == j ? x.ld(f.c66m, 7 * i)
-                                           : 0.5 * (x.ld(f.c66m, 6 * i + j) + x.ld(f.c66m, 6 * j + i));
-#pragma unroll
-            for (int i = 0; i < 21; i++) cml[i * BS] = Lf[i];
-            ldl6(Lf);
-#pragma unroll
-            for (int j = 0; j < 6; j++) {  // columns of cm^-1 (upper part kept)
-                double e[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                e[j] = 1.0;
-                ldl6_solve(Lf, e);
-#pragma unroll
-                for (int i = 0; i <= j; i++) A[sp6(i, j)] = e[i];
-            }
-        }
+        for (int i = 0; i < 21; i++) A[i] = x.ld(f.sop, 21 + i);
         spmv6(A, lam, b);
         for (int i = 0; i < 6; i++) b[i] += D[i];
-        // + s66/tdot (symmetric part) into A, and s66 sgt/tdot into b
 #pragma unroll
         for (int i = 0; i < 6; i++)
 #pragma unroll
             for (int j = i; j < 6; j++) {
-                const double sij = i == j ? x.ld(f.sg66, 7 * i) / tdot
-                                          : 0.5 * (x.ld(f.sg66, 6 * i + j) + x.ld(f.sg66, 6 * j + i)) / tdot;
+                const double sij = x.ld(f.sop, 42 + sp6(i, j));
                 A[sp6(i, j)] += sij;
                 b[i] += sij * sgt[j];
                 if (j != i) b[j] += sij * sgt[i];
@@ -651,7 +635,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
         for (int c = 0; c < 6; c++) sev[c] = parked ? x.ld(f.edotp, c) : sgold[c];
         for (int i = 0; i < 6; i++) {
             double s = 0.0;
-            for (int k = 0; k < 6; k++) s += x.ld(f.sg66, 6 * i + k) * ((sev[k] - sgt[k]) / tdot);
+            for (int k = 0; k < 6; k++) s += x.ld(f.sop, 42 + sp6(i, k)) * (sev[k] - sgt[k]);
             const double e = (i < 5 ? ep[i] : 0.0) + s - D[i];
             dd2 += e * e;
         }
@@ -1374,6 +1358,42 @@ __global__ __launch_bounds__(BS) void k_hardening(Fields f, PhaseTables pt, doub
     harden_voxel(f, pt.data, P, Vox(v, V), tdot, &g1, true);
     f.gacc[v] = g1;
 }
+// Operators of the symmetric Newton solve for one increment (constant while c066mod,
+// sg66 and tdot are): f.sop = [cm (21) | cm^-1 (21) | S = sym(s66)/tdot (21)], packed
+// upper triangles; cm^-1 from the L D L^T of cm.
+__global__ __launch_bounds__(BS) void k_solve_ops(Fields f, double tdot) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    if (v >= V || f.gas[v]) return;
+    const Vox x(v, V);
+    double Lf[21];
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+        for (int j = i; j < 6; j++)
+            Lf[sp6(i, j)] = i == j ? x.ld(f.c66m, 7 * i) : 0.5 * (x.ld(f.c66m, 6 * i + j) + x.ld(f.c66m, 6 * j + i));
+#pragma unroll
+    for (int i = 0; i < 21; i++) x.st(f.sop, i, Lf[i]);
+    ldl6(Lf);
+#pragma unroll
+    for (int j = 0; j < 6; j++) {  // columns of cm^-1 (upper part kept)
+        double e[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        e[j] = 1.0;
+        ldl6_solve(Lf, e);
+#pragma unroll
+        for (int i = 0; i <= j; i++) x.st(f.sop, 21 + sp6(i, j), e[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+        for (int j = i; j < 6; j++)
+            x.st(f.sop, 42 + sp6(i, j),
+                 i == j ? x.ld(f.sg66, 7 * i) / tdot : 0.5 * (x.ld(f.sg66, 6 * i + j) + x.ld(f.sg66, 6 * j + i)) / tdot);
+}
+void launch_solve_ops(const Fields& f, double tdot, hipStream_t s) {
+    hipLaunchKernelGGL(k_solve_ops, dim3(nblk(f.V)), dim3(BS), 0, s, f, tdot);
+}
+
 void launch_hardening(const Fields& f, const PhaseTables& pt, double tdot, hipStream_t s) {
     hipLaunchKernelGGL(k_hardening, dim3(nblk(f.V)), dim3(BS), 0, s, f, pt, tdot);
 }
