@@ -395,7 +395,7 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     rc |= dalloc(c, &f.xnode, 3 * (size_t)(f.n1 + 1) * (f.n2 + 1) * c->n3node);
     rc |= dalloc(c, &f.red, 256);
     if (rc) return rc;
-    if (ensure_part(c, (size_t)((V + 255) / 256 + 64) * 81)) return -3;
+    if (ensure_part(c, (size_t)((V + 63) / 64 + 64) * 81)) return -3;  // the solve may run 64-thread blocks
     HIPCHK(c, hipHostMalloc(&c->h_red, 256 * sizeof(double)));
     if (dalloc(c, &c->d_sgsum, 16) || dalloc(c, &c->d_fluct, 16)) return -3;
     c->fft = fft_create(f, c->err, sizeof c->err);
@@ -554,7 +554,7 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     }
     launch_correct_solve_full(f, pt, pc, c->dvelgradavg, finv_inv, c->d_fluct, c->s, &nb, c->int_pow, c->sym_tau, c->small_ns);
     tstop(c, 1);
-    launch_reduce_final(f.part, nb, R_NSOLVE, f.red, c->s);
+    launch_reduce_final_cm(f.part, nb, R_NSOLVE, f.red, c->s);
     // next pass' <sigma> = sum_all w sigma
     launch_copy(c->d_sgsum, f.red + R_SGALL, 9, c->s);
     HIPCHK(c, hipMemcpyAsync(c->h_red, f.red, R_NSOLVE * sizeof(double), hipMemcpyDeviceToHost, c->s));
@@ -660,7 +660,7 @@ int cg_rollback(cg_ctx* c, cg_process* p) {
 int cg_kinematic_substep(cg_ctx* c, double tdot, double* out4, int* nsub) {
     if (!c || !c->initialised) return -1;
     int nb;
-    if (ensure_part(c, (size_t)((c->V + 255) / 256 + 64) * 81)) return -3;
+    if (ensure_part(c, (size_t)((c->V + 63) / 64 + 64) * 81)) return -3;
     launch_kinematic(c->f, tdot, c->s, &nb);
     double r[4];
     HIPCHK(c, hipMemcpyAsync(c->h_red, c->f.red, 4 * sizeof(double), hipMemcpyDeviceToHost, c->s));

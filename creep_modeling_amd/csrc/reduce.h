@@ -103,4 +103,48 @@ __device__ __forceinline__ void block_reduce_lds(const double* acc, double* part
     }
 }
 
+// Column-major variants: block b writes its sums to part[(off + i) * gridDim.x + b], so the
+// final reduction of value i reads one contiguous column (launches with many blocks).
+template <int NV, int BS>
+__device__ __forceinline__ void block_reduce_store_cm(const double* acc, double* part, int off) {
+    static_assert(NV <= BS, "NV must not exceed the block size");
+    __shared__ double sm[(BS / 64) * NV];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        double v = acc[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += wave_shfl_down(v, o);
+        if (lane == 0) sm[wid * NV + i] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = sm[threadIdx.x];
+        for (int w = 1; w < BS / 64; w++) s += sm[w * NV + threadIdx.x];
+        part[(size_t)(off + threadIdx.x) * gridDim.x + blockIdx.x] = s;
+    }
+}
+template <int NV, int BS>
+__device__ __forceinline__ void block_reduce_lds_cm(const double* acc, double* part, int off, double* sm) {
+    static_assert(NV * (BS / 32) <= BS, "one thread per (value, chunk)");
+    constexpr int NC = BS / 32;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < NV; i++) sm[i * BS + t] = acc[i];
+    __syncthreads();
+    double s = 0.0;
+    if (t < NV * NC) {
+        const double* c = sm + (t / NC) * BS + (t % NC) * 32;
+        for (int k = 0; k < 32; k++) s += c[k];
+    }
+    __syncthreads();
+    if (t < NV * NC) sm[(t / NC) * BS + (t % NC) * 32] = s;
+    __syncthreads();
+    if (t < NV) {
+        double r = sm[t * BS];
+        for (int c = 1; c < NC; c++) r += sm[t * BS + c * 32];
+        part[(size_t)(off + t) * gridDim.x + blockIdx.x] = r;
+    }
+}
+
 }  // namespace cg

@@ -14,6 +14,21 @@
 namespace cg {
 
 constexpr int BS = 256;
+// threads per block of the correction + constitutive kernel
+#ifndef CG_SOLVE_BS
+#define CG_SOLVE_BS 64
+#endif
+constexpr int SBS = CG_SOLVE_BS;
+// Block b of a launch runs on XCD b % 8.  With CG_XCD_MAP the solve kernel gives each XCD a
+// contiguous eighth of the voxels, walked in order.
+#ifndef CG_XCD_MAP
+#define CG_XCD_MAP 0
+#endif
+__device__ __forceinline__ long solve_block(long b, long nb) {
+    if (!CG_XCD_MAP) return b;
+    const long q = nb / 8;
+    return b < 8 * q ? (b % 8) * q + b / 8 : b;
+}
 
 // Phase tables seen through the constant address space: with a wave-uniform phase id
 // and uniform indices (readfirstlane) the compiler reads them with scalar loads, which
@@ -63,6 +78,26 @@ __global__ __launch_bounds__(BS) void k_reduce_final(const double* __restrict__ 
         for (int w = 1; w < BS / 64; w++) t += sm[w];
         out[i] = t;
     }
+}
+// the same over a column-major partial array part[i * nblocks + b]
+__global__ __launch_bounds__(BS) void k_reduce_final_cm(const double* __restrict__ part, int nblocks,
+                                                        double* __restrict__ out) {
+    const int i = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += BS) s += part[(size_t)i * nblocks + b];
+    __shared__ double sm[BS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (lane == 0) sm[wid] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = sm[0];
+        for (int w = 1; w < BS / 64; w++) t += sm[w];
+        out[i] = t;
+    }
+}
+void launch_reduce_final_cm(const double* part, int nblocks, int nvals, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_final_cm, dim3(nvals), dim3(BS), 0, s, part, nblocks, out);
 }
 void launch_reduce_final(const double* part, int nblocks, int nvals, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_final, dim3(nvals), dim3(BS), 0, s, part, nblocks, nvals, out);
@@ -168,7 +203,7 @@ __device__ __forceinline__ void slip_eval(const Fields& f, const double* __restr
     }
 }
 // slip_eval for the symmetric Newton solve: the Schmid tensors are rebuilt from A, A^-1
-// (this thread's column of an LDS array [18][BS]) and the wave-uniform crystal b, n, so
+// (this thread's column of an LDS array [18][SBS]) and the wave-uniform crystal b, n, so
 // an evaluation reads 2 n_s trial strengths from memory instead of 5 n_s Schmid
 // components; the next system's strengths are loaded while the current one is computed.
 // SYMTAU: the forward and backward trial strengths are equal (tau0f = tau0b for every
@@ -201,8 +236,8 @@ __device__ __forceinline__ void slip_eval_ab(const Fields& f, TB tab, const PH& 
     double A[9], Ai[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        A[k] = sal[k * BS];
-        Ai[k] = sal[(9 + k) * BS];
+        A[k] = sal[k * SBS];
+        Ai[k] = sal[(9 + k) * SBS];
     }
     if (ACC) {
         for (int i = 0; i < 5; i++) ep[i] = 0.0;
@@ -495,10 +530,10 @@ __device__ __forceinline__ double harden_voxel_chunked(const Fields& f, TB tab, 
 template <bool INTPOW, bool SYMTAU, bool SMALLNS>
 // p0 is wave-uniform (the caller iterates over the phases present in the wave), so the
 // phase tables are read through scalar loads.
-// cml: this thread's column of a block-shared LDS array [21][BS] holding the packed
+// cml: this thread's column of a block-shared LDS array [21][SBS] holding the packed
 // symmetric cm, read once per evaluation for the residual norm (keeping it out of the
 // registers lets three waves share a SIMD).
-// sal: this thread's column of an LDS array [18][BS] that receives A, A^-1 (schA).
+// sal: this thread's column of an LDS array [18][SBS] that receives A, A^-1 (schA).
 __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTables& pt, const PassConst& pc,
                                                 const Vox& x, int p0, double* cml, double* sal, double* sgf,
                                                 double* acc) {
@@ -506,7 +541,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
     cdbl* tab = (cdbl*)pt.data;
     const double tdot = pc.tdot;
 #pragma unroll
-    for (int k = 0; k < 18; k++) sal[k * BS] = x.ld(f.schA, k);
+    for (int k = 0; k < 18; k++) sal[k * SBS] = x.ld(f.schA, k);
     double A[21], b[6];
     {
         double lam[6], sgt[6], D[6], L[9];
@@ -519,7 +554,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
         // the operators of this increment (k_solve_ops): packed cm -> LDS, A = cm^-1 + S,
         // b = cm^-1 lambda + D + S sgt with S = sym(s66)/tdot
 #pragma unroll
-        for (int i = 0; i < 21; i++) cml[i * BS] = x.ld(f.sop, i);
+        for (int i = 0; i < 21; i++) cml[i * SBS] = x.ld(f.sop, i);
 #pragma unroll
         for (int i = 0; i < 21; i++) A[i] = x.ld(f.sop, 21 + i);
         spmv6(A, lam, b);
@@ -583,7 +618,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
                 for (int i = 0; i < 6; i++) {
                     double s = 0.0;
 #pragma unroll
-                    for (int k = 0; k < 6; k++) s += cml[sp6(i, k) * BS] * rp[k];
+                    for (int k = 0; k < 6; k++) s += cml[sp6(i, k) * SBS] * rp[k];
                     rn += s * s;
                 }
                 resn = sqrt(rn);
@@ -652,17 +687,17 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
 }
 
 template <bool MODAL, bool INTPOW, bool SYMTAU, bool SMALLNS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 2))) void k_correct_solve(Fields f, PhaseTables pt, SolveArgs a) {
+__global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 2))) void k_correct_solve(Fields f, PhaseTables pt, SolveArgs a) {
     const long V = f.V;
-    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    const long v = solve_block(blockIdx.x, gridDim.x) * SBS + threadIdx.x;
     const bool act = v < V;
     const PassConst& pc = a.pc;
     const Vox x(v, V);
     // LDS of the symmetric solve: per thread 21 (packed cm) + 18 (A, A^-1) doubles;
     // the block reductions reuse it when no solve is using it
-    __shared__ double pool[MODAL ? 1 : 39 * BS];
+    __shared__ double pool[MODAL ? 1 : 39 * SBS];
     double* const cm_lds = pool;
-    double* const sa_lds = pool + 21 * BS;
+    double* const sa_lds = pool + 21 * SBS;
     // ---- phase A: correction_to_current + dL_iter/mixing; its 31 sums are reduced
     // before the constitutive solve so their accumulators are not live across it
     double w = 0.0;
@@ -715,9 +750,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 
             for (int c = 0; c < 9; c++) x.st(f.L, c, L[c]);
         }
         if (MODAL) {
-            block_reduce_store_at<R_NSOLVE - R_NPHASEB, BS>(accA, f.part, R_NSOLVE, R_NPHASEB);
+            block_reduce_store_cm<R_NSOLVE - R_NPHASEB, SBS>(accA, f.part, R_NPHASEB);
         } else {  // scratch aliased onto the solve's LDS columns, free until phase B
-            block_reduce_lds<R_NSOLVE - R_NPHASEB, BS>(accA, f.part, R_NSOLVE, R_NPHASEB, pool);
+            block_reduce_lds_cm<R_NSOLVE - R_NPHASEB, SBS>(accA, f.part, R_NPHASEB, pool);
             __syncthreads();
         }
     }
@@ -916,8 +951,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 
         }
     }
     __syncthreads();
-    if (MODAL) block_reduce_store_at<R_NPHASEB, BS>(acc, f.part, R_NSOLVE, 0);
-    else block_reduce_lds<R_NPHASEB, BS>(acc, f.part, R_NSOLVE, 0, pool);  // every solve has finished
+    if (MODAL) block_reduce_store_cm<R_NPHASEB, SBS>(acc, f.part, 0);
+    else block_reduce_lds_cm<R_NPHASEB, SBS>(acc, f.part, 0, pool);  // every solve has finished
 }
 
 // Public launcher with the full argument set.
@@ -928,15 +963,15 @@ void launch_correct_solve_full(const Fields& f, const PhaseTables& pt, const Pas
     a.pc = pc;
     for (int i = 0; i < 9; i++) { a.dvelgradavg[i] = dvelgradavg[i]; a.finv_inv[i] = finv_inv[i]; }
     a.fluct = fluct_dev;
-    *nb = nblk(f.V);
+    *nb = (int)((f.V + SBS - 1) / SBS);
     if (pc.modal) {
-        if (int_pow) hipLaunchKernelGGL((k_correct_solve<true, true, false, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
-        else hipLaunchKernelGGL((k_correct_solve<true, false, false, false>), dim3(*nb), dim3(BS), 0, s, f, pt, a);
+        if (int_pow) hipLaunchKernelGGL((k_correct_solve<true, true, false, false>), dim3(*nb), dim3(SBS), 0, s, f, pt, a);
+        else hipLaunchKernelGGL((k_correct_solve<true, false, false, false>), dim3(*nb), dim3(SBS), 0, s, f, pt, a);
         return;
     }
     // the symmetric-solve kernel is instantiated per (integer exponents, equal forward and
     // backward strengths, at most 12 systems per phase)
-#define CG_SOLVE(IP, ST, SM) hipLaunchKernelGGL((k_correct_solve<false, IP, ST, SM>), dim3(*nb), dim3(BS), 0, s, f, pt, a)
+#define CG_SOLVE(IP, ST, SM) hipLaunchKernelGGL((k_correct_solve<false, IP, ST, SM>), dim3(*nb), dim3(SBS), 0, s, f, pt, a)
     switch ((int_pow ? 4 : 0) + (sym_tau ? 2 : 0) + (small_ns ? 1 : 0)) {
         case 0: CG_SOLVE(false, false, false); break;
         case 1: CG_SOLVE(false, false, true); break;
