@@ -109,6 +109,45 @@ def cpu_baseline(problem, max_passes=3):
                        "setup and updates), %.1f s" % (*problem.n, r.passes, t1 - t0))
 
 
+def bench_sweep(args):
+    """The batched sweep: `rves` independent FCC RVEs, `warmup + steps` increments each, all
+    contexts created (and their microstructures uploaded) before the timed region; the timed
+    region runs every RVE's increments with `workers` RVEs in flight."""
+    import torch
+
+    from creep_modeling_amd import batch, driver, inputs, parse
+
+    cases = inputs.sweep_cases(args.rves, nsteps=args.warmup + args.steps)
+    problems = [parse.problem_from_case(*c) for _, c in cases]
+    torch.cuda.init()
+    for prec in ((0,) if args.no_mixed else (0, 1)):
+        solvers = [driver.Solver(p, precision=prec, scan_cond=False) for p in problems]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rows = batch.run_batch(problems, workers=args.workers, solvers=solvers)
+        torch.cuda.synchronize()
+        T = time.perf_counter() - t0
+        vi = sum(sum(r["passes"] for r in rr) * p.nvox for rr, p in zip(rows, problems))
+        print("sweep precision %d: %.2f s" % (prec, T), file=sys.stderr, flush=True)
+        if prec == 0:
+            out = {
+                "metric": "voxel-iters/s, batched sweep of independent RVEs (aggregate)",
+                "value": vi / T, "unit": "voxel-iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": T / (args.warmup + args.steps) * 1e3, "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f64",
+                "data": "synthetic (seeded periodic Voronoi RVEs, uniform SO(3) orientations)",
+                "config": {"workload": "%d FCC RVEs, edges in {32,48,64,96}, 32-512 grains, sigma33 200-400 MPa, "
+                                       "n 8-14, Arrhenius-scaled gamma0 (475/525/575 C); %d increments each, "
+                                       "%d RVEs in flight; step = one increment of every RVE"
+                                       % (args.rves, args.warmup + args.steps, args.workers),
+                           "voxels": sum(p.nvox for p in problems),
+                           "passes": sum(sum(r["passes"] for r in rr) for rr in rows)},
+            }
+        else:
+            out["mixed"] = {"value": vi / T, "ratio_vs_fp64": (vi / T) / out["value"]}
+    print(json.dumps(out))
+
+
 def copy_calibration(gib=1.0, reps=5):
     """A plain device copy of known size (torch's device-to-device copy, which runs the HIP
     runtime's copy kernel), to scale the FETCH_SIZE / WRITE_SIZE counters: each launch
@@ -133,8 +172,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3),
-                    help="1: FCC n=10 (the headline workload); 2: FCC n=15; 3: BCC 48 systems, biaxial, dt=0.5 s")
+    ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3, 4),
+                    help="1: FCC n=10 (the headline workload); 2: FCC n=15; 3: BCC 48 systems, biaxial, dt=0.5 s; "
+                         "4: batched sweep of independent FCC RVEs (32^3-96^3)")
+    ap.add_argument("--rves", type=int, default=64, help="config 4: number of RVEs in the sweep")
+    ap.add_argument("--workers", type=int, default=4, help="config 4: RVEs in flight at a time")
     ap.add_argument("--n", type=int, default=None, help="grid edge (default 128 for configs 1-2, 256 for 3)")
     ap.add_argument("--grains", type=int, default=None, help="Voronoi grains (default 512, 2048 for config 3)")
     ap.add_argument("--s33", type=float, default=None, help="creep stress sigma33 in MPa (config default 300)")
@@ -154,6 +196,9 @@ def main():
 
     from creep_modeling_amd import inputs, parse
 
+    if args.config == 4:
+        bench_sweep(args)
+        return
     if args.n is None:
         args.n = 256 if args.config == 3 else 128
     if args.grains is None:

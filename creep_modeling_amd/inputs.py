@@ -326,6 +326,36 @@ def hall_petch_shift(nvox):
     return (632.0 / 2.5) / math.sqrt(d)
 
 
+# Arrhenius scaling of the reference slip rate for the batched sweep: Q = 280 kJ/mol, the
+# factor is 1 at 525 C
+SWEEP_Q = 280.0e3
+SWEEP_TREF = 525.0 + 273.15
+R_GAS = 8.314462618
+
+
+def sweep_cases(count=64, nsteps=100, seed=4000, sizes=(32, 48, 64, 96)):
+    """The batched sweep of independent FCC RVEs: grid edge uniform over `sizes`, grain count
+    uniform in 32..512, sigma33 uniform in 200-400 MPa, exponent n uniform in 8-14, and
+    gamma0_dot scaled by exp(-Q/R (1/T - 1/Tref)) for T uniform over {475, 525, 575} C.
+    RVE i uses microstructure seed seed + i.  Returns a list of (params, (ms, mat, [proc]))."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.choice(sizes))
+        ng = int(rng.integers(32, 513))
+        s33 = float(rng.uniform(200.0, 400.0))
+        nrs = float(rng.uniform(8.0, 14.0))
+        tc = float(rng.choice((475.0, 525.0, 575.0)))
+        fac = float(np.exp(-SWEEP_Q / R_GAS * (1.0 / (tc + 273.15) - 1.0 / SWEEP_TREF)))
+        mat = fcc_material(n=nrs, gamd0=1.0e-3 * fac)
+        proc = CreepProcess(s33=s33, tdot=1.0, tdotref=1.0, tdotmin=1.0, nsteps=nsteps)
+        ms = voronoi_rve(n, ng, seed + i)
+        out.append((dict(n=n, grains=ng, s33=s33, nrs=nrs, temp_c=tc, gamd0=1.0e-3 * fac), (ms, mat, [proc])))
+    return out
+
+
 def config_case(config, n=None, ngrains=None, nsteps=None, s33=None, nrs=None):
     """(Microstructure, Material, [CreepProcess]) for the named workloads.
 

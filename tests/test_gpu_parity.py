@@ -242,3 +242,13 @@ def test_mixed_precision_close_to_fp64():
         # macroscopic creep rate and stress of the mixed form against the FP64 GPU form
         assert abs(r1["edotpavg"][2] - r0["edotpavg"][2]) <= 1e-5 * abs(r0["edotpavg"][2])
         assert relerr(r0["sgsolavg"], r1["sgsolavg"]) < 1e-5
+
+
+def test_non_integer_exponent_matches_oracle():
+    # n = 9.5: the slip law goes through pow() instead of exponentiation by squaring
+    p = problem(0, n=8, ngrains=4, nsteps=2, nrs=9.5)
+    assert p.phases[0].nrs[0] == 9.5
+    rows, fo, fg = run_pair(p, 2)
+    check_rows(rows)
+    for k in fo:
+        assert relerr(fo[k], fg[k]) < FIELD_TOL, k
