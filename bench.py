@@ -176,7 +176,7 @@ def main():
                     help="1: FCC n=10 (the headline workload); 2: FCC n=15; 3: BCC 48 systems, biaxial, dt=0.5 s; "
                          "4: batched sweep of independent FCC RVEs (32^3-96^3)")
     ap.add_argument("--rves", type=int, default=64, help="config 4: number of RVEs in the sweep")
-    ap.add_argument("--workers", type=int, default=4, help="config 4: RVEs in flight at a time")
+    ap.add_argument("--workers", type=int, default=8, help="config 4: RVEs in flight at a time")
     ap.add_argument("--n", type=int, default=None, help="grid edge (default 128 for configs 1-2, 256 for 3)")
     ap.add_argument("--grains", type=int, default=None, help="Voronoi grains (default 512, 2048 for config 3)")
     ap.add_argument("--s33", type=float, default=None, help="creep stress sigma33 in MPa (config default 300)")

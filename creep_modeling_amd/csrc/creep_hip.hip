@@ -138,6 +138,7 @@ struct cg_ctx {
     double* d_fluct = nullptr;
     size_t part_cap = 0;
     int sop_valid = 0;     // f.sop holds the operators for sop_tdot and the current c066mod, sg66
+    int taun_valid = 0;    // f.taun = |ttau| (false after ttau was written other than by a harden)
     double sop_tdot = 0.0;
     int sgsum_valid = 0, lold_same = 0, int_pow = 0, sym_tau = 0, small_ns = 0, ngas = 0, n3node = 1;
     long last_solid = -1;
@@ -224,7 +225,7 @@ int cg_create(int n1, int n2, int n3, int nph, double complgas, int modal, int d
 static void free_fields(cg_ctx* c) {
     Fields& f = c->f;
     void* ptrs[] = {f.phase, f.gas, f.w, f.sg, f.sgt, f.L, f.Lold, f.dLref, f.F, f.Fi, f.J, f.Fe, f.Fp, f.ag,
-                    f.sg66, f.c66m, f.cgm, f.cgas, f.sop, f.sch, f.schA, f.crss, f.ttau, f.gdot, f.gacc, f.edotp, f.ept, f.dgs,
+                    f.sg66, f.c66m, f.cgm, f.cgas, f.sop, f.sch, f.schA, f.crss, f.ttau, f.taun, f.gdot, f.gacc, f.edotp, f.ept, f.dgs,
                     f.epvm, f.Finc, f.Feinc, f.s_L, f.s_Lold, f.s_sg, f.s_edotp, f.s_gdot, f.s_ttau, f.spec,
                     f.specf, f.xnode, f.part, f.red};
     for (void* p : ptrs)
@@ -382,6 +383,7 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     else rc |= dalloc(c, &f.sop, 63 * V);
     if (c->ngas) rc |= dalloc(c, &f.cgas, 36 * V);
     rc |= dalloc(c, &f.sch, 5 * ns * V); rc |= dalloc(c, &f.schA, 18 * V); rc |= dalloc(c, &f.crss, 2 * ns * V); rc |= dalloc(c, &f.ttau, 2 * ns * V);
+    rc |= dalloc(c, &f.taun, V);
     rc |= dalloc(c, &f.gdot, ns * V); rc |= dalloc(c, &f.gacc, V);
     rc |= dalloc(c, &f.edotp, 6 * V); rc |= dalloc(c, &f.ept, 6 * V); rc |= dalloc(c, &f.dgs, 6 * V);
     rc |= dalloc(c, &f.epvm, V); rc |= dalloc(c, &f.Finc, 9 * V); rc |= dalloc(c, &f.Feinc, 9 * V);
@@ -437,6 +439,7 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     int nb;
     launch_init_xnode(f, c->n3node, c->s);
     launch_init_voxels(f, pt, d_eul, d_gr, d_hp, p1->udot, p1->tdot, c->s, &nb);
+    c->taun_valid = 0;
     double sums[36];
     if (reduce_to_host(c, nb, 36, sums)) return -2;
     hipFree(d_eul); hipFree(d_hp); hipFree(d_gr);
@@ -547,6 +550,10 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     inv3(c->defgradinvavgc, finv_inv);
     PhaseTables pt{c->d_ph, c->d_tab};
     tstart(c, 1);
+    if (!c->taun_valid) {
+        launch_taun(f, pt, c->s);
+        c->taun_valid = 1;
+    }
     if (!c->modal && (!c->sop_valid || c->sop_tdot != p->tdot)) {
         launch_solve_ops(f, p->tdot, c->s);
         c->sop_valid = 1;
@@ -652,6 +659,7 @@ int cg_rollback(cg_ctx* c, cg_process* p) {
     memcpy(p->scauchy, c->scauchy_snap, sizeof c->scauchy_snap);
     launch_snapshot(c->f, 1, c->s);
     c->sgsum_valid = 0;
+    c->taun_valid = 0;
     c->lold_same = 0;
     HIPCHK(c, hipStreamSynchronize(c->s));
     return 0;
@@ -910,6 +918,7 @@ int cg_get_field(cg_ctx* c, const char* name, double* out, size_t n) {
 int cg_set_field(cg_ctx* c, const char* name, const double* in, size_t n) {
     if (!c || !c->initialised || !name || !in) return -1;
     c->sop_valid = 0;
+    c->taun_valid = 0;
     FieldDesc d;
     if (!field_desc(c, name, d) || !d.p) return fail(c, -1, "unknown or absent field %s", name);
     const long V = c->V;

@@ -41,6 +41,7 @@ struct Fields {
     double *sg66, *c66m, *cgm, *cgas;   // 36 each (cgm only if modal, cgas only if gas voxels)
     double *sop;          // 63 (without modified AL): packed cm, cm^-1, sym(s66)/tdot of the symmetric solve
     double *sch, *crss, *ttau, *gdot, *gacc;  // 5ns, 2ns, 2ns, ns, 1
+    double *taun;         // 1: |ttau| (columns in order), written with ttau by every harden and by k_taun
     double *schA;         // 18: A = Fe ag and A^-1 of update_schmid (sch_s = sym(A b_s x A^-T n_s))
     double *edotp, *ept, *dgs, *epvm;   // 6,6,6,1
     double *Finc, *Feinc;               // 9,9
@@ -93,6 +94,7 @@ void launch_tensor_config_a(const Fields& f, hipStream_t s, int* nb);
 void launch_tensor_config_b(const Fields& f, const double* dud, hipStream_t s);
 void launch_hardening(const Fields& f, const PhaseTables& pt, double tdot, hipStream_t s);
 void launch_solve_ops(const Fields& f, double tdot, hipStream_t s);
+void launch_taun(const Fields& f, const PhaseTables& pt, hipStream_t s);
 void launch_averages(const Fields& f, const PhaseTables& pt, hipStream_t s, int* nb);
 void launch_diagnostics(const Fields& f, double lfloor, int scan_cond, hipStream_t s, int* nb);
 void launch_calc_c0(const Fields& f, const PhaseTables& pt, double tdot, double tdotref, hipStream_t s, int* nb,

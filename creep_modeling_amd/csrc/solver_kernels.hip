@@ -333,7 +333,7 @@ __device__ __forceinline__ double harden_voxel(const Fields& f, TB tab, const PH
     const double gamma1 = gamma0 + deltgam;
     *gamma1_out = gamma1;
     const TB hard = tab + P.off_hard;
-    double e2 = 0.0;
+    double e2 = 0.0, tn2 = 0.0;  // tn2: |new ttau|^2, columns in order
     int is = 0;
     for (int m0 = 0; m0 < P.nmodes; m0++) {
         const int m = uidx<UNI>(m0);
@@ -356,6 +356,8 @@ __device__ __forceinline__ double harden_voxel(const Fields& f, TB tab, const PH
             const double t1 = x.ld(f.crss, 2 * is + 1) + dtau * sc;
             const double o0 = x.ld(f.ttau, 2 * is), o1 = x.ld(f.ttau, 2 * is + 1);
             e2 += (o0 - t0) * (o0 - t0) + (o1 - t1) * (o1 - t1);
+            tn2 += t0 * t0;
+            tn2 += t1 * t1;
             x.st(f.ttau, 2 * is, t0);
             x.st(f.ttau, 2 * is + 1, t1);
             if (store_crss) {
@@ -364,6 +366,7 @@ __device__ __forceinline__ double harden_voxel(const Fields& f, TB tab, const PH
             }
         }
     }
+    x.st(f.taun, 0, sqrt(tn2));
     return e2;
 }
 
@@ -410,7 +413,7 @@ __device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH
         const double voce = deltgam * th1 + tau1 * (ex0 - ex1) + th1 * (gamma0 * ex0 - gamma1 * ex1);
         sc = voce * tdot / deltgam;
     }
-    double e2 = 0.0;
+    double e2 = 0.0, tn2 = 0.0;  // tn2: |new ttau|^2, columns in order
 #pragma unroll
     for (int c0 = 0; c0 < NR; c0 += HC) {
         if (c0 >= ns) break;
@@ -428,6 +431,8 @@ __device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH
             const double t1 = cr[2 * k + 1] + dtau * sc;
             const double o0 = ot[2 * k], o1 = ot[2 * k + 1];
             e2 += (o0 - t0) * (o0 - t0) + (o1 - t1) * (o1 - t1);
+            tn2 += t0 * t0;
+            tn2 += t1 * t1;
             x.st(f.ttau, 2 * is, t0);
             x.st(f.ttau, 2 * is + 1, t1);
         }
@@ -437,6 +442,7 @@ __device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH
             ot[k] = otn[k];
         }
     }
+    x.st(f.taun, 0, sqrt(tn2));
     return e2;
 }
 
@@ -468,7 +474,7 @@ __device__ __forceinline__ double harden_voxel_chunked(const Fields& f, TB tab, 
     }
     const double gamma0 = x.ld(f.gacc, 0);
     const double gamma1 = gamma0 + deltgam;
-    double e2 = 0.0;
+    double e2 = 0.0, tn2 = 0.0;  // tn2: |new ttau|^2, columns in order
     int is = 0;
     for (int m0 = 0; m0 < P.nmodes; m0++) {
         const int m = __builtin_amdgcn_readfirstlane(m0);
@@ -517,6 +523,8 @@ __device__ __forceinline__ double harden_voxel_chunked(const Fields& f, TB tab, 
                     const double t1 = cr[2 * k + 1] + dt[s0 + k] * sc;
                     const double o0 = ot[2 * k], o1 = ot[2 * k + 1];
                     e2 += (o0 - t0) * (o0 - t0) + (o1 - t1) * (o1 - t1);
+                    tn2 += t0 * t0;
+                    tn2 += t1 * t1;
                     x.st(f.ttau, 2 * i, t0);
                     x.st(f.ttau, 2 * i + 1, t1);
                 }
@@ -524,6 +532,7 @@ __device__ __forceinline__ double harden_voxel_chunked(const Fields& f, TB tab, 
         }
         is += nm;
     }
+    x.st(f.taun, 0, sqrt(tn2));
     return e2;
 }
 
@@ -578,18 +587,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
     double sgnorm = 0.0, taun = 0.0;
     for (int c = 0; c < 6; c++) sgnorm += sg[c] * sg[c];
     sgnorm = sqrt(sgnorm);
-    {  // |ttau|: 8 loads in flight at a time (clamped indices, the extra terms are exact zeros)
-        const int nt = __builtin_amdgcn_readfirstlane(2 * P.nsyst);
-        for (int k00 = 0; k00 < nt; k00 += 8) {
-            const int k0 = __builtin_amdgcn_readfirstlane(k00);
-            double t[8];
-#pragma unroll
-            for (int q = 0; q < 8; q++) t[q] = x.ld(f.ttau, __builtin_amdgcn_readfirstlane(k0 + q < nt ? k0 + q : nt - 1));
-#pragma unroll
-            for (int q = 0; q < 8; q++) taun += k0 + q < nt ? t[q] * t[q] : 0.0;
-        }
-    }
-    taun = sqrt(taun);
+    taun = x.ld(f.taun, 0);  // |ttau| as the last harden (or k_taun) left it
     const double tol = fmax(pc.error, pc.erroral);
     int itertau = 0;
     double errtau = 2.0 * tol;
@@ -1424,6 +1422,23 @@ __global__ __launch_bounds__(BS) void k_solve_ops(Fields f, double tdot) {
         for (int j = i; j < 6; j++)
             x.st(f.sop, 42 + sp6(i, j),
                  i == j ? x.ld(f.sg66, 7 * i) / tdot : 0.5 * (x.ld(f.sg66, 6 * i + j) + x.ld(f.sg66, 6 * j + i)) / tdot);
+}
+// |ttau| per solid voxel (columns in order, as the solve's own norm) after the trial
+// strengths were written by initialisation, a rollback or the host
+__global__ __launch_bounds__(BS) void k_taun(Fields f, PhaseTables pt) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    if (v >= V || f.gas[v]) return;
+    const int ns = pt.ph[f.phase[v]].nsyst;
+    double t2 = 0.0;
+    for (int k = 0; k < 2 * ns; k++) {
+        const double t = f.ttau[(long)k * V + v];
+        t2 += t * t;
+    }
+    f.taun[v] = sqrt(t2);
+}
+void launch_taun(const Fields& f, const PhaseTables& pt, hipStream_t s) {
+    hipLaunchKernelGGL(k_taun, dim3(nblk(f.V)), dim3(BS), 0, s, f, pt);
 }
 void launch_solve_ops(const Fields& f, double tdot, hipStream_t s) {
     hipLaunchKernelGGL(k_solve_ops, dim3(nblk(f.V)), dim3(BS), 0, s, f, tdot);
