@@ -380,7 +380,7 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     rc |= dalloc(c, &f.Fe, 9 * V); rc |= dalloc(c, &f.Fp, 9 * V); rc |= dalloc(c, &f.ag, 9 * V);
     rc |= dalloc(c, &f.sg66, 36 * V); rc |= dalloc(c, &f.c66m, 36 * V);
     if (c->modal) rc |= dalloc(c, &f.cgm, 36 * V);
-    else rc |= dalloc(c, &f.sop, 63 * V);
+    else rc |= dalloc(c, &f.sop, 42 * V);
     if (c->ngas) rc |= dalloc(c, &f.cgas, 36 * V);
     rc |= dalloc(c, &f.sch, 5 * ns * V); rc |= dalloc(c, &f.schA, 18 * V); rc |= dalloc(c, &f.crss, 2 * ns * V); rc |= dalloc(c, &f.ttau, 2 * ns * V);
     rc |= dalloc(c, &f.taun, V);

@@ -39,7 +39,7 @@ struct Fields {
     double *L, *Lold, *dLref;  // 9
     double *F, *Fi, *J, *Fe, *Fp, *ag;  // 9,9,1,9,9,9
     double *sg66, *c66m, *cgm, *cgas;   // 36 each (cgm only if modal, cgas only if gas voxels)
-    double *sop;          // 63 (without modified AL): packed cm, cm^-1, sym(s66)/tdot of the symmetric solve
+    double *sop;          // 42 (without modified AL): L D L^T of cm^-1 and sym(s66)/tdot of the symmetric solve
     double *sch, *crss, *ttau, *gdot, *gacc;  // 5ns, 2ns, 2ns, ns, 1
     double *taun;         // 1: |ttau| (columns in order), written with ttau by every harden and by k_taun
     double *schA;         // 18: A = Fe ag and A^-1 of update_schmid (sch_s = sym(A b_s x A^-T n_s))

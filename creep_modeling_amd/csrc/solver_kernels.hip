@@ -539,9 +539,9 @@ __device__ __forceinline__ double harden_voxel_chunked(const Fields& f, TB tab, 
 template <bool INTPOW, bool SYMTAU, bool SMALLNS>
 // p0 is wave-uniform (the caller iterates over the phases present in the wave), so the
 // phase tables are read through scalar loads.
-// cml: this thread's column of a block-shared LDS array [21][SBS] holding the packed
-// symmetric cm, read once per evaluation for the residual norm (keeping it out of the
-// registers lets three waves share a SIMD).
+// cml: this thread's column of a block-shared LDS array [21][SBS] holding the L D L^T
+// factors of cm^-1, read once per evaluation for the residual norm |cm R'| (keeping them out
+// of the registers keeps the Newton state within two waves per SIMD).
 // sal: this thread's column of an LDS array [18][SBS] that receives A, A^-1 (schA).
 __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTables& pt, const PassConst& pc,
                                                 const Vox& x, int p0, double* cml, double* sal, double* sgf,
@@ -560,19 +560,35 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
         }
         for (int c = 0; c < 9; c++) L[c] = x.ld(f.L, c);
         to6(L, D);
-        // the operators of this increment (k_solve_ops): packed cm -> LDS, A = cm^-1 + S,
-        // b = cm^-1 lambda + D + S sgt with S = sym(s66)/tdot
+        // the operators of this increment (k_solve_ops): the L D L^T factors of cm^-1 (unit
+        // L below the diagonal, 1/d on it) -> LDS, for the residual norm |cm R'|; cm^-1 =
+        // L D L^T rebuilt here; A = cm^-1 + S, b = cm^-1 lambda + D + S sgt, S = sym(s66)/tdot
+        {
+            double F[21], dk[6];
 #pragma unroll
-        for (int i = 0; i < 21; i++) cml[i * SBS] = x.ld(f.sop, i);
+            for (int i = 0; i < 21; i++) F[i] = x.ld(f.sop, i);
 #pragma unroll
-        for (int i = 0; i < 21; i++) A[i] = x.ld(f.sop, 21 + i);
+            for (int i = 0; i < 21; i++) cml[i * SBS] = F[i];
+#pragma unroll
+            for (int k = 0; k < 6; k++) dk[k] = 1.0 / F[sp6(k, k)];
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+#pragma unroll
+                for (int j = i; j < 6; j++) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int k = 0; k < i; k++) s += F[sp6(i, k)] * dk[k] * F[sp6(j, k)];
+                    s += dk[i] * (j == i ? 1.0 : F[sp6(j, i)]);
+                    A[sp6(i, j)] = s;
+                }
+        }
         spmv6(A, lam, b);
         for (int i = 0; i < 6; i++) b[i] += D[i];
 #pragma unroll
         for (int i = 0; i < 6; i++)
 #pragma unroll
             for (int j = i; j < 6; j++) {
-                const double sij = x.ld(f.sop, 42 + sp6(i, j));
+                const double sij = x.ld(f.sop, 21 + sp6(i, j));
                 A[sp6(i, j)] += sij;
                 b[i] += sij * sgt[j];
                 if (j != i) b[j] += sij * sgt[i];
@@ -612,12 +628,26 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
 #pragma unroll
                 for (int i = 0; i < 6; i++) rp[i] += (i < 5 ? ep[i] : 0.0) - b[i];
                 double rn = 0.0;
+                {  // |cm R'| with cm = (L D L^T)^-1: y = L^-T D^-1 L^-1 R'
+                    double y[6];
 #pragma unroll
-                for (int i = 0; i < 6; i++) {
-                    double s = 0.0;
+                    for (int i = 0; i < 6; i++) {
+                        double s = rp[i];
 #pragma unroll
-                    for (int k = 0; k < 6; k++) s += cml[sp6(i, k) * SBS] * rp[k];
-                    rn += s * s;
+                        for (int k = 0; k < i; k++) s -= cml[sp6(i, k) * SBS] * y[k];
+                        y[i] = s;
+                    }
+#pragma unroll
+                    for (int i = 0; i < 6; i++) y[i] *= cml[sp6(i, i) * SBS];
+#pragma unroll
+                    for (int i = 4; i >= 0; i--) {
+                        double s = y[i];
+#pragma unroll
+                        for (int k = i + 1; k < 6; k++) s -= cml[sp6(k, i) * SBS] * y[k];
+                        y[i] = s;
+                    }
+#pragma unroll
+                    for (int i = 0; i < 6; i++) rn += y[i] * y[i];
                 }
                 resn = sqrt(rn);
                 if (iternr == 1) resoldn = resn;
@@ -635,8 +665,11 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
             for (int i = 0; i < 6; i++)
 #pragma unroll
                 for (int j = i; j < 6; j++) K[sp6(i, j)] = A[sp6(i, j)] + (j < 5 ? dd[sym5(i, j)] : 0.0);
-            ldl6(K);
-            ldl6_solve(K, rp);
+            {
+                double rd[6];
+                ldl6r(K, rd);
+                ldl6_solve_r(K, rd, rp);
+            }
             double d1 = 0.0;
             for (int c = 0; c < 6; c++) {
                 sg[c] -= rp[c];
@@ -668,7 +701,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
         for (int c = 0; c < 6; c++) sev[c] = parked ? x.ld(f.edotp, c) : sgold[c];
         for (int i = 0; i < 6; i++) {
             double s = 0.0;
-            for (int k = 0; k < 6; k++) s += x.ld(f.sop, 42 + sp6(i, k)) * (sev[k] - sgt[k]);
+            for (int k = 0; k < 6; k++) s += x.ld(f.sop, 21 + sp6(i, k)) * (sev[k] - sgt[k]);
             const double e = (i < 5 ? ep[i] : 0.0) + s - D[i];
             dd2 += e * e;
         }
@@ -1392,21 +1425,19 @@ __global__ __launch_bounds__(BS) void k_hardening(Fields f, PhaseTables pt, doub
     f.gacc[v] = g1;
 }
 // Operators of the symmetric Newton solve for one increment (constant while c066mod,
-// sg66 and tdot are): f.sop = [cm (21) | cm^-1 (21) | S = sym(s66)/tdot (21)], packed
-// upper triangles; cm^-1 from the L D L^T of cm.
+// sg66 and tdot are): f.sop = [L D L^T factors of cm^-1 (21: unit L below the diagonal,
+// 1/d on it) | S = sym(s66)/tdot (21, packed)]; cm^-1 itself from the L D L^T of cm.
 __global__ __launch_bounds__(BS) void k_solve_ops(Fields f, double tdot) {
     const long V = f.V;
     const long v = (long)blockIdx.x * BS + threadIdx.x;
     if (v >= V || f.gas[v]) return;
     const Vox x(v, V);
-    double Lf[21];
+    double Lf[21], Ci[21];
 #pragma unroll
     for (int i = 0; i < 6; i++)
 #pragma unroll
         for (int j = i; j < 6; j++)
             Lf[sp6(i, j)] = i == j ? x.ld(f.c66m, 7 * i) : 0.5 * (x.ld(f.c66m, 6 * i + j) + x.ld(f.c66m, 6 * j + i));
-#pragma unroll
-    for (int i = 0; i < 21; i++) x.st(f.sop, i, Lf[i]);
     ldl6(Lf);
 #pragma unroll
     for (int j = 0; j < 6; j++) {  // columns of cm^-1 (upper part kept)
@@ -1414,13 +1445,18 @@ __global__ __launch_bounds__(BS) void k_solve_ops(Fields f, double tdot) {
         e[j] = 1.0;
         ldl6_solve(Lf, e);
 #pragma unroll
-        for (int i = 0; i <= j; i++) x.st(f.sop, 21 + sp6(i, j), e[i]);
+        for (int i = 0; i <= j; i++) Ci[sp6(i, j)] = e[i];
     }
+    ldl6(Ci);  // cm^-1 = L D L^T
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+        for (int j = 0; j <= i; j++) x.st(f.sop, sp6(i, j), i == j ? 1.0 / Ci[sp6(i, i)] : Ci[sp6(i, j)]);
 #pragma unroll
     for (int i = 0; i < 6; i++)
 #pragma unroll
         for (int j = i; j < 6; j++)
-            x.st(f.sop, 42 + sp6(i, j),
+            x.st(f.sop, 21 + sp6(i, j),
                  i == j ? x.ld(f.sg66, 7 * i) / tdot : 0.5 * (x.ld(f.sg66, 6 * i + j) + x.ld(f.sg66, 6 * j + i)) / tdot);
 }
 // |ttau| per solid voxel (columns in order, as the solve's own norm) after the trial

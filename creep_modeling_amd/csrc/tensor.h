@@ -213,6 +213,44 @@ CG_HD void ldl6_solve(const double* P, double* b) {
         b[i] = s;
     }
 }
+// ldl6 that also returns the reciprocal pivots rd[i] = 1/d_i, and the solve that
+// multiplies by them (no division in the solve)
+CG_HD void ldl6r(double* P, double* rd) {
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        double d = P[sp6(j, j)];
+#pragma unroll
+        for (int k = 0; k < j; k++) d -= P[sp6(j, k)] * P[sp6(j, k)] * P[sp6(k, k)];
+        P[sp6(j, j)] = d;
+        const double r = 1.0 / d;
+        rd[j] = r;
+#pragma unroll
+        for (int i = j + 1; i < 6; i++) {
+            double s = P[sp6(i, j)];
+#pragma unroll
+            for (int k = 0; k < j; k++) s -= P[sp6(i, k)] * P[sp6(j, k)] * P[sp6(k, k)];
+            P[sp6(i, j)] = s * r;
+        }
+    }
+}
+CG_HD void ldl6_solve_r(const double* P, const double* rd, double* b) {
+#pragma unroll
+    for (int i = 1; i < 6; i++) {
+        double s = b[i];
+#pragma unroll
+        for (int k = 0; k < i; k++) s -= P[sp6(i, k)] * b[k];
+        b[i] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) b[i] *= rd[i];
+#pragma unroll
+    for (int i = 4; i >= 0; i--) {
+        double s = b[i];
+#pragma unroll
+        for (int k = i + 1; k < 6; k++) s -= P[sp6(k, i)] * b[k];
+        b[i] = s;
+    }
+}
 // LU factorisation with partial pivoting of a 6x6 (in place; unit lower L)
 CG_HD bool lu6(double* A, int* piv) {
     bool ok = true;
