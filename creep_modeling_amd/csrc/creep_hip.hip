@@ -923,6 +923,14 @@ int cg_set_field(cg_ctx* c, const char* name, const double* in, size_t n) {
     if (!field_desc(c, name, d) || !d.p) return fail(c, -1, "unknown or absent field %s", name);
     const long V = c->V;
     if (n < (size_t)d.ncomp * V) return fail(c, -1, "input buffer too small");
+    if (c->sym_tau && (d.p == c->f.crss || d.p == c->f.ttau)) {
+        // the kernels of a context whose forward and backward strengths are equal read one
+        // column of each pair
+        for (long v = 0; v < V; v++)
+            for (int k = 0; k + 1 < d.ncomp; k += 2)
+                if (in[(size_t)d.ncomp * v + k] != in[(size_t)d.ncomp * v + k + 1])
+                    return fail(c, -1, "%s: forward and backward values differ in a context built with equal ones", name);
+    }
     std::vector<double> h((size_t)d.stored * V);
     for (long v = 0; v < V; v++) {
         if (d.sym) {

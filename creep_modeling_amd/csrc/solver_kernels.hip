@@ -376,7 +376,7 @@ __device__ __forceinline__ double harden_voxel(const Fields& f, TB tab, const PH
 // and old strengths are in flight together, and each later chunk of HC systems is read
 // while the previous one computes.  The sums run over the same terms in the same order
 // as harden_voxel (the added terms are exact zeros).
-template <class PH, class TB>
+template <bool SYMTAU, class PH, class TB>
 __device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH& P, const Vox& x, double tdot) {
     const int ns = __builtin_amdgcn_readfirstlane(P.nsyst);
     const int oh = __builtin_amdgcn_readfirstlane(P.off_hard);
@@ -388,9 +388,10 @@ __device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH
         for (int k = 0; k < HC; k++) {
             const int i = clampi(c0 + k);
             c2[2 * k] = x.ld(f.crss, 2 * i);
-            c2[2 * k + 1] = x.ld(f.crss, 2 * i + 1);
             o2[2 * k] = x.ld(f.ttau, 2 * i);
-            o2[2 * k + 1] = x.ld(f.ttau, 2 * i + 1);
+            // with equal forward and backward strengths the two columns are bitwise equal
+            c2[2 * k + 1] = SYMTAU ? c2[2 * k] : x.ld(f.crss, 2 * i + 1);
+            o2[2 * k + 1] = SYMTAU ? o2[2 * k] : x.ld(f.ttau, 2 * i + 1);
         }
     };
     load_chunk(0, cr, ot);
@@ -545,18 +546,19 @@ template <bool INTPOW, bool SYMTAU, bool SMALLNS>
 // sal: this thread's column of an LDS array [18][SBS] that receives A, A^-1 (schA).
 __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTables& pt, const PassConst& pc,
                                                 const Vox& x, int p0, double* cml, double* sal, double* sgf,
-                                                double* acc) {
+                                                double* acc, double w) {
     const cphase& P = ((cphase*)pt.ph)[p0];
     cdbl* tab = (cdbl*)pt.data;
     const double tdot = pc.tdot;
 #pragma unroll
     for (int k = 0; k < 18; k++) sal[k * SBS] = x.ld(f.schA, k);
-    double A[21], b[6];
+    double A[21], b[6], sg[6];
     {
         double lam[6], sgt[6], D[6], L[9];
         for (int c = 0; c < 6; c++) {
             lam[c] = x.ld(f.sg, c);
             sgt[c] = x.ld(f.sgt, c);
+            sg[c] = lam[c];  // the solve starts from lambda
         }
         for (int c = 0; c < 9; c++) L[c] = x.ld(f.L, c);
         to6(L, D);
@@ -597,9 +599,8 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
     // The stress of the last evaluation is sgold after the line search, unless the
     // search ended on a cut (icut = 10); that rare stress is parked in the edotp
     // columns (written with the final edotp below) and flagged.
-    double sg[6], sgold[6] = {0, 0, 0, 0, 0, 0}, ep[5], dd[15], rp[6];
+    double sgold[6] = {0, 0, 0, 0, 0, 0}, ep[5], dd[15], rp[6];
     bool parked = false;
-    for (int c = 0; c < 6; c++) sg[c] = x.ld(f.sg, c);
     double sgnorm = 0.0, taun = 0.0;
     for (int c = 0; c < 6; c++) sgnorm += sg[c] * sg[c];
     sgnorm = sqrt(sgnorm);
@@ -680,33 +681,46 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
         // harden with the slip rates of the last evaluation
         if (pc.iuphard == 1) {
             double g1;
-            const double e2 = SMALLNS ? harden_small(f, tab, P, x, tdot) : harden_voxel_chunked(f, tab, P, x, tdot);
+            const double e2 = SMALLNS ? harden_small<SYMTAU>(f, tab, P, x, tdot) : harden_voxel_chunked(f, tab, P, x, tdot);
             errtau = sqrt(e2) / taun;
         } else {
             errtau = 0.0;
         }
     }
-    // errors: dsgnorm = |sg - lambda|^2, ddgnorm = |edot(last evaluation) - D|^2
+    // errors: dsgnorm = |sg - lambda|^2, ddgnorm = |e|^2 with e = edot(last evaluation) - D
+    // = ep + S (sev - sgt) - D.  R'(s) = A s - b + ep = cm^-1 (s - lambda) + e(s), so e is
+    // R'(sev) - cm^-1 (sev - lambda), formed from A, b in registers and the L D L^T factors
+    // of cm^-1 in LDS: of the inputs only lambda is read again.
     double dsgn = 0.0, dd2 = 0.0;
     {
-        double lam[6], sgt[6], L[9], D[6];
-        for (int c = 0; c < 6; c++) {
-            lam[c] = x.ld(f.sg, c);
-            sgt[c] = x.ld(f.sgt, c);
-        }
-        for (int c = 0; c < 9; c++) L[c] = x.ld(f.L, c);
-        to6(L, D);
+        double lam[6], sev[6], e[6], q[6];
+        for (int c = 0; c < 6; c++) lam[c] = x.ld(f.sg, c);
         for (int c = 0; c < 6; c++) dsgn += (sg[c] - lam[c]) * (sg[c] - lam[c]);
-        double sev[6];
         for (int c = 0; c < 6; c++) sev[c] = parked ? x.ld(f.edotp, c) : sgold[c];
+        spmv6(A, sev, e);
+#pragma unroll
         for (int i = 0; i < 6; i++) {
-            double s = 0.0;
-            for (int k = 0; k < 6; k++) s += x.ld(f.sop, 21 + sp6(i, k)) * (sev[k] - sgt[k]);
-            const double e = (i < 5 ? ep[i] : 0.0) + s - D[i];
-            dd2 += e * e;
+            e[i] += (i < 5 ? ep[i] : 0.0) - b[i];
+            q[i] = sev[i] - lam[i];
         }
+        // q <- cm^-1 q = L (D (L^T q))
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            double s = q[i];
+#pragma unroll
+            for (int k = i + 1; k < 6; k++) s += cml[sp6(k, i) * SBS] * q[k];
+            q[i] = s / cml[sp6(i, i) * SBS];
+        }
+#pragma unroll
+        for (int i = 5; i >= 0; i--) {
+            double s = q[i];
+#pragma unroll
+            for (int k = 0; k < i; k++) s += cml[sp6(i, k) * SBS] * q[k];
+            q[i] = s;
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) dd2 += (e[i] - q[i]) * (e[i] - q[i]);
     }
-    const double w = x.ld(f.w, 0);
     acc[R_ERRS] = dsgn * w;
     acc[R_ERRE] = dd2 * w;
     for (int c = 0; c < 6; c++) sgf[c] = sg[c];
@@ -802,7 +816,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 :
                 const int p0 = __builtin_amdgcn_readfirstlane(ph);
                 if (ph == p0) {
                     solve_solid_sym<INTPOW, SYMTAU, SMALLNS>(f, pt, pc, x, p0, cm_lds + threadIdx.x, sa_lds + threadIdx.x, sgf,
-                                                    acc);
+                                                    acc, w);
                     break;
                 }
             }
