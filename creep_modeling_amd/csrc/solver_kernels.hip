@@ -546,7 +546,7 @@ template <bool INTPOW, bool SYMTAU, bool SMALLNS>
 // sal: this thread's column of an LDS array [18][SBS] that receives A, A^-1 (schA).
 __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTables& pt, const PassConst& pc,
                                                 const Vox& x, int p0, double* cml, double* sal, double* sgf,
-                                                double* acc, double w) {
+                                                double* acc, double w, const double* Lmix) {
     const cphase& P = ((cphase*)pt.ph)[p0];
     cdbl* tab = (cdbl*)pt.data;
     const double tdot = pc.tdot;
@@ -554,14 +554,13 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
     for (int k = 0; k < 18; k++) sal[k * SBS] = x.ld(f.schA, k);
     double A[21], b[6], sg[6];
     {
-        double lam[6], sgt[6], D[6], L[9];
+        double lam[6], sgt[6], D[6];
         for (int c = 0; c < 6; c++) {
             lam[c] = x.ld(f.sg, c);
             sgt[c] = x.ld(f.sgt, c);
             sg[c] = lam[c];  // the solve starts from lambda
         }
-        for (int c = 0; c < 9; c++) L[c] = x.ld(f.L, c);
-        to6(L, D);
+        to6(Lmix, D);
         // the operators of this increment (k_solve_ops): the L D L^T factors of cm^-1 (unit
         // L below the diagonal, 1/d on it) -> LDS, for the residual norm |cm R'|; cm^-1 =
         // L D L^T rebuilt here; A = cm^-1 + S, b = cm^-1 lambda + D + S sgt, S = sym(s66)/tdot
@@ -745,7 +744,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 :
     double* const sa_lds = pool + 21 * SBS;
     // ---- phase A: correction_to_current + dL_iter/mixing; its 31 sums are reduced
     // before the constitutive solve so their accumulators are not live across it
-    double w = 0.0;
+    double w = 0.0, Lmix[9];  // Lmix: this voxel's velocity gradient after the mixing (phase A)
     bool gas = false;
     {
         double accA[R_NSOLVE - R_NPHASEB];
@@ -792,7 +791,10 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 :
             } else {
                 for (int c = 0; c < 9; c++) L[c] = pc.mix * L[c];
             }
-            for (int c = 0; c < 9; c++) x.st(f.L, c, L[c]);
+            for (int c = 0; c < 9; c++) {
+                x.st(f.L, c, L[c]);
+                Lmix[c] = L[c];
+            }
         }
         if (MODAL) {
             block_reduce_store_cm<R_NSOLVE - R_NPHASEB, SBS>(accA, f.part, R_NPHASEB);
@@ -816,7 +818,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 :
                 const int p0 = __builtin_amdgcn_readfirstlane(ph);
                 if (ph == p0) {
                     solve_solid_sym<INTPOW, SYMTAU, SMALLNS>(f, pt, pc, x, p0, cm_lds + threadIdx.x, sa_lds + threadIdx.x, sgf,
-                                                    acc, w);
+                                                    acc, w, Lmix);
                     break;
                 }
             }
