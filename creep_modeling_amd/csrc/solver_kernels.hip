@@ -550,8 +550,17 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
     const cphase& P = ((cphase*)pt.ph)[p0];
     cdbl* tab = (cdbl*)pt.data;
     const double tdot = pc.tdot;
+    {  // A = Fe ag from memory; A^-1 by the same inv3 as update_schmid's
+        double Am[9], Ai[9];
 #pragma unroll
-    for (int k = 0; k < 18; k++) sal[k * SBS] = x.ld(f.schA, k);
+        for (int k = 0; k < 9; k++) Am[k] = x.ld(f.schA, k);
+        inv3(Am, Ai);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            sal[k * SBS] = Am[k];
+            sal[(9 + k) * SBS] = Ai[k];
+        }
+    }
     double A[21], b[6], sg[6];
     {
         double lam[6], sgt[6], D[6];
