@@ -76,7 +76,7 @@ enum {
 
 // ---- solver_kernels.hip ----
 void launch_reduce_final(const double* part, int nblocks, int nvals, double* out, hipStream_t s);
-void launch_reduce_final_cm(const double* part, int nblocks, int nvals, double* out, hipStream_t s);
+void launch_reduce_final_cm(double* part, int nblocks, int nvals, double* out, hipStream_t s);
 void launch_sigma_sum(const Fields& f, hipStream_t s, int* nblocks_out);
 void launch_correct_solve_full(const Fields& f, const PhaseTables& pt, const PassConst& pc,
                                const double* dvelgradavg, const double* finv_inv, const double* fluct_dev,

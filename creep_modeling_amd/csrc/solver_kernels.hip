@@ -96,8 +96,43 @@ __global__ __launch_bounds__(BS) void k_reduce_final_cm(const double* __restrict
         out[i] = t;
     }
 }
-void launch_reduce_final_cm(const double* part, int nblocks, int nvals, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_final_cm, dim3(nvals), dim3(BS), 0, s, part, nblocks, out);
+// First level of the column-major reduction: block (c, i) sums the partials
+// [c*CH, (c+1)*CH) of value i into tmp[i * gridDim.x + c] (fixed order).
+constexpr int RCH = 1024;
+__global__ __launch_bounds__(BS) void k_reduce_cols(const double* __restrict__ part, int nblocks,
+                                                    double* __restrict__ tmp) {
+    const int c = blockIdx.x, i = blockIdx.y;
+    const long b0 = (long)c * RCH;
+    const long n = nblocks - b0 < RCH ? nblocks - b0 : RCH;
+    const double* col = part + (size_t)i * nblocks + b0;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < RCH / BS; k++) {
+        const long j = threadIdx.x + (long)k * BS;
+        if (j < n) s += col[j];
+    }
+    __shared__ double sm[BS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (lane == 0) sm[wid] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = sm[0];
+        for (int w = 1; w < BS / 64; w++) t += sm[w];
+        tmp[(size_t)i * gridDim.x + c] = t;
+    }
+}
+// part holds nvals columns of nblocks partials; the space after them (nvals * ceil(nblocks /
+// 1024) doubles) is scratch for the two-level reduction of long columns
+void launch_reduce_final_cm(double* part, int nblocks, int nvals, double* out, hipStream_t s) {
+    const int nch = (nblocks + RCH - 1) / RCH;
+    if (nch <= 1) {
+        hipLaunchKernelGGL(k_reduce_final_cm, dim3(nvals), dim3(BS), 0, s, part, nblocks, out);
+        return;
+    }
+    double* tmp = part + (size_t)nvals * nblocks;
+    hipLaunchKernelGGL(k_reduce_cols, dim3(nch, nvals), dim3(BS), 0, s, part, nblocks, tmp);
+    hipLaunchKernelGGL(k_reduce_final_cm, dim3(nvals), dim3(BS), 0, s, tmp, nch, out);
 }
 void launch_reduce_final(const double* part, int nblocks, int nvals, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_final, dim3(nvals), dim3(BS), 0, s, part, nblocks, nvals, out);
