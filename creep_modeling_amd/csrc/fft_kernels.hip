@@ -748,7 +748,7 @@ namespace cg {
 
 constexpr int XBS = 128;  // threads per block: x passes (a few rows of 9 components)
 constexpr int YBS = 256;  // threads per block: y passes
-constexpr int ZBS = 144;  // threads per block: z pass (whole kx columns of 9 components)
+
 
 template <int N> struct RegDims {
     static constexpr int T = Shape<N>::T, E = N / T, U = (E + T - 1) / T;
@@ -756,8 +756,11 @@ template <int N> struct RegDims {
     static constexpr int NS = YBS / T;                 // signals per y block
     static constexpr int NSX = XBS / T;                // signals per x block
     static constexpr int RW = NSX / 5;                 // rows per x block (5 signals a row)
-    static constexpr int NSZ = ZBS / T;                // signals per z block
-    static constexpr int KT = NSZ / 9;                 // kx columns per z block
+    // z pass: KT whole kx columns of 9 components per block, 24 signals' worth of threads
+    // (216) for T = 4 and 8, one column (144 threads) for T = 16
+    static constexpr int KT = T >= 16 ? 1 : 24 / T;    // kx columns per z block
+    static constexpr int NSZ = 9 * KT;                 // signals per z block
+    static constexpr int ZB = NSZ * T;                 // threads per z block
     static_assert(RW >= 1 && KT >= 1, "block shapes");
 };
 
@@ -783,21 +786,23 @@ __global__ __launch_bounds__(XBS) void k_x_fwd_reg(Fields f, const typename Cx<R
     const int h1 = f.h1;
     double sa[9];
     for (int c = 0; c < 9; c++) sa[c] = sgsum[c];
-    for (int e = threadIdx.x; e < RW * N; e += XBS) {
+    // each thread's voxels: every load is issued before the products (rows past the grid
+    // read the last row and are zeroed)
+    constexpr int NE = (RW * N + XBS - 1) / XBS;
+#pragma unroll
+    for (int q = 0; q < NE; q++) {
+        const int e = threadIdx.x + q * XBS;
+        if (RW * N % XBS != 0 && e >= RW * N) break;
         const int r = e / N, i = e - r * N;
         const long row = row0 + r;
-        double pk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        double J = 0.0;
-        if (row < nrows) {
-            const long v = row * N + i;
-            double s6[6], s[9], Fi[9];
-            for (int c = 0; c < 6; c++) s6[c] = f.sg[c * V + v];
-            for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * V + v];
-            from6(s6, s);
-            J = f.J[v];
-            for (int c = 0; c < 9; c++) s[c] -= sa[c];
-            mmt3(s, Fi, pk);  // (sigma - <sigma>) F^-T
-        }
+        const long v = (row < nrows ? row : nrows - 1) * N + i;
+        double s6[6], s[9], Fi[9], pk[9];
+        for (int c = 0; c < 6; c++) s6[c] = f.sg[c * V + v];
+        for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * V + v];
+        const double J = row < nrows ? f.J[v] : 0.0;
+        from6(s6, s);
+        for (int c = 0; c < 9; c++) s[c] -= sa[c];
+        mmt3(s, Fi, pk);  // (sigma - <sigma>) F^-T
         for (int c = 0; c < 9; c++) stage[(r * 9 + c) * N + i] = (R)(pk[c] * J * inv_scale);
     }
     __syncthreads();
@@ -871,7 +876,7 @@ __global__ __launch_bounds__(YBS) void k_y_reg(Fields f, const typename Cx<R>::T
 
 // z pass with the Green operator: KT whole kx columns (9 components each) of one y row
 template <int N, typename R, bool GENERAL>
-__global__ __launch_bounds__(ZBS) void k_z_green_reg(Fields f, const typename Cx<R>::T* __restrict__ tw, GreenDev g,
+__global__ __launch_bounds__(RegDims<N>::ZB) void k_z_green_reg(Fields f, const typename Cx<R>::T* __restrict__ tw, GreenDev g,
                                                      int ntile) {
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
@@ -898,7 +903,7 @@ __global__ __launch_bounds__(ZBS) void k_z_green_reg(Fields f, const typename Cx
             for (int jj = 0; jj < T; jj++) ex[(q + E * jj) * NS + s] = a[u * T + jj];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < KTe * N; e += ZBS) {
+    for (int e = threadIdx.x; e < KTe * N; e += D::ZB) {
         const int kz = e / KTe, kxl = e - kz * KTe;
         green_apply<R, GENERAL>(g, kx0 + kxl, j, kz, f.n1, f.n2, N, ex + kz * NS + kxl * 9, 1);
     }
@@ -976,19 +981,23 @@ __global__ __launch_bounds__(XBS) void k_x_inv_reg(Fields f, const typename Cx<R
         }
     __syncthreads();
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int e = threadIdx.x; e < RW * N; e += XBS) {
+    constexpr int NE = (RW * N + XBS - 1) / XBS;
+#pragma unroll
+    for (int q = 0; q < NE; q++) {
+        const int e = threadIdx.x + q * XBS;
+        if (RW * N % XBS != 0 && e >= RW * N) break;
         const int rr = e / N, i = e - rr * N;
         const long row = row0 + rr;
-        if (row >= nrows) continue;
-        const long v = row * N + i;
+        const bool in = row < nrows;
+        const long v = (in ? row : nrows - 1) * N + i;  // rows past the grid: loads only
         double d[9], Fi[9], q9[9];
+        for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * V + v];
+        const double wv = in ? f.w[v] : 0.0;
         for (int c = 0; c < 9; c++) {
             d[c] = (double)out[(rr * 9 + c) * N + i] * out_scale;
-            f.dLref[c * V + v] = d[c];
-            Fi[c] = f.Fi[c * V + v];
+            if (in) f.dLref[c * V + v] = d[c];
         }
         mm3(d, Fi, q9);
-        const double wv = f.w[v];
         for (int c = 0; c < 9; c++) acc[c] += q9[c] * wv;
     }
     block_reduce_store<9, XBS>(acc, f.part);
@@ -1017,8 +1026,8 @@ template <int N, bool INV, typename R> static void launch_y_reg(const Fields& f,
 template <int N, typename R> static void launch_z_reg(const Fields& f, const typename Cx<R>::T* tw, const GreenDev& g,
                                                       hipStream_t s) {
     const int ntile = (f.h1 + RegDims<N>::KT - 1) / RegDims<N>::KT;
-    if (g.igamma == 0) hipLaunchKernelGGL((k_z_green_reg<N, R, false>), dim3(f.n2 * ntile), dim3(ZBS), 0, s, f, tw, g, ntile);
-    else hipLaunchKernelGGL((k_z_green_reg<N, R, true>), dim3(f.n2 * ntile), dim3(ZBS), 0, s, f, tw, g, ntile);
+    if (g.igamma == 0) hipLaunchKernelGGL((k_z_green_reg<N, R, false>), dim3(f.n2 * ntile), dim3(RegDims<N>::ZB), 0, s, f, tw, g, ntile);
+    else hipLaunchKernelGGL((k_z_green_reg<N, R, true>), dim3(f.n2 * ntile), dim3(RegDims<N>::ZB), 0, s, f, tw, g, ntile);
 }
 
 #define CG_SWITCH_N(n, CALL)                                             \
