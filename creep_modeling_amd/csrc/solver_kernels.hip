@@ -243,7 +243,7 @@ __device__ __forceinline__ void slip_eval(const Fields& f, const double* __restr
 // components; the next system's strengths are loaded while the current one is computed.
 // SYMTAU: the forward and backward trial strengths are equal (tau0f = tau0b for every
 // system: init and harden then keep the two columns bitwise equal), so one is read.
-// ACC: accumulate edotp and its derivative; STORE: store gdot (the last evaluation's
+// ACC: accumulate edotp and its derivative; STORE, store: store gdot (the last evaluation's
 // rates feed harden and the per-increment updates).
 // Trial strengths are read for a group of TauGroup systems at a time, one group ahead:
 // the loads of group g+1 are in flight while group g computes.  Group indices are
@@ -267,7 +267,8 @@ __device__ __forceinline__ void load_tau_group(const Fields& f, const Vox& x, in
 }
 template <bool INTPOW, bool SYMTAU, bool ACC, bool STORE, class PH, class TB>
 __device__ __forceinline__ void slip_eval_ab(const Fields& f, TB tab, const PH& P, const Vox& x, const double* sal,
-                                             const double* sg, double* ep, double* dd, double* tpn, double* tnn) {
+                                             const double* sg, double* ep, double* dd, double* tpn, double* tnn,
+                                             bool store = true) {
     double A[9], Ai[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
@@ -309,7 +310,7 @@ __device__ __forceinline__ void slip_eval_ab(const Fields& f, TB tab, const PH& 
             const double n = tab[rb + 6];
             const double dum = tab[rb + 7] * powm1<INTPOW>(fabs(rss), n);
             const double r2 = rss * dum;
-            if (STORE) x.st(f.gdot, is, r2);
+            if (STORE && store) x.st(f.gdot, is, r2);
             if (ACC) {
                 const double r1 = n * dum * ti;
                 ep[0] += s0 * r2; ep[1] += s1 * r2; ep[2] += s2 * r2; ep[3] += s3 * r2; ep[4] += s4 * r2;
@@ -667,7 +668,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
             while (resn > resoldn && icut < 10) {
                 nev++;
                 parked = false;
-                slip_eval_ab<INTPOW, SYMTAU, true, true>(f, tab, P, x, sal, sg, ep, dd, tpn, tnn);
+                slip_eval_ab<INTPOW, SYMTAU, true, true>(f, tab, P, x, sal, sg, ep, dd, tpn, tnn, iternr > 1);
                 spmv6(A, sg, rp);
 #pragma unroll
                 for (int i = 0; i < 6; i++) rp[i] += (i < 5 ? ep[i] : 0.0) - b[i];
@@ -721,6 +722,10 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
             }
             erral = sqrt(d1) / sgnorm;
         }
+        // The first Newton step's evaluation does not store its rates: a loop ends there
+        // only when that step is already below the tolerance (its one evaluation, at
+        // sgold, makes no line-search cut), and then the rates are evaluated again.
+        if (iternr == 1) slip_eval_ab<INTPOW, SYMTAU, false, true>(f, tab, P, x, sal, sgold, ep, dd, tpn, tnn);
         // harden with the slip rates of the last evaluation
         if (pc.iuphard == 1) {
             double g1;
