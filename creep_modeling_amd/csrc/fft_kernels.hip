@@ -783,7 +783,6 @@ __global__ __launch_bounds__(XBS) void k_x_fwd_reg(Fields f, const typename Cx<R
     const long V = f.V;
     const long nrows = (long)f.n2 * f.n3;
     const long row0 = (long)blockIdx.x * RW;
-    const int h1 = f.h1;
     double sa[9];
     for (int c = 0; c < 9; c++) sa[c] = sgsum[c];
     // each thread's voxels: every load is issued before the products (rows past the grid
@@ -832,8 +831,11 @@ __global__ __launch_bounds__(XBS) void k_x_fwd_reg(Fields f, const typename Cx<R
     }
     __syncthreads();
     T2* spec = spec_of<R>(f);
-    const int w = h1 * 9;
-    for (int e = threadIdx.x; e < RW * w; e += XBS) {
+    constexpr int w = (N / 2 + 1) * 9;  // h1 = n1/2 + 1 complex values of 9 components a row
+#pragma unroll
+    for (int q = 0; q < (RW * w + XBS - 1) / XBS; q++) {
+        const int e = threadIdx.x + q * XBS;
+        if (RW * w % XBS != 0 && e >= RW * w) break;
         const int rr = e / w, rem = e - rr * w;
         const int kx = rem / 9, c = rem - kx * 9;
         const long row = row0 + rr;
@@ -938,11 +940,36 @@ __global__ __launch_bounds__(XBS) void k_x_inv_reg(Fields f, const typename Cx<R
     const long V = f.V;
     const long nrows = (long)f.n2 * f.n3;
     const long row0 = (long)blockIdx.x * RW;
-    const int w = H * 9;
+    constexpr int w = H * 9;
     const T2* spec = spec_of<R>(f);
-    for (int e = threadIdx.x; e < RW * w; e += XBS) {
-        const long row = row0 + e / w;
-        ex[e] = row < nrows ? spec[row0 * w + e] : T2{0, 0};
+    // the block's spectrum rows: every load is issued before the first is written to LDS
+    // (rows past the grid read element 0 and are zeroed)
+    constexpr int NL = (RW * w + XBS - 1) / XBS;
+    T2 sv[NL];
+#pragma unroll
+    for (int q = 0; q < NL; q++) {
+        const int e = threadIdx.x + q * XBS;
+        const int ec = RW * w % XBS == 0 || e < RW * w ? e : RW * w - 1;
+        sv[q] = spec[row0 + ec / w < nrows ? row0 * w + ec : 0];
+    }
+    // then F^-1 and w of this thread's output voxels: their loads are in flight through the
+    // transform (rows past the grid read the last row)
+    constexpr int NE = (RW * N + XBS - 1) / XBS;
+    double Fp[NE][9], wp[NE];
+#pragma unroll
+    for (int q = 0; q < NE; q++) {
+        const int e = threadIdx.x + q * XBS;
+        if (RW * N % XBS != 0 && e >= RW * N) break;
+        const int rr = e / N, i = e - rr * N;
+        const long row = row0 + rr;
+        const long v = (row < nrows ? row : nrows - 1) * N + i;
+        for (int c = 0; c < 9; c++) Fp[q][c] = f.Fi[c * V + v];
+        wp[q] = row < nrows ? f.w[v] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < NL; q++) {
+        const int e = threadIdx.x + q * XBS;
+        if (RW * w % XBS == 0 || e < RW * w) ex[e] = row0 + e / w < nrows ? sv[q] : T2{0, 0};
     }
     __syncthreads();
     const int s = threadIdx.x % NS, t = threadIdx.x / NS;
@@ -981,7 +1008,6 @@ __global__ __launch_bounds__(XBS) void k_x_inv_reg(Fields f, const typename Cx<R
         }
     __syncthreads();
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    constexpr int NE = (RW * N + XBS - 1) / XBS;
 #pragma unroll
     for (int q = 0; q < NE; q++) {
         const int e = threadIdx.x + q * XBS;
@@ -989,16 +1015,14 @@ __global__ __launch_bounds__(XBS) void k_x_inv_reg(Fields f, const typename Cx<R
         const int rr = e / N, i = e - rr * N;
         const long row = row0 + rr;
         const bool in = row < nrows;
-        const long v = (in ? row : nrows - 1) * N + i;  // rows past the grid: loads only
-        double d[9], Fi[9], q9[9];
-        for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * V + v];
-        const double wv = in ? f.w[v] : 0.0;
+        const long v = (in ? row : nrows - 1) * N + i;
+        double d[9], q9[9];
         for (int c = 0; c < 9; c++) {
             d[c] = (double)out[(rr * 9 + c) * N + i] * out_scale;
             if (in) f.dLref[c * V + v] = d[c];
         }
-        mm3(d, Fi, q9);
-        for (int c = 0; c < 9; c++) acc[c] += q9[c] * wv;
+        mm3(d, Fp[q], q9);
+        for (int c = 0; c < 9; c++) acc[c] += q9[c] * wp[q];
     }
     block_reduce_store<9, XBS>(acc, f.part);
 }
