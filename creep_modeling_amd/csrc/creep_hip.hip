@@ -551,7 +551,7 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     PhaseTables pt{c->d_ph, c->d_tab};
     tstart(c, 1);
     if (!c->taun_valid) {
-        launch_taun(f, pt, c->s);
+        launch_taun(f, pt, c->sym_tau, c->s);
         c->taun_valid = 1;
     }
     if (!c->modal && (!c->sop_valid || c->sop_tdot != p->tdot)) {
@@ -900,9 +900,18 @@ int cg_get_field(cg_ctx* c, const char* name, double* out, size_t n) {
     if (!field_desc(c, name, d) || !d.p) return fail(c, -1, "unknown or absent field %s", name);
     const long V = c->V;
     if (n < (size_t)d.ncomp * V) return fail(c, -1, "output buffer too small");
+    if (d.p == c->f.Lold && c->lold_same) {  // velgradold == velgrad since the last pass
+        launch_copy(c->f.Lold, c->f.L, 9 * V, c->s);
+        c->lold_same = 0;
+    }
     std::vector<double> h((size_t)d.stored * V);
     HIPCHK(c, hipStreamSynchronize(c->s));
     HIPCHK(c, hipMemcpy(h.data(), d.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    // equal forward and backward strengths: the hardens inside the solve write only the
+    // forward column of trialtau (the increment's last harden writes both)
+    if (c->sym_tau && d.p == c->f.ttau)
+        for (int k = 0; k + 1 < d.stored; k += 2)
+            std::copy(h.begin() + (size_t)k * V, h.begin() + (size_t)(k + 1) * V, h.begin() + (size_t)(k + 1) * V);
     for (long v = 0; v < V; v++) {
         if (d.sym) {
             double s6[6];
@@ -923,6 +932,10 @@ int cg_set_field(cg_ctx* c, const char* name, const double* in, size_t n) {
     if (!field_desc(c, name, d) || !d.p) return fail(c, -1, "unknown or absent field %s", name);
     const long V = c->V;
     if (n < (size_t)d.ncomp * V) return fail(c, -1, "input buffer too small");
+    if (c->lold_same) {  // keep velgradold (== velgrad since the last pass) before any field changes
+        launch_copy(c->f.Lold, c->f.L, 9 * V, c->s);
+        c->lold_same = 0;
+    }
     if (c->sym_tau && (d.p == c->f.crss || d.p == c->f.ttau)) {
         // the kernels of a context whose forward and backward strengths are equal read one
         // column of each pair
@@ -944,7 +957,6 @@ int cg_set_field(cg_ctx* c, const char* name, const double* in, size_t n) {
     HIPCHK(c, hipStreamSynchronize(c->s));
     HIPCHK(c, hipMemcpy(d.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     c->sgsum_valid = 0;
-    c->lold_same = 0;
     return 0;
 }
 

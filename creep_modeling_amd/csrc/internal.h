@@ -94,7 +94,7 @@ void launch_tensor_config_a(const Fields& f, hipStream_t s, int* nb);
 void launch_tensor_config_b(const Fields& f, const double* dud, hipStream_t s);
 void launch_hardening(const Fields& f, const PhaseTables& pt, double tdot, hipStream_t s);
 void launch_solve_ops(const Fields& f, double tdot, hipStream_t s);
-void launch_taun(const Fields& f, const PhaseTables& pt, hipStream_t s);
+void launch_taun(const Fields& f, const PhaseTables& pt, int sym, hipStream_t s);
 void launch_averages(const Fields& f, const PhaseTables& pt, hipStream_t s, int* nb);
 void launch_diagnostics(const Fields& f, double lfloor, int scan_cond, hipStream_t s, int* nb);
 void launch_calc_c0(const Fields& f, const PhaseTables& pt, double tdot, double tdotref, hipStream_t s, int* nb,

@@ -242,7 +242,8 @@ __device__ __forceinline__ void slip_eval(const Fields& f, const double* __restr
 // an evaluation reads 2 n_s trial strengths from memory instead of 5 n_s Schmid
 // components; the next system's strengths are loaded while the current one is computed.
 // SYMTAU: the forward and backward trial strengths are equal (tau0f = tau0b for every
-// system: init and harden then keep the two columns bitwise equal), so one is read.
+// system: init and the increment's final harden write the two columns bitwise equal), so
+// the forward one is read, and the hardens inside the solve write only that one.
 // ACC: accumulate edotp and its derivative; STORE, store: store gdot (the last evaluation's
 // rates feed harden and the per-increment updates).
 // Trial strengths are read for a group of TauGroup systems at a time, one group ahead:
@@ -425,7 +426,7 @@ __device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH
             const int i = clampi(c0 + k);
             c2[2 * k] = x.ld(f.crss, 2 * i);
             o2[2 * k] = x.ld(f.ttau, 2 * i);
-            // with equal forward and backward strengths the two columns are bitwise equal
+            // with equal forward and backward strengths the forward column stands for both
             c2[2 * k + 1] = SYMTAU ? c2[2 * k] : x.ld(f.crss, 2 * i + 1);
             o2[2 * k + 1] = SYMTAU ? o2[2 * k] : x.ld(f.ttau, 2 * i + 1);
         }
@@ -471,7 +472,7 @@ __device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH
             tn2 += t0 * t0;
             tn2 += t1 * t1;
             x.st(f.ttau, 2 * is, t0);
-            x.st(f.ttau, 2 * is + 1, t1);
+            if (!SYMTAU) x.st(f.ttau, 2 * is + 1, t1);
         }
 #pragma unroll
         for (int k = 0; k < 2 * HC; k++) {
@@ -487,8 +488,8 @@ __device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH
 // of a mode at a time, the rates read 4 at a time (in flight together) and each |gdot_js|
 // once per chunk; the rows of the interaction matrix are read 4 wide (the table is padded;
 // rates past the last system weigh zero).  dtau sums the same terms in the same order as
-// harden_voxel.
-template <class PH, class TB>
+// harden_voxel.  SYMTAU: as harden_small.
+template <bool SYMTAU, class PH, class TB>
 __device__ __forceinline__ double harden_voxel_chunked(const Fields& f, TB tab, const PH& P, const Vox& x,
                                                        double tdot) {
     const int ns = __builtin_amdgcn_readfirstlane(P.nsyst);
@@ -548,9 +549,9 @@ __device__ __forceinline__ double harden_voxel_chunked(const Fields& f, TB tab, 
                 for (int k = 0; k < SB; k++) {
                     const int i = __builtin_amdgcn_readfirstlane(i0 + (s0 + k < nk ? s0 + k : nk - 1));
                     cr[2 * k] = x.ld(f.crss, 2 * i);
-                    cr[2 * k + 1] = x.ld(f.crss, 2 * i + 1);
+                    cr[2 * k + 1] = SYMTAU ? cr[2 * k] : x.ld(f.crss, 2 * i + 1);
                     ot[2 * k] = x.ld(f.ttau, 2 * i);
-                    ot[2 * k + 1] = x.ld(f.ttau, 2 * i + 1);
+                    ot[2 * k + 1] = SYMTAU ? ot[2 * k] : x.ld(f.ttau, 2 * i + 1);
                 }
 #pragma unroll
                 for (int k = 0; k < SB; k++) {
@@ -563,7 +564,7 @@ __device__ __forceinline__ double harden_voxel_chunked(const Fields& f, TB tab, 
                     tn2 += t0 * t0;
                     tn2 += t1 * t1;
                     x.st(f.ttau, 2 * i, t0);
-                    x.st(f.ttau, 2 * i + 1, t1);
+                    if (!SYMTAU) x.st(f.ttau, 2 * i + 1, t1);
                 }
             }
         }
@@ -729,7 +730,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
         // harden with the slip rates of the last evaluation
         if (pc.iuphard == 1) {
             double g1;
-            const double e2 = SMALLNS ? harden_small<SYMTAU>(f, tab, P, x, tdot) : harden_voxel_chunked(f, tab, P, x, tdot);
+            const double e2 = SMALLNS ? harden_small<SYMTAU>(f, tab, P, x, tdot) : harden_voxel_chunked<SYMTAU>(f, tab, P, x, tdot);
             errtau = sqrt(e2) / taun;
         } else {
             errtau = 0.0;
@@ -1525,21 +1526,22 @@ __global__ __launch_bounds__(BS) void k_solve_ops(Fields f, double tdot) {
                  i == j ? x.ld(f.sg66, 7 * i) / tdot : 0.5 * (x.ld(f.sg66, 6 * i + j) + x.ld(f.sg66, 6 * j + i)) / tdot);
 }
 // |ttau| per solid voxel (columns in order, as the solve's own norm) after the trial
-// strengths were written by initialisation, a rollback or the host
-__global__ __launch_bounds__(BS) void k_taun(Fields f, PhaseTables pt) {
+// strengths were written by initialisation, a rollback or the host; sym: the forward
+// column stands for both
+__global__ __launch_bounds__(BS) void k_taun(Fields f, PhaseTables pt, int sym) {
     const long V = f.V;
     const long v = (long)blockIdx.x * BS + threadIdx.x;
     if (v >= V || f.gas[v]) return;
     const int ns = pt.ph[f.phase[v]].nsyst;
     double t2 = 0.0;
     for (int k = 0; k < 2 * ns; k++) {
-        const double t = f.ttau[(long)k * V + v];
+        const double t = f.ttau[(long)(sym ? k & ~1 : k) * V + v];
         t2 += t * t;
     }
     f.taun[v] = sqrt(t2);
 }
-void launch_taun(const Fields& f, const PhaseTables& pt, hipStream_t s) {
-    hipLaunchKernelGGL(k_taun, dim3(nblk(f.V)), dim3(BS), 0, s, f, pt);
+void launch_taun(const Fields& f, const PhaseTables& pt, int sym, hipStream_t s) {
+    hipLaunchKernelGGL(k_taun, dim3(nblk(f.V)), dim3(BS), 0, s, f, pt, sym);
 }
 void launch_solve_ops(const Fields& f, double tdot, hipStream_t s) {
     hipLaunchKernelGGL(k_solve_ops, dim3(nblk(f.V)), dim3(BS), 0, s, f, tdot);

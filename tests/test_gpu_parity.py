@@ -252,3 +252,33 @@ def test_non_integer_exponent_matches_oracle():
     check_rows(rows)
     for k in fo:
         assert relerr(fo[k], fg[k]) < FIELD_TOL, k
+
+
+def test_equal_strengths_mid_increment():
+    """Equal forward and backward strengths: in the middle of an increment trialtau reads back
+    with equal columns, and writing it back (both columns from the host, |ttau| rebuilt) leaves
+    the following passes unchanged."""
+    import ctypes as C
+
+    from creep_modeling_amd import driver, lib
+
+    p = problem(0, n=16, ngrains=8, nsteps=3)
+    res = []
+    for rewrite in (False, True):
+        g = driver.Solver(p)
+        g.step()
+        P = g.procs[0]
+        assert g.L.cg_begin_increment(g.h, C.byref(P), int(P.iupdate == 1)) == 0
+        st, out = lib.CgIterState(0, 1.0), lib.CgIterOut()
+        for _ in range(4):
+            assert g.L.cg_iterate(g.h, C.byref(P), C.byref(st), C.byref(out)) == 0
+            if rewrite:
+                tt = g.ctx.field("trialtau").reshape(p.nvox, -1)
+                assert np.array_equal(tt[:, 0::2], tt[:, 1::2])
+                g.ctx.set_field("trialtau", tt)
+        res.append([g.ctx.field(k).reshape(p.nvox, -1) for k in ("sg", "trialtau", "crss")])
+        g.close()
+    (sa, ta, ca), (sb, tb, cb) = res
+    assert not np.array_equal(ta[:, 0::2], ca[:, 0::2])  # the solve's hardens moved the strengths
+    assert np.array_equal(ta[:, 0::2], ta[:, 1::2])
+    assert relerr(sa, sb) < 1e-12 and relerr(ta, tb) < 1e-12
