@@ -746,7 +746,6 @@ int fft_integrate(const Fields& f, FftCtx* c, int ideriv, double* dv, hipStream_
 
 namespace cg {
 
-constexpr int XBS = 128;  // threads per block: x passes (a few rows of 9 components)
 constexpr int YBS = 256;  // threads per block: y passes
 
 
@@ -754,14 +753,22 @@ template <int N> struct RegDims {
     static constexpr int T = Shape<N>::T, E = N / T, U = (E + T - 1) / T;
     static constexpr int AN = U * T > E ? U * T : E;  // register slots per thread
     static constexpr int NS = YBS / T;                 // signals per y block
-    static constexpr int NSX = XBS / T;                // signals per x block
-    static constexpr int RW = NSX / 5;                 // rows per x block (5 signals a row)
     // z pass: KT whole kx columns of 9 components per block, 24 signals' worth of threads
     // (216) for T = 4 and 8, one column (144 threads) for T = 16
     static constexpr int KT = T >= 16 ? 1 : 24 / T;    // kx columns per z block
     static constexpr int NSZ = 9 * KT;                 // signals per z block
     static constexpr int ZB = NSZ * T;                 // threads per z block
-    static_assert(RW >= 1 && KT >= 1, "block shapes");
+    static_assert(KT >= 1, "z block shape");
+};
+// x passes: XB threads per block, RW rows of 9 components (5 signals a row).  FP64 rows run
+// in one-wave blocks (more blocks per CU, and their barriers hold one wave); FP32 rows and
+// T = 16 transforms in two-wave blocks.
+template <int N, typename R> struct XDims {
+    static constexpr int T = Shape<N>::T;
+    static constexpr int XB = (T >= 16 || sizeof(R) < 8) ? 128 : 64;
+    static constexpr int NSX = XB / T;  // signals per block
+    static constexpr int RW = NSX / 5;  // rows per block
+    static_assert(RW >= 1, "x block shape");
 };
 
 template <typename R> __device__ __forceinline__ typename Cx<R>::T* spec_of(const Fields& f) {
@@ -772,11 +779,12 @@ template <typename R> __device__ __forceinline__ typename Cx<R>::T* spec_of(cons
 // Components (2p, 2p+1) of a row form one complex signal; the two half spectra are
 // separated with X[N-k] afterwards.
 template <int N, typename R>
-__global__ __launch_bounds__(XBS) void k_x_fwd_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
+__global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
                                                    const double* __restrict__ sgsum, double inv_scale) {
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
-    constexpr int T = D::T, E = D::E, U = D::U, NS = D::NSX, RW = D::RW;
+    using X = XDims<N, R>;
+    constexpr int T = D::T, E = D::E, U = D::U, NS = X::NSX, RW = X::RW, XBS = X::XB;
     __shared__ T2 ex[N * NS];
     R* stage = (R*)ex;  // RW*9*N reals, dead before the exchange
     static_assert(RW * 9 * N * sizeof(R) <= N * NS * sizeof(T2), "stage fits the exchange");
@@ -928,11 +936,12 @@ __global__ __launch_bounds__(RegDims<N>::ZB) void k_z_green_reg(Fields f, const 
 // spectrum (imaginary parts at kx = 0 and n1/2 dropped, as Re() of the reference's
 // c2c inverse does); block partial sums of w dLref F^-1.
 template <int N, typename R>
-__global__ __launch_bounds__(XBS) void k_x_inv_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
+__global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
                                                    double out_scale) {
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
-    constexpr int T = D::T, E = D::E, U = D::U, NS = D::NSX, RW = D::RW;
+    using X = XDims<N, R>;
+    constexpr int T = D::T, E = D::E, U = D::U, NS = X::NSX, RW = X::RW, XBS = X::XB;
     constexpr int H = N / 2 + 1;
     __shared__ T2 ex[N * NS];
     static_assert(RW * H * 9 <= N * NS, "spectrum rows fit the exchange");
@@ -1033,14 +1042,16 @@ bool fft_reg_supported(const Fields& f) { return reg_size(f.n1) && reg_size(f.n2
 template <int N, typename R> static void launch_x_fwd_reg(const Fields& f, const typename Cx<R>::T* tw, const double* sgsum,
                                                           double inv_scale, hipStream_t s) {
     const long nrows = (long)f.n2 * f.n3;
-    const int nb = (int)((nrows + RegDims<N>::RW - 1) / RegDims<N>::RW);
-    hipLaunchKernelGGL((k_x_fwd_reg<N, R>), dim3(nb), dim3(XBS), 0, s, f, tw, sgsum, inv_scale);
+    using X = XDims<N, R>;
+    const int nb = (int)((nrows + X::RW - 1) / X::RW);
+    hipLaunchKernelGGL((k_x_fwd_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, f, tw, sgsum, inv_scale);
 }
 template <int N, typename R> static int launch_x_inv_reg(const Fields& f, const typename Cx<R>::T* tw, double out_scale,
                                                          hipStream_t s) {
     const long nrows = (long)f.n2 * f.n3;
-    const int nb = (int)((nrows + RegDims<N>::RW - 1) / RegDims<N>::RW);
-    hipLaunchKernelGGL((k_x_inv_reg<N, R>), dim3(nb), dim3(XBS), 0, s, f, tw, out_scale);
+    using X = XDims<N, R>;
+    const int nb = (int)((nrows + X::RW - 1) / X::RW);
+    hipLaunchKernelGGL((k_x_inv_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, f, tw, out_scale);
     return nb;
 }
 template <int N, bool INV, typename R> static void launch_y_reg(const Fields& f, const typename Cx<R>::T* tw, hipStream_t s) {
