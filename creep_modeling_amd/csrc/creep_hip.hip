@@ -538,7 +538,7 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     if (c->mixed) scale = std::max(c->svmavg, 1.0);
     tstart(c, 0);
     nb = fft_green(f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, scale, c->s);
-    launch_reduce_final(f.part, nb, 9, c->d_fluct, c->s);
+    launch_reduce_final_cm(f.part, nb, 9, c->d_fluct, c->s);
     tstop(c, 0);
     // correction + mixing + constitutive solve (one kernel)
     PassConst pc;

@@ -451,7 +451,7 @@ __global__ __launch_bounds__(FBS) void k_x_inv(Fields f, Plan3 p, int rows, doub
         const double w = f.w[v];
         for (int c = 0; c < 9; c++) acc[c] += q[c] * w;
     }
-    block_reduce_store<9, FBS>(acc, f.part);
+    block_reduce_store_cm<9, FBS>(acc, f.part, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1033,7 +1033,7 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_reg(Fields f, const
         mm3(d, Fp[q], q9);
         for (int c = 0; c < 9; c++) acc[c] += q9[c] * wp[q];
     }
-    block_reduce_store<9, XBS>(acc, f.part);
+    block_reduce_store_cm<9, XBS>(acc, f.part, 0);
 }
 
 static bool reg_size(int n) { return n == 16 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128 || n == 256; }

@@ -113,7 +113,7 @@ struct FftCtx;
 FftCtx* fft_create(const Fields& f, char* err, int errlen);
 void fft_destroy(FftCtx* c);
 // PK1 -> Gamma convolution -> dLref; writes the 9 block partial sums of
-// w dLref F^-1 to f.part and returns the number of blocks.
+// w dLref F^-1 to f.part, column-major (part[i * nblocks + b]), and returns the number of blocks.
 int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
               int mixed, double scale, hipStream_t s);
 // spectral integration of L F for the nodal update: dv (3 x V) and 9 block partial
