@@ -205,8 +205,8 @@ int cg_create(int n1, int n2, int n3, int nph, double complgas, int modal, int d
     if (!out) return -1;
     *out = nullptr;
     if (n1 < 2 || n2 < 1 || n3 < 1 || nph < 1 || nph > MAXPH) return -1;
-    // field columns are addressed through buffer resources over 8 columns: 64 V < 2^32
-    if ((long)n1 * n2 * n3 >= (1L << 26)) return -1;
+    // field columns are addressed through buffer resources over 8 columns: 64 ld < 2^32
+    if ((long)n1 * n2 * n3 >= (1L << 26) - 8192) return -1;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return -4;
     cg_ctx* c = new cg_ctx();
@@ -221,6 +221,12 @@ int cg_create(int n1, int n2, int n3, int nph, double complgas, int modal, int d
     *out = c;
     return 0;
 }
+
+// Column stride of the per-voxel fields: V rounded up to 4096 doubles plus 256 (2 KiB), so that
+// the columns of a field (and of different fields) do not all start at the same offset modulo
+// the HBM channel interleave; at power-of-two V an unpadded stride serialises a kernel's column
+// streams on a few channels.
+static long field_stride(long V) { return (V + 4095) / 4096 * 4096 + 256; }
 
 static void free_fields(cg_ctx* c) {
     Fields& f = c->f;
@@ -298,6 +304,8 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     const long V = c->V;
     Fields& f = c->f;
     f.n1 = c->n1; f.n2 = c->n2; f.n3 = c->n3; f.h1 = c->n1 / 2 + 1; f.V = V;
+    f.ld = field_stride(V);
+    const long LD = f.ld;
     f.ns = std::max(c->ns, 1);
     const size_t ns = f.ns;
     // phase tables
@@ -374,21 +382,21 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     HIPCHK(c, hipMemcpy(f.phase, hph.data(), V, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(f.gas, hgas.data(), V, hipMemcpyHostToDevice));
     int rc = 0;
-    rc |= dalloc(c, &f.w, V); rc |= dalloc(c, &f.sg, 6 * V); rc |= dalloc(c, &f.sgt, 6 * V);
-    rc |= dalloc(c, &f.L, 9 * V); rc |= dalloc(c, &f.Lold, 9 * V); rc |= dalloc(c, &f.dLref, 9 * V);
-    rc |= dalloc(c, &f.F, 9 * V); rc |= dalloc(c, &f.Fi, 9 * V); rc |= dalloc(c, &f.J, V);
-    rc |= dalloc(c, &f.Fe, 9 * V); rc |= dalloc(c, &f.Fp, 9 * V); rc |= dalloc(c, &f.ag, 9 * V);
-    rc |= dalloc(c, &f.sg66, 36 * V); rc |= dalloc(c, &f.c66m, 36 * V);
-    if (c->modal) rc |= dalloc(c, &f.cgm, 36 * V);
-    else rc |= dalloc(c, &f.sop, 42 * V);
-    if (c->ngas) rc |= dalloc(c, &f.cgas, 36 * V);
-    rc |= dalloc(c, &f.sch, 5 * ns * V); rc |= dalloc(c, &f.schA, 18 * V); rc |= dalloc(c, &f.crss, 2 * ns * V); rc |= dalloc(c, &f.ttau, 2 * ns * V);
-    rc |= dalloc(c, &f.taun, V);
-    rc |= dalloc(c, &f.gdot, ns * V); rc |= dalloc(c, &f.gacc, V);
-    rc |= dalloc(c, &f.edotp, 6 * V); rc |= dalloc(c, &f.ept, 6 * V); rc |= dalloc(c, &f.dgs, 6 * V);
-    rc |= dalloc(c, &f.epvm, V); rc |= dalloc(c, &f.Finc, 9 * V); rc |= dalloc(c, &f.Feinc, 9 * V);
-    rc |= dalloc(c, &f.s_L, 9 * V); rc |= dalloc(c, &f.s_Lold, 9 * V); rc |= dalloc(c, &f.s_sg, 6 * V);
-    rc |= dalloc(c, &f.s_edotp, 6 * V); rc |= dalloc(c, &f.s_gdot, ns * V); rc |= dalloc(c, &f.s_ttau, 2 * ns * V);
+    rc |= dalloc(c, &f.w, LD); rc |= dalloc(c, &f.sg, 6 * LD); rc |= dalloc(c, &f.sgt, 6 * LD);
+    rc |= dalloc(c, &f.L, 9 * LD); rc |= dalloc(c, &f.Lold, 9 * LD); rc |= dalloc(c, &f.dLref, 9 * LD);
+    rc |= dalloc(c, &f.F, 9 * LD); rc |= dalloc(c, &f.Fi, 9 * LD); rc |= dalloc(c, &f.J, LD);
+    rc |= dalloc(c, &f.Fe, 9 * LD); rc |= dalloc(c, &f.Fp, 9 * LD); rc |= dalloc(c, &f.ag, 9 * LD);
+    rc |= dalloc(c, &f.sg66, 36 * LD); rc |= dalloc(c, &f.c66m, 36 * LD);
+    if (c->modal) rc |= dalloc(c, &f.cgm, 36 * LD);
+    else rc |= dalloc(c, &f.sop, 42 * LD);
+    if (c->ngas) rc |= dalloc(c, &f.cgas, 36 * LD);
+    rc |= dalloc(c, &f.sch, 5 * ns * LD); rc |= dalloc(c, &f.schA, 18 * LD); rc |= dalloc(c, &f.crss, 2 * ns * LD); rc |= dalloc(c, &f.ttau, 2 * ns * LD);
+    rc |= dalloc(c, &f.taun, LD);
+    rc |= dalloc(c, &f.gdot, ns * LD); rc |= dalloc(c, &f.gacc, LD);
+    rc |= dalloc(c, &f.edotp, 6 * LD); rc |= dalloc(c, &f.ept, 6 * LD); rc |= dalloc(c, &f.dgs, 6 * LD);
+    rc |= dalloc(c, &f.epvm, LD); rc |= dalloc(c, &f.Finc, 9 * LD); rc |= dalloc(c, &f.Feinc, 9 * LD);
+    rc |= dalloc(c, &f.s_L, 9 * LD); rc |= dalloc(c, &f.s_Lold, 9 * LD); rc |= dalloc(c, &f.s_sg, 6 * LD);
+    rc |= dalloc(c, &f.s_edotp, 6 * LD); rc |= dalloc(c, &f.s_gdot, ns * LD); rc |= dalloc(c, &f.s_ttau, 2 * ns * LD);
     const size_t nspec = (size_t)f.h1 * f.n2 * f.n3 * 9;
     // the FP64 spectrum is also used by the nodal integration in the mixed form
     if (c->mixed) HIPCHK(c, hipMalloc(&f.specf, nspec * sizeof(float2)));
@@ -688,7 +696,7 @@ int cg_end_increment(cg_ctx* c, const cg_process* p, int scan_diagnostics, cg_in
     PhaseTables pt{c->d_ph, c->d_tab};
     tstart(c, 3);
     if (c->lold_same) {  // velgradold == velgrad after the last pass
-        launch_copy(f.Lold, f.L, 9 * V, c->s);
+        launch_copy(f.Lold, f.L, 9 * f.ld, c->s);
         c->lold_same = 0;
     }
     int nb;
@@ -746,8 +754,8 @@ int cg_end_increment(cg_ctx* c, const cg_process* p, int scan_diagnostics, cg_in
     double last[13] = {0};
     if (c->last_solid >= 0) {
         for (int k = 0; k < 6; k++) {
-            HIPCHK(c, hipMemcpy(&last[k], f.dgs + k * V + c->last_solid, sizeof(double), hipMemcpyDeviceToHost));
-            HIPCHK(c, hipMemcpy(&last[6 + k], f.ept + k * V + c->last_solid, sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(c, hipMemcpy(&last[k], f.dgs + k * f.ld + c->last_solid, sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(c, hipMemcpy(&last[6 + k], f.ept + k * f.ld + c->last_solid, sizeof(double), hipMemcpyDeviceToHost));
         }
         HIPCHK(c, hipMemcpy(&last[12], f.w + c->last_solid, sizeof(double), hipMemcpyDeviceToHost));
     }
@@ -901,12 +909,13 @@ int cg_get_field(cg_ctx* c, const char* name, double* out, size_t n) {
     const long V = c->V;
     if (n < (size_t)d.ncomp * V) return fail(c, -1, "output buffer too small");
     if (d.p == c->f.Lold && c->lold_same) {  // velgradold == velgrad since the last pass
-        launch_copy(c->f.Lold, c->f.L, 9 * V, c->s);
+        launch_copy(c->f.Lold, c->f.L, 9 * c->f.ld, c->s);
         c->lold_same = 0;
     }
     std::vector<double> h((size_t)d.stored * V);
     HIPCHK(c, hipStreamSynchronize(c->s));
-    HIPCHK(c, hipMemcpy(h.data(), d.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < d.stored; k++)
+        HIPCHK(c, hipMemcpy(h.data() + (size_t)k * V, d.p + (size_t)k * c->f.ld, V * sizeof(double), hipMemcpyDeviceToHost));
     // equal forward and backward strengths: the hardens inside the solve write only the
     // forward column of trialtau (the increment's last harden writes both)
     if (c->sym_tau && d.p == c->f.ttau)
@@ -933,7 +942,7 @@ int cg_set_field(cg_ctx* c, const char* name, const double* in, size_t n) {
     const long V = c->V;
     if (n < (size_t)d.ncomp * V) return fail(c, -1, "input buffer too small");
     if (c->lold_same) {  // keep velgradold (== velgrad since the last pass) before any field changes
-        launch_copy(c->f.Lold, c->f.L, 9 * V, c->s);
+        launch_copy(c->f.Lold, c->f.L, 9 * c->f.ld, c->s);
         c->lold_same = 0;
     }
     if (c->sym_tau && (d.p == c->f.crss || d.p == c->f.ttau)) {
@@ -955,7 +964,8 @@ int cg_set_field(cg_ctx* c, const char* name, const double* in, size_t n) {
         }
     }
     HIPCHK(c, hipStreamSynchronize(c->s));
-    HIPCHK(c, hipMemcpy(d.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    for (int k = 0; k < d.stored; k++)
+        HIPCHK(c, hipMemcpy(d.p + (size_t)k * c->f.ld, h.data() + (size_t)k * V, V * sizeof(double), hipMemcpyHostToDevice));
     c->sgsum_valid = 0;
     return 0;
 }

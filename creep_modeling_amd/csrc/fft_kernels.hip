@@ -148,8 +148,8 @@ __global__ __launch_bounds__(FBS) void k_x_fwd_pk1(Fields f, Plan3 p, const doub
         const int r = e / n1, i = e - r * n1;
         const long v = (row0 + r) * n1 + i;
         double s6[6], s[9], Fi[9], pk[9];
-        for (int c = 0; c < 6; c++) s6[c] = f.sg[c * V + v];
-        for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * V + v];
+        for (int c = 0; c < 6; c++) s6[c] = f.sg[c * f.ld + v];
+        for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * f.ld + v];
         from6(s6, s);
         const double J = f.J[v];
         for (int c = 0; c < 9; c++) s[c] -= sa[c];
@@ -444,8 +444,8 @@ __global__ __launch_bounds__(FBS) void k_x_inv(Fields f, Plan3 p, int rows, doub
         double d[9], Fi[9], q[9];
         for (int c = 0; c < 9; c++) {
             d[c] = (double)res[(r * 9 + c) * ss + i].x * out_scale;
-            f.dLref[c * V + v] = d[c];
-            Fi[c] = f.Fi[c * V + v];
+            f.dLref[c * f.ld + v] = d[c];
+            Fi[c] = f.Fi[c * f.ld + v];
         }
         mm3(d, Fi, q);
         const double w = f.w[v];
@@ -592,7 +592,7 @@ __global__ __launch_bounds__(FBS) void k_x_fwd_lf(Fields f, Plan3 p, int rows) {
         const int r = e / n1, i = e - r * n1;
         const long v = (row0 + r) * n1 + i;
         double L[9], F[9], q[9];
-        for (int c = 0; c < 9; c++) { L[c] = f.L[c * V + v]; F[c] = f.F[c * V + v]; }
+        for (int c = 0; c < 9; c++) { L[c] = f.L[c * f.ld + v]; F[c] = f.F[c * f.ld + v]; }
         mm3(L, F, q);
         for (int c = 0; c < 9; c++) {
             b0[(r * 9 + c) * ss + i] = make_double2(q[c], 0.0);
@@ -804,8 +804,8 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_reg(Fields f, const
         const long row = row0 + r;
         const long v = (row < nrows ? row : nrows - 1) * N + i;
         double s6[6], s[9], Fi[9], pk[9];
-        for (int c = 0; c < 6; c++) s6[c] = f.sg[c * V + v];
-        for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * V + v];
+        for (int c = 0; c < 6; c++) s6[c] = f.sg[c * f.ld + v];
+        for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * f.ld + v];
         const double J = row < nrows ? f.J[v] : 0.0;
         from6(s6, s);
         for (int c = 0; c < 9; c++) s[c] -= sa[c];
@@ -972,7 +972,7 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_reg(Fields f, const
         const int rr = e / N, i = e - rr * N;
         const long row = row0 + rr;
         const long v = (row < nrows ? row : nrows - 1) * N + i;
-        for (int c = 0; c < 9; c++) Fp[q][c] = f.Fi[c * V + v];
+        for (int c = 0; c < 9; c++) Fp[q][c] = f.Fi[c * f.ld + v];
         wp[q] = row < nrows ? f.w[v] : 0.0;
     }
 #pragma unroll
@@ -1028,7 +1028,7 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_reg(Fields f, const
         double d[9], q9[9];
         for (int c = 0; c < 9; c++) {
             d[c] = (double)out[(rr * 9 + c) * N + i] * out_scale;
-            if (in) f.dLref[c * V + v] = d[c];
+            if (in) f.dLref[c * f.ld + v] = d[c];
         }
         mm3(d, Fp[q], q9);
         for (int c = 0; c < 9; c++) acc[c] += q9[c] * wp[q];

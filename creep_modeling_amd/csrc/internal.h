@@ -31,6 +31,8 @@ struct PhaseTables {
 struct Fields {
     int n1, n2, n3, h1;   // h1 = n1/2 + 1 (half spectrum along x)
     long V;
+    long ld;              // column stride of every per-voxel field (a[c*ld + v]), V padded so that the
+                          // columns of a field start at staggered HBM channel offsets
     int ns;               // slip systems per voxel (max over phases)
     uint8_t* phase;       // 0-based phase id
     uint8_t* gas;

@@ -165,7 +165,7 @@ __global__ __launch_bounds__(BS) void k_sigma_sum(Fields f) {
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (v < f.V) {
         double s6[6], s[9];
-        for (int c = 0; c < 6; c++) s6[c] = f.sg[c * f.V + v];
+        for (int c = 0; c < 6; c++) s6[c] = f.sg[c * f.ld + v];
         from6(s6, s);
         const double w = f.w[v];
         for (int c = 0; c < 9; c++) acc[c] = s[c] * w;
@@ -786,7 +786,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 :
     const long v = solve_block(blockIdx.x, gridDim.x) * SBS + threadIdx.x;
     const bool act = v < V;
     const PassConst& pc = a.pc;
-    const Vox x(v, V);
+    const Vox x(v, f.ld);
     // LDS of the symmetric solve: per thread 21 (packed cm) + 18 (A, A^-1) doubles;
     // the block reductions reuse it when no solve is using it
     __shared__ double pool[MODAL ? 1 : 39 * SBS];
@@ -1093,17 +1093,17 @@ __global__ __launch_bounds__(BS) void k_update_schmid(Fields f, PhaseTables pt) 
     const double* dn = pt.data + P.off_dnca;
     const double* db = pt.data + P.off_dbca;
     double Fe[9], ag[9], A[9], Ai[9];
-    for (int c = 0; c < 9; c++) { Fe[c] = f.Fe[c * V + v]; ag[c] = f.ag[c * V + v]; }
+    for (int c = 0; c < 9; c++) { Fe[c] = f.Fe[c * f.ld + v]; ag[c] = f.ag[c * f.ld + v]; }
     mm3(Fe, ag, A);
     inv3(A, Ai);
     for (int c = 0; c < 9; c++) {
-        f.schA[c * V + v] = A[c];
-        f.schA[(9 + c) * V + v] = Ai[c];
+        f.schA[c * f.ld + v] = A[c];
+        f.schA[(9 + c) * f.ld + v] = Ai[c];
     }
     for (int is = 0; is < P.nsyst; is++) {
         double s5[5];
         schmid5(A, Ai, db + 3 * is, dn + 3 * is, s5);
-        for (int c = 0; c < 5; c++) f.sch[(5 * is + c) * V + v] = s5[c];
+        for (int c = 0; c < 5; c++) f.sch[(5 * is + c) * f.ld + v] = s5[c];
     }
 }
 void launch_update_schmid(const Fields& f, const PhaseTables& pt, hipStream_t s) {
@@ -1124,17 +1124,17 @@ __global__ __launch_bounds__(BS) void k_c066mod(Fields f, C066Args a, int modal)
     const long v = (long)blockIdx.x * BS + threadIdx.x;
     if (v >= V) return;
     double F[9], M[36], c[36];
-    for (int i = 0; i < 9; i++) F[i] = f.F[i * V + v];
+    for (int i = 0; i < 9; i++) F[i] = f.F[i * f.ld + v];
     const double J = f.J[v];
     push_matrix(F, M);
     congr6(M, a.c066, c);
     for (int i = 0; i < 36; i++) {
         c[i] /= J;
-        f.c66m[(long)i * V + v] = c[i];
+        f.c66m[(long)i * f.ld + v] = c[i];
     }
     if (modal) {
         double Fi[9], Y[6][9], g66[36];
-        for (int i = 0; i < 9; i++) Fi[i] = f.Fi[i * V + v];
+        for (int i = 0; i < 9; i++) Fi[i] = f.Fi[i * f.ld + v];
         for (int cc = 0; cc < 6; cc++) {
             double b[9];
             basis(cc, b);
@@ -1155,7 +1155,7 @@ __global__ __launch_bounds__(BS) void k_c066mod(Fields f, C066Args a, int modal)
         }
         double r[36];
         mm6(c, g66, r);
-        for (int i = 0; i < 36; i++) f.cgm[(long)i * V + v] = r[i];
+        for (int i = 0; i < 36; i++) f.cgm[(long)i * f.ld + v] = r[i];
     }
 }
 void launch_c066mod(const Fields& f, const double* c066, const double* goperr0, int modal, hipStream_t s) {
@@ -1172,15 +1172,15 @@ __global__ __launch_bounds__(BS) void k_gas_compl(Fields f, double tdot, double 
     if (v >= V) return;
     double a[36];
     if (!f.gas[v]) {
-        for (int i = 0; i < 36; i++) f.cgas[(long)i * V + v] = 0.0;
+        for (int i = 0; i < 36; i++) f.cgas[(long)i * f.ld + v] = 0.0;
         return;
     }
     for (int i = 0; i < 36; i++) {
-        a[i] = (i % 7 == 0 ? 1.0 : 0.0) + f.c66m[(long)i * V + v] / tdot * complgas;
-        if (modal) a[i] -= f.cgm[(long)i * V + v];
+        a[i] = (i % 7 == 0 ? 1.0 : 0.0) + f.c66m[(long)i * f.ld + v] / tdot * complgas;
+        if (modal) a[i] -= f.cgm[(long)i * f.ld + v];
     }
     inv6(a);
-    for (int i = 0; i < 36; i++) f.cgas[(long)i * V + v] = a[i];
+    for (int i = 0; i < 36; i++) f.cgas[(long)i * f.ld + v] = a[i];
 }
 void launch_gas_compl(const Fields& f, double tdot, double complgas, int modal, hipStream_t s) {
     hipLaunchKernelGGL(k_gas_compl, dim3(nblk(f.V)), dim3(BS), 0, s, f, tdot, complgas, modal);
@@ -1302,13 +1302,13 @@ __global__ __launch_bounds__(BS) void k_accumulate(Fields f, double tdot) {
     if (v < V) {
         double ep[6], L[9], d6[6];
         for (int c = 0; c < 6; c++) {
-            ep[c] = f.edotp[c * V + v];
-            f.ept[c * V + v] += ep[c] * tdot;
-            f.sgt[c * V + v] = f.sg[c * V + v];
+            ep[c] = f.edotp[c * f.ld + v];
+            f.ept[c * f.ld + v] += ep[c] * tdot;
+            f.sgt[c * f.ld + v] = f.sg[c * f.ld + v];
         }
-        for (int c = 0; c < 9; c++) L[c] = f.L[c * V + v];
+        for (int c = 0; c < 9; c++) L[c] = f.L[c * f.ld + v];
         to6(L, d6);
-        for (int c = 0; c < 6; c++) f.dgs[c * V + v] += d6[c] * tdot;
+        for (int c = 0; c < 6; c++) f.dgs[c * f.ld + v] += d6[c] * tdot;
         f.epvm[v] += vm_strain6(ep) * tdot;
         if (!f.gas[v]) acc[0] = vm_strain6(d6);
     }
@@ -1328,22 +1328,22 @@ __global__ __launch_bounds__(BS) void k_plastic_defgrad(Fields f, PhaseTables pt
     const double* dn = pt.data + P.off_dnca;
     const double* db = pt.data + P.off_dbca;
     double ag[9], Lp[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < 9; c++) ag[c] = f.ag[c * V + v];
+    for (int c = 0; c < 9; c++) ag[c] = f.ag[c * f.ld + v];
     for (int is = 0; is < P.nsyst; is++) {
         double n[3], b[3];
         for (int i = 0; i < 3; i++) {
             n[i] = ag[3 * i] * dn[3 * is] + ag[3 * i + 1] * dn[3 * is + 1] + ag[3 * i + 2] * dn[3 * is + 2];
             b[i] = ag[3 * i] * db[3 * is] + ag[3 * i + 1] * db[3 * is + 1] + ag[3 * i + 2] * db[3 * is + 2];
         }
-        const double g = f.gdot[is * V + v];
+        const double g = f.gdot[is * f.ld + v];
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) Lp[3 * i + j] += b[i] * n[j] * g;
     }
     double d[9], E[9], Fp[9];
-    for (int c = 0; c < 9; c++) { d[c] = Lp[c] * tdot; Fp[c] = f.Fp[c * V + v]; }
+    for (int c = 0; c < 9; c++) { d[c] = Lp[c] * tdot; Fp[c] = f.Fp[c * f.ld + v]; }
     expm3(d, E);
     mm3(E, Fp, Fp);
-    for (int c = 0; c < 9; c++) f.Fp[c * V + v] = Fp[c];
+    for (int c = 0; c < 9; c++) f.Fp[c * f.ld + v] = Fp[c];
 }
 void launch_plastic_defgrad(const Fields& f, const PhaseTables& pt, double tdot, hipStream_t s) {
     hipLaunchKernelGGL(k_plastic_defgrad, dim3(nblk(f.V)), dim3(BS), 0, s, f, pt, tdot);
@@ -1358,7 +1358,7 @@ __global__ __launch_bounds__(BS) void k_update_defgrad(Fields f, double tdot) {
     for (int i = 0; i < 39; i++) acc[i] = 0.0;
     if (v < V) {
         double L[9], F[9], d[9], E[9], Fi[9];
-        for (int c = 0; c < 9; c++) { L[c] = f.L[c * V + v]; F[c] = f.F[c * V + v]; d[c] = L[c] * tdot; }
+        for (int c = 0; c < 9; c++) { L[c] = f.L[c * f.ld + v]; F[c] = f.F[c * f.ld + v]; d[c] = L[c] * tdot; }
         expm3(d, E);
         mm3(E, F, F);
         // determinant33 (tensor_functions.f90:442-456) term order
@@ -1367,9 +1367,9 @@ __global__ __launch_bounds__(BS) void k_update_defgrad(Fields f, double tdot) {
         inv3(F, Fi);
         const double wg = 1.0 / (double)V;
         for (int c = 0; c < 9; c++) {
-            f.F[c * V + v] = F[c];
-            f.Fi[c * V + v] = Fi[c];
-            f.Finc[c * V + v] = E[c];
+            f.F[c * f.ld + v] = F[c];
+            f.Fi[c * f.ld + v] = Fi[c];
+            f.Finc[c * f.ld + v] = E[c];
             acc[c] = F[c] * wg;
             acc[9 + c] = Fi[c] * wg * J;
             acc[(f.gas[v] ? 27 : 18) + c] = Fi[c] * wg * J;
@@ -1402,21 +1402,21 @@ __global__ __launch_bounds__(BS) void k_elastic_stiff(Fields f, PhaseTables pt) 
     if (v >= V || f.gas[v]) return;
     const PhaseDev& P = pt.ph[f.phase[v]];
     double F[9], Fp[9], Fpi[9], Feo[9], Feoi[9], Fe[9], Fei[9];
-    for (int c = 0; c < 9; c++) { F[c] = f.F[c * V + v]; Fp[c] = f.Fp[c * V + v]; Feo[c] = f.Fe[c * V + v]; }
+    for (int c = 0; c < 9; c++) { F[c] = f.F[c * f.ld + v]; Fp[c] = f.Fp[c * f.ld + v]; Feo[c] = f.Fe[c * f.ld + v]; }
     inv3(Feo, Feoi);
     inv3(Fp, Fpi);
     mm3(F, Fpi, Fe);
     mm3(Fe, Feoi, Fei);
-    for (int c = 0; c < 9; c++) { f.Fe[c * V + v] = Fe[c]; f.Feinc[c * V + v] = Fei[c]; }
+    for (int c = 0; c < 9; c++) { f.Fe[c * f.ld + v] = Fe[c]; f.Feinc[c * f.ld + v] = Fei[c]; }
     double ag[9], A[9], T[36], c66[36];
-    for (int c = 0; c < 9; c++) ag[c] = f.ag[c * V + v];
+    for (int c = 0; c < 9; c++) ag[c] = f.ag[c * f.ld + v];
     mm3(Fe, ag, A);
     rot_matrix(A, T);
     congr6(T, P.c66cr, c66);
     const double dFe = det3(Fe);
     for (int i = 0; i < 36; i++) c66[i] /= dFe;
     inv6(c66);
-    for (int i = 0; i < 36; i++) f.sg66[(long)i * V + v] = c66[i];
+    for (int i = 0; i < 36; i++) f.sg66[(long)i * f.ld + v] = c66[i];
 }
 void launch_elastic_stiff(const Fields& f, const PhaseTables& pt, hipStream_t s) {
     hipLaunchKernelGGL(k_elastic_stiff, dim3(nblk(f.V)), dim3(BS), 0, s, f, pt);
@@ -1429,32 +1429,32 @@ __global__ __launch_bounds__(BS) void k_tensor_config_a(Fields f) {
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (v < V) {
         double Fn[9], Fni[9], L[9];
-        for (int c = 0; c < 9; c++) { Fn[c] = f.Finc[c * V + v]; L[c] = f.L[c * V + v]; }
+        for (int c = 0; c < 9; c++) { Fn[c] = f.Finc[c * f.ld + v]; L[c] = f.L[c * f.ld + v]; }
         inv3(Fn, Fni);
         if (!f.gas[v]) {
             double R[9], lam[3], Fe[9];
             polar3(Fn, R, lam);
-            for (int c = 0; c < 9; c++) Fe[c] = f.Feinc[c * V + v];
+            for (int c = 0; c < 9; c++) Fe[c] = f.Feinc[c * f.ld + v];
             const double dfe = det3(Fe);
             double s6[6], s[9], t[9];
             // disgradsym, ept <- R (.) R^T
-            for (int c = 0; c < 6; c++) s6[c] = f.dgs[c * V + v];
+            for (int c = 0; c < 6; c++) s6[c] = f.dgs[c * f.ld + v];
             from6(s6, s); mm3(R, s, t); mmt3(t, R, s); to6(s, s6);
-            for (int c = 0; c < 6; c++) f.dgs[c * V + v] = s6[c];
-            for (int c = 0; c < 6; c++) s6[c] = f.ept[c * V + v];
+            for (int c = 0; c < 6; c++) f.dgs[c * f.ld + v] = s6[c];
+            for (int c = 0; c < 6; c++) s6[c] = f.ept[c * f.ld + v];
             from6(s6, s); mm3(R, s, t); mmt3(t, R, s); to6(s, s6);
-            for (int c = 0; c < 6; c++) f.ept[c * V + v] = s6[c];
+            for (int c = 0; c < 6; c++) f.ept[c * f.ld + v] = s6[c];
             // sgt <- Fe_inc sgt Fe_inc^T / det(Fe_inc)
-            for (int c = 0; c < 6; c++) s6[c] = f.sgt[c * V + v];
+            for (int c = 0; c < 6; c++) s6[c] = f.sgt[c * f.ld + v];
             from6(s6, s); mm3(Fe, s, t); mmt3(t, Fe, s);
             for (int c = 0; c < 9; c++) s[c] /= dfe;
             to6(s, s6);
-            for (int c = 0; c < 6; c++) f.sgt[c * V + v] = s6[c];
+            for (int c = 0; c < 6; c++) f.sgt[c * f.ld + v] = s6[c];
         }
         mm3(L, Fni, L);
         const double w = f.w[v];
         for (int c = 0; c < 9; c++) {
-            f.L[c * V + v] = L[c];
+            f.L[c * f.ld + v] = L[c];
             acc[c] = L[c] * w;
         }
     }
@@ -1470,9 +1470,9 @@ __global__ __launch_bounds__(BS) void k_tensor_config_b(Fields f, Mat9 dud) {
     const long v = (long)blockIdx.x * BS + threadIdx.x;
     if (v >= V) return;
     double Fi[9], q[9];
-    for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * V + v];
+    for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * f.ld + v];
     mm3(dud.m, Fi, q);
-    for (int c = 0; c < 9; c++) f.L[c * V + v] += q[c];
+    for (int c = 0; c < 9; c++) f.L[c * f.ld + v] += q[c];
 }
 void launch_tensor_config_b(const Fields& f, const double* dud, hipStream_t s) {
     Mat9 m;
@@ -1487,7 +1487,7 @@ __global__ __launch_bounds__(BS) void k_hardening(Fields f, PhaseTables pt, doub
     if (v >= V || f.gas[v]) return;
     const PhaseDev& P = pt.ph[f.phase[v]];
     double g1;
-    harden_voxel(f, pt.data, P, Vox(v, V), tdot, &g1, true);
+    harden_voxel(f, pt.data, P, Vox(v, f.ld), tdot, &g1, true);
     f.gacc[v] = g1;
 }
 // Operators of the symmetric Newton solve for one increment (constant while c066mod,
@@ -1497,7 +1497,7 @@ __global__ __launch_bounds__(BS) void k_solve_ops(Fields f, double tdot) {
     const long V = f.V;
     const long v = (long)blockIdx.x * BS + threadIdx.x;
     if (v >= V || f.gas[v]) return;
-    const Vox x(v, V);
+    const Vox x(v, f.ld);
     double Lf[21], Ci[21];
 #pragma unroll
     for (int i = 0; i < 6; i++)
@@ -1535,7 +1535,7 @@ __global__ __launch_bounds__(BS) void k_taun(Fields f, PhaseTables pt, int sym) 
     const int ns = pt.ph[f.phase[v]].nsyst;
     double t2 = 0.0;
     for (int k = 0; k < 2 * ns; k++) {
-        const double t = f.ttau[(long)(sym ? k & ~1 : k) * V + v];
+        const double t = f.ttau[(long)(sym ? k & ~1 : k) * f.ld + v];
         t2 += t * t;
     }
     f.taun[v] = sqrt(t2);
@@ -1562,24 +1562,24 @@ __global__ __launch_bounds__(BS) void k_averages(Fields f, PhaseTables pt) {
     if (v < V && !f.gas[v]) {
         const double w = f.w[v];
         double s6[6], s[9];
-        for (int c = 0; c < 6; c++) s6[c] = f.ept[c * V + v];
+        for (int c = 0; c < 6; c++) s6[c] = f.ept[c * f.ld + v];
         from6(s6, s);
         for (int c = 0; c < 9; c++) acc[c] = s[c] * w;
-        for (int c = 0; c < 6; c++) s6[c] = f.dgs[c * V + v];
+        for (int c = 0; c < 6; c++) s6[c] = f.dgs[c * f.ld + v];
         from6(s6, s);
         for (int c = 0; c < 9; c++) acc[9 + c] = s[c] * w;
-        for (int c = 0; c < 6; c++) s6[c] = f.edotp[c * V + v];
+        for (int c = 0; c < 6; c++) s6[c] = f.edotp[c * f.ld + v];
         from6(s6, s);
         for (int c = 0; c < 9; c++) acc[27 + c] = s[c] * w;
-        for (int c = 0; c < 6; c++) s6[c] = f.sgt[c * V + v];
+        for (int c = 0; c < 6; c++) s6[c] = f.sgt[c * f.ld + v];
         from6(s6, s);
         for (int c = 0; c < 9; c++) acc[36 + c] = s[c] * w;
         acc[45] = f.gacc[v];
         acc[46] = 1.0;
         const PhaseDev& P = pt.ph[f.phase[v]];
         for (int is = 0; is < P.nsyst; is++) {
-            const double g = 0.5 * (f.crss[(2 * is) * V + v] + f.crss[(2 * is + 1) * V + v]);
-            const double a = fabs(f.gdot[is * V + v]);
+            const double g = 0.5 * (f.crss[(2 * is) * f.ld + v] + f.crss[(2 * is + 1) * f.ld + v]);
+            const double a = fabs(f.gdot[is * f.ld + v]);
             acc[47] += g;
             acc[48] += a;
             acc[49] += a * g;
@@ -1606,18 +1606,18 @@ __global__ __launch_bounds__(BS) void k_calc_c0(Fields f, PhaseTables pt, double
     if (v < V && !f.gas[v]) {
         const PhaseDev& P = pt.ph[f.phase[v]];
         double sg[6], ep[5], dd[15], T[36];
-        for (int c = 0; c < 6; c++) sg[c] = f.sg[c * V + v];
+        for (int c = 0; c < 6; c++) sg[c] = f.sg[c * f.ld + v];
         // plastic_strain_rate on a local copy (gdot is not modified: STORE=false)
-        slip_eval<INTPOW, false>(f, pt.data, P, Vox(v, V), sg, ep, dd);
+        slip_eval<INTPOW, false>(f, pt.data, P, Vox(v, f.ld), sg, ep, dd);
         for (int i = 0; i < 6; i++)
             for (int j = 0; j < 6; j++) {
-                double e = f.sg66[(long)(6 * i + j) * V + v] / tdot * tdot / tdotref;
+                double e = f.sg66[(long)(6 * i + j) * f.ld + v] / tdot * tdot / tdotref;
                 if (i < 5 && j < 5) e += dd[sym5(i, j)];
                 T[6 * i + j] = e;
             }
         inv6(T);
         double Fi[9], N[36], dP[36];
-        for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * V + v];
+        for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * f.ld + v];
         push_matrix(Fi, N);
         congr6(N, T, dP);
         const double J = f.J[v], w = f.w[v], wg = 1.0 / (double)V;
@@ -1661,7 +1661,7 @@ __global__ __launch_bounds__(BS) void k_kinematic(Fields f, double tdot) {
     double mx[3] = {0.0, 0.0, 0.0}, mn[1] = {1.7976931348623157e308};
     if (v < V) {
         double d[9], E[9];
-        for (int c = 0; c < 9; c++) d[c] = f.L[c * V + v] * tdot;
+        for (int c = 0; c < 9; c++) d[c] = f.L[c * f.ld + v] * tdot;
         expm3(d, E);
         const double Jinc = E[0] * E[4] * E[8] + E[1] * E[5] * E[6] + E[2] * E[3] * E[7] - E[6] * E[4] * E[2] -
                             E[7] * E[5] * E[0] - E[8] * E[3] * E[1];
@@ -1752,7 +1752,7 @@ __global__ __launch_bounds__(BS) void k_diagnostics(Fields f, double lfloor, int
     if (v < V) {
         double d2 = 0.0, l0 = 0.0;
         for (int c = 0; c < 9; c++) {
-            const double a = f.L[c * V + v], b = f.s_L[c * V + v];
+            const double a = f.L[c * f.ld + v], b = f.s_L[c * f.ld + v];
             d2 += (a - b) * (a - b);
             l0 += b * b;
         }
@@ -1765,18 +1765,18 @@ __global__ __launch_bounds__(BS) void k_diagnostics(Fields f, double lfloor, int
         if (gas) { sm[4] = rel * rel * w; sm[5] = w; mx[2] = rel; jg = J; ig_ = v; }
         else { sm[2] = rel * rel * w; sm[3] = w; mx[1] = rel; js = J; is_ = v; }
         double Fn[9], vol, rot, str;
-        for (int c = 0; c < 9; c++) Fn[c] = f.Finc[c * V + v];
+        for (int c = 0; c < 9; c++) Fn[c] = f.Finc[c * f.ld + v];
         kin_voxel(Fn, &vol, &rot, &str);
         mx[gas ? 4 : 3] = vol;
         mx[gas ? 6 : 5] = str;
         mx[gas ? 8 : 7] = rot;
         if (scan_cond) {
             double A[36];
-            for (int i = 0; i < 36; i++) A[i] = f.c66m[(long)i * V + v];
+            for (int i = 0; i < 36; i++) A[i] = f.c66m[(long)i * f.ld + v];
             const double cA = cond1_est6(A);
             if (gas) {
                 mx[10] = cA;
-                for (int i = 0; i < 36; i++) A[i] = f.cgas[(long)i * V + v];
+                for (int i = 0; i < 36; i++) A[i] = f.cgas[(long)i * f.ld + v];
                 mx[11] = cond1_est6(A);
             } else {
                 mx[9] = cA;
@@ -1857,7 +1857,7 @@ void launch_diagnostics(const Fields& f, double lfloor, int scan_cond, hipStream
 
 // take_snapshot / rollback (:2839-2895)
 void launch_snapshot(const Fields& f, int restore, hipStream_t s) {
-    const size_t V = f.V, ns = f.ns;
+    const size_t V = f.ld, ns = f.ns;  // whole columns, padding included
     struct { double* live; double* snap; size_t n; } T[] = {
         {f.L, f.s_L, 9 * V}, {f.Lold, f.s_Lold, 9 * V}, {f.sg, f.s_sg, 6 * V},
         {f.edotp, f.s_edotp, 6 * V}, {f.gdot, f.s_gdot, ns * V}, {f.ttau, f.s_ttau, 2 * ns * V}};
@@ -1886,22 +1886,22 @@ __global__ __launch_bounds__(BS) void k_init_voxels(Fields f, PhaseTables pt, co
         f.w[v] = wg;
         f.J[v] = 1.0;
         for (int c = 0; c < 9; c++) {
-            f.F[c * V + v] = I3[c]; f.Fi[c * V + v] = I3[c]; f.Fe[c * V + v] = I3[c]; f.Fp[c * V + v] = I3[c];
-            f.Finc[c * V + v] = I3[c]; f.Feinc[c * V + v] = I3[c];
-            f.L[c * V + v] = a.udot[c];
-            f.Lold[c * V + v] = 0.0;
-            f.dLref[c * V + v] = 0.0;
+            f.F[c * f.ld + v] = I3[c]; f.Fi[c * f.ld + v] = I3[c]; f.Fe[c * f.ld + v] = I3[c]; f.Fp[c * f.ld + v] = I3[c];
+            f.Finc[c * f.ld + v] = I3[c]; f.Feinc[c * f.ld + v] = I3[c];
+            f.L[c * f.ld + v] = a.udot[c];
+            f.Lold[c * f.ld + v] = 0.0;
+            f.dLref[c * f.ld + v] = 0.0;
         }
         for (int c = 0; c < 6; c++) {
-            f.sg[c * V + v] = 0.0; f.sgt[c * V + v] = 0.0; f.edotp[c * V + v] = 0.0;
-            f.ept[c * V + v] = 0.0; f.dgs[c * V + v] = 0.0;
+            f.sg[c * f.ld + v] = 0.0; f.sgt[c * f.ld + v] = 0.0; f.edotp[c * f.ld + v] = 0.0;
+            f.ept[c * f.ld + v] = 0.0; f.dgs[c * f.ld + v] = 0.0;
         }
         f.epvm[v] = 0.0;
         f.gacc[v] = 0.0;
-        for (int k = 0; k < f.ns; k++) f.gdot[(long)k * V + v] = 0.0;
-        for (int k = 0; k < 2 * f.ns; k++) { f.crss[(long)k * V + v] = 0.0; f.ttau[(long)k * V + v] = 0.0; }
-        for (int k = 0; k < 5 * f.ns; k++) f.sch[(long)k * V + v] = 0.0;
-        for (int k = 0; k < 18; k++) f.schA[(long)k * V + v] = 0.0;
+        for (int k = 0; k < f.ns; k++) f.gdot[(long)k * f.ld + v] = 0.0;
+        for (int k = 0; k < 2 * f.ns; k++) { f.crss[(long)k * f.ld + v] = 0.0; f.ttau[(long)k * f.ld + v] = 0.0; }
+        for (int k = 0; k < 5 * f.ns; k++) f.sch[(long)k * f.ld + v] = 0.0;
+        for (int k = 0; k < 18; k++) f.schA[(long)k * f.ld + v] = 0.0;
         if (!f.gas[v]) {
             const PhaseDev& P = pt.ph[f.phase[v]];
             // Bunge angles -> a (sample -> crystal), tensor_functions.f90:525-569; ag = a^T
@@ -1914,13 +1914,13 @@ __global__ __launch_bounds__(BS) void k_init_voxels(Fields f, PhaseTables pt, co
             A[6] = sph * sth;                    A[7] = -sth * cph;                   A[8] = cth;
             double ag[9];
             transp3(A, ag);
-            for (int c = 0; c < 9; c++) f.ag[c * V + v] = ag[c];
+            for (int c = 0; c < 9; c++) f.ag[c * f.ld + v] = ag[c];
             double T[36], c66[36], s66[36];
             rot_matrix(ag, T);
             congr6(T, P.c66cr, c66);
             for (int i = 0; i < 36; i++) { s66[i] = c66[i]; acc[i] = c66[i] * wg; }
             inv6(s66);
-            for (int i = 0; i < 36; i++) f.sg66[(long)i * V + v] = s66[i];
+            for (int i = 0; i < 36; i++) f.sg66[(long)i * f.ld + v] = s66[i];
             // initial_guess: sigma = cg66 (udot6 tdot)
             double u6[6], s6[6];
             to6(a.udot, u6);
@@ -1929,7 +1929,7 @@ __global__ __launch_bounds__(BS) void k_init_voxels(Fields f, PhaseTables pt, co
                 for (int k = 0; k < 6; k++) s += c66[6 * i + k] * (u6[k] * a.tdot);
                 s6[i] = s;
             }
-            for (int c = 0; c < 6; c++) f.sg[c * V + v] = s6[c];
+            for (int c = 0; c < 6; c++) f.sg[c * f.ld + v] = s6[c];
             // Hall-Petch + directional term; trialtau(:,1) = crss(:,2)
             const double* tau = pt.data + P.off_tau;
             const double* hpf = pt.data + P.off_hpfac;
@@ -1939,14 +1939,14 @@ __global__ __launch_bounds__(BS) void k_init_voxels(Fields f, PhaseTables pt, co
                 const double by = ag[3] * db[3 * is] + ag[4] * db[3 * is + 1] + ag[5] * db[3 * is + 2];
                 const double dd = hpf[is] * sqrt(fabs(by));
                 const double c1 = tau[3 * is] + hps + dd, c2 = tau[3 * is + 1] + hps + dd;
-                f.crss[(2 * is) * V + v] = c1;
-                f.crss[(2 * is + 1) * V + v] = c2;
-                f.ttau[(2 * is) * V + v] = c2;
-                f.ttau[(2 * is + 1) * V + v] = c2;
+                f.crss[(2 * is) * f.ld + v] = c1;
+                f.crss[(2 * is + 1) * f.ld + v] = c2;
+                f.ttau[(2 * is) * f.ld + v] = c2;
+                f.ttau[(2 * is + 1) * f.ld + v] = c2;
             }
         } else {
-            for (int c = 0; c < 9; c++) f.ag[c * V + v] = 0.0;
-            for (int i = 0; i < 36; i++) f.sg66[(long)i * V + v] = 0.0;
+            for (int c = 0; c < 9; c++) f.ag[c * f.ld + v] = 0.0;
+            for (int i = 0; i < 36; i++) f.sg66[(long)i * f.ld + v] = 0.0;
         }
     }
     block_reduce_store<36, BS>(acc, f.part);
