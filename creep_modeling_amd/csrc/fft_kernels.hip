@@ -777,6 +777,9 @@ template <int N, typename R> struct XDims {
 #ifndef CG_FFT_XCD
 #define CG_FFT_XCD 1
 #endif
+#ifndef CG_XPASS_XCD
+#define CG_XPASS_XCD 1  // the x passes too (their spectrum rows share edge lines; 2% off the FP32 stage)
+#endif
 __device__ __forceinline__ int fft_block() {
     const int b = blockIdx.x, nb = gridDim.x;
     if (!CG_FFT_XCD) return b;
@@ -803,7 +806,7 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_reg(Fields f, const
     static_assert(RW * 9 * N * sizeof(R) <= N * NS * sizeof(T2), "stage fits the exchange");
     const long V = f.V;
     const long nrows = (long)f.n2 * f.n3;
-    const long row0 = (long)blockIdx.x * RW;
+    const long row0 = (long)(CG_XPASS_XCD ? fft_block() : (int)blockIdx.x) * RW;
     double sa[9];
     for (int c = 0; c < 9; c++) sa[c] = sgsum[c];
     // each thread's voxels: every load is issued before the products (rows past the grid
@@ -963,7 +966,7 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_reg(Fields f, const
     static_assert(RW * 9 * N * sizeof(R) <= N * NS * sizeof(T2), "output rows fit the exchange");
     const long V = f.V;
     const long nrows = (long)f.n2 * f.n3;
-    const long row0 = (long)blockIdx.x * RW;
+    const long row0 = (long)(CG_XPASS_XCD ? fft_block() : (int)blockIdx.x) * RW;
     constexpr int w = H * 9;
     const T2* spec = spec_of<R>(f);
     // the block's spectrum rows: every load is issued before the first is written to LDS

@@ -306,7 +306,7 @@ __device__ __forceinline__ void slip_eval_ab(const Fields& f, TB tab, const PH& 
             schmid5(A, Ai, b3, n3, sv);
             const double s0 = sv[0], s1 = sv[1], s2 = sv[2], s3 = sv[3], s4 = sv[4];
             double rss = s0 * sg[0] + s1 * sg[1] + s2 * sg[2] + s3 * sg[3] + s4 * sg[4];
-            const double ti = 1.0 / (rss < 0.0 ? tn[k] : tp[k]);
+            const double ti = recip(rss < 0.0 ? tn[k] : tp[k]);
             rss = rss * ti;
             const double n = tab[rb + 6];
             const double dum = tab[rb + 7] * powm1<INTPOW>(fabs(rss), n);

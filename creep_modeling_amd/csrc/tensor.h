@@ -14,6 +14,25 @@
 
 namespace cg {
 
+// 1/x for a positive normal x.  On the GPU: the hardware estimate refined by two Newton-Raphson
+// steps r <- r + r (1 - x r), within an ulp or so of the rounded quotient in five instructions
+// (the IEEE division takes ten).  CG_FAST_RCP=0: the division.
+#ifndef CG_FAST_RCP
+#define CG_FAST_RCP 1
+#endif
+CG_HD double recip(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (CG_FAST_RCP) {
+        double r = __builtin_amdgcn_rcp(x);
+        double e = fma(-x, r, 1.0);
+        r = fma(r, e, r);
+        e = fma(-x, r, 1.0);
+        return fma(r, e, r);
+    }
+#endif
+    return 1.0 / x;
+}
+
 constexpr double R2 = 0.70710678118654752440;   // 1/sqrt2
 constexpr double R3 = 0.57735026918962576451;   // 1/sqrt3
 constexpr double R6 = 0.40824829046386301637;   // 1/sqrt6
@@ -183,7 +202,7 @@ CG_HD bool ldl6(double* P) {
         for (int k = 0; k < j; k++) d -= P[sp6(j, k)] * P[sp6(j, k)] * P[sp6(k, k)];
         if (!(d > 0.0)) ok = false;
         P[sp6(j, j)] = d;
-        const double r = 1.0 / d;
+        const double r = recip(d);  // K is SPD: d > 0
 #pragma unroll
         for (int i = j + 1; i < 6; i++) {
             double s = P[sp6(i, j)];
@@ -222,7 +241,7 @@ CG_HD void ldl6r(double* P, double* rd) {
 #pragma unroll
         for (int k = 0; k < j; k++) d -= P[sp6(j, k)] * P[sp6(j, k)] * P[sp6(k, k)];
         P[sp6(j, j)] = d;
-        const double r = 1.0 / d;
+        const double r = recip(d);  // K is SPD: d > 0
         rd[j] = r;
 #pragma unroll
         for (int i = j + 1; i < 6; i++) {
