@@ -128,6 +128,7 @@ __device__ __forceinline__ void fft_dist(T2* a, int s, int t, int NS, T2* ex, co
     constexpr int E = N / T;
     constexpr int U = (E + T - 1) / T;
     static_assert(E * T == N, "N = E T");
+    static_assert(T <= E, "level 2 gives thread t the transforms q = t + T u < E: needs T <= E");
     T2 b[E];
     Dft<E, INV>::run(a, b);
 #pragma unroll
