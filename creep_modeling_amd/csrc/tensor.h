@@ -271,49 +271,104 @@ CG_HD void ldl6_solve_r(const double* P, const double* rd, double* b) {
     }
 }
 // LU factorisation with partial pivoting of a 6x6 (in place; unit lower L)
+// LU with partial pivoting.  Every loop is unrolled and the row exchanges are selects over the
+// candidate rows, so A, piv and b stay in registers on the GPU (indexing them with the pivot
+// would place them in scratch memory); the arithmetic and the pivot choice are those of the
+// textbook loop.
 CG_HD bool lu6(double* A, int* piv) {
     bool ok = true;
+#pragma unroll
     for (int k = 0; k < 6; k++) {
         int p = k;
         double mx = fabs(A[6 * k + k]);
+#pragma unroll
         for (int i = k + 1; i < 6; i++) {
             double v = fabs(A[6 * i + k]);
             if (v > mx) { mx = v; p = i; }
         }
         piv[k] = p;
         if (mx == 0.0) { ok = false; continue; }
-        if (p != k)
-            for (int j = 0; j < 6; j++) { double t = A[6 * k + j]; A[6 * k + j] = A[6 * p + j]; A[6 * p + j] = t; }
+#pragma unroll
+        for (int i = k + 1; i < 6; i++) {
+            const bool sw = p == i;
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                const double t = A[6 * k + j];
+                A[6 * k + j] = sw ? A[6 * i + j] : t;
+                A[6 * i + j] = sw ? t : A[6 * i + j];
+            }
+        }
         double r = 1.0 / A[6 * k + k];
+#pragma unroll
         for (int i = k + 1; i < 6; i++) {
             double l = A[6 * i + k] * r;
             A[6 * i + k] = l;
+#pragma unroll
             for (int j = k + 1; j < 6; j++) A[6 * i + j] -= l * A[6 * k + j];
         }
     }
     return ok;
 }
+// b[k] <-> b[piv[k]] with the exchange as selects over the candidate rows
+CG_HD void swap_piv6(const int* piv, double* b, int k) {
+#pragma unroll
+    for (int i = k + 1; i < 6; i++) {
+        const bool sw = piv[k] == i;
+        const double t = b[k];
+        b[k] = sw ? b[i] : t;
+        b[i] = sw ? t : b[i];
+    }
+}
 CG_HD void lu6_solve(const double* LU, const int* piv, double* b) {
-    for (int k = 0; k < 6; k++) { int p = piv[k]; if (p != k) { double t = b[k]; b[k] = b[p]; b[p] = t; } }
-    for (int i = 1; i < 6; i++) { double s = b[i]; for (int j = 0; j < i; j++) s -= LU[6 * i + j] * b[j]; b[i] = s; }
-    for (int i = 5; i >= 0; i--) { double s = b[i]; for (int j = i + 1; j < 6; j++) s -= LU[6 * i + j] * b[j]; b[i] = s / LU[6 * i + i]; }
+#pragma unroll
+    for (int k = 0; k < 6; k++) swap_piv6(piv, b, k);
+#pragma unroll
+    for (int i = 1; i < 6; i++) {
+        double s = b[i];
+#pragma unroll
+        for (int j = 0; j < i; j++) s -= LU[6 * i + j] * b[j];
+        b[i] = s;
+    }
+#pragma unroll
+    for (int i = 5; i >= 0; i--) {
+        double s = b[i];
+#pragma unroll
+        for (int j = i + 1; j < 6; j++) s -= LU[6 * i + j] * b[j];
+        b[i] = s / LU[6 * i + i];
+    }
 }
 // solve A^T x = b given the LU of A
 CG_HD void lu6_solve_t(const double* LU, const int* piv, double* b) {
-    for (int i = 0; i < 6; i++) { double s = b[i]; for (int j = 0; j < i; j++) s -= LU[6 * j + i] * b[j]; b[i] = s / LU[6 * i + i]; }
-    for (int i = 4; i >= 0; i--) { double s = b[i]; for (int j = i + 1; j < 6; j++) s -= LU[6 * j + i] * b[j]; b[i] = s; }
-    for (int k = 5; k >= 0; k--) { int p = piv[k]; if (p != k) { double t = b[k]; b[k] = b[p]; b[p] = t; } }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        double s = b[i];
+#pragma unroll
+        for (int j = 0; j < i; j++) s -= LU[6 * j + i] * b[j];
+        b[i] = s / LU[6 * i + i];
+    }
+#pragma unroll
+    for (int i = 4; i >= 0; i--) {
+        double s = b[i];
+#pragma unroll
+        for (int j = i + 1; j < 6; j++) s -= LU[6 * j + i] * b[j];
+        b[i] = s;
+    }
+#pragma unroll
+    for (int k = 5; k >= 0; k--) swap_piv6(piv, b, k);
 }
 // inverse of a 6x6 through its LU factors (in place)
 CG_HD bool inv6(double* A) {
     double LU[36];
     int piv[6];
+#pragma unroll
     for (int i = 0; i < 36; i++) LU[i] = A[i];
     bool ok = lu6(LU, piv);
+#pragma unroll
     for (int j = 0; j < 6; j++) {
         double e[6] = {0, 0, 0, 0, 0, 0};
         e[j] = 1.0;
         lu6_solve(LU, piv, e);
+#pragma unroll
         for (int i = 0; i < 6; i++) A[6 * i + j] = e[i];
     }
     return ok;
