@@ -1485,10 +1485,18 @@ __global__ __launch_bounds__(BS) void k_hardening(Fields f, PhaseTables pt, doub
     const long V = f.V;
     const long v = (long)blockIdx.x * BS + threadIdx.x;
     if (v >= V || f.gas[v]) return;
-    const PhaseDev& P = pt.ph[f.phase[v]];
-    double g1;
-    harden_voxel(f, pt.data, P, Vox(v, f.ld), tdot, &g1, true);
-    f.gacc[v] = g1;
+    // one pass per distinct phase among the wave's lanes: inside it the phase is uniform and
+    // its tables (modes, interaction matrix) are read with scalar loads
+    const int ph = f.phase[v];
+    for (;;) {
+        const int p0 = __builtin_amdgcn_readfirstlane(ph);
+        if (ph == p0) {
+            double g1;
+            harden_voxel<true>(f, (cdbl*)pt.data, ((cphase*)pt.ph)[p0], Vox(v, f.ld), tdot, &g1, true);
+            f.gacc[v] = g1;
+            break;
+        }
+    }
 }
 // Operators of the symmetric Newton solve for one increment (constant while c066mod,
 // sg66 and tdot are): f.sop = [L D L^T factors of cm^-1 (21: unit L below the diagonal,
