@@ -777,6 +777,9 @@ template <int N, typename R> struct XDims {
 #ifndef CG_FFT_XCD
 #define CG_FFT_XCD 1
 #endif
+#ifndef CG_XPAD
+#define CG_XPAD 1
+#endif
 #ifndef CG_XPASS_XCD
 #define CG_XPASS_XCD 1  // the x passes too (their spectrum rows share edge lines; 2% off the FP32 stage)
 #endif
@@ -803,7 +806,10 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_reg(Fields f, const
     constexpr int T = D::T, E = D::E, U = D::U, NS = X::NSX, RW = X::RW, XBS = X::XB;
     __shared__ T2 ex[N * NS];
     R* stage = (R*)ex;  // RW*9*N reals, dead before the exchange
-    static_assert(RW * 9 * N * sizeof(R) <= N * NS * sizeof(T2), "stage fits the exchange");
+    // component rows of the stage are SP = N + CG_XPAD reals apart, so the rows 2p that the five
+    // signals of a row read together fall in different LDS banks
+    constexpr int SP = N + CG_XPAD;
+    static_assert(RW * 9 * SP * sizeof(R) <= N * NS * sizeof(T2), "stage fits the exchange");
     const long V = f.V;
     const long nrows = (long)f.n2 * f.n3;
     const long row0 = (long)(CG_XPASS_XCD ? fft_block() : (int)blockIdx.x) * RW;
@@ -826,7 +832,7 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_reg(Fields f, const
         from6(s6, s);
         for (int c = 0; c < 9; c++) s[c] -= sa[c];
         mmt3(s, Fi, pk);  // (sigma - <sigma>) F^-T
-        for (int c = 0; c < 9; c++) stage[(r * 9 + c) * N + i] = (R)(pk[c] * J * inv_scale);
+        for (int c = 0; c < 9; c++) stage[(r * 9 + c) * SP + i] = (R)(pk[c] * J * inv_scale);
     }
     __syncthreads();
     const int s = threadIdx.x % NS, t = threadIdx.x / NS;
@@ -838,8 +844,8 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_reg(Fields f, const
         const int i = t + T * m;
         R re = 0, im = 0;
         if (act) {
-            re = stage[(r * 9 + 2 * p) * N + i];
-            if (p < 4) im = stage[(r * 9 + 2 * p + 1) * N + i];
+            re = stage[(r * 9 + 2 * p) * SP + i];
+            if (p < 4) im = stage[(r * 9 + 2 * p + 1) * SP + i];
         }
         a[m] = T2{re, im};
     }
@@ -963,7 +969,8 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_reg(Fields f, const
     constexpr int H = N / 2 + 1;
     __shared__ T2 ex[N * NS];
     static_assert(RW * H * 9 <= N * NS, "spectrum rows fit the exchange");
-    static_assert(RW * 9 * N * sizeof(R) <= N * NS * sizeof(T2), "output rows fit the exchange");
+    constexpr int SP = N + CG_XPAD;  // component rows of the output stage (as in the x forward pass)
+    static_assert(RW * 9 * SP * sizeof(R) <= N * NS * sizeof(T2), "output rows fit the exchange");
     const long V = f.V;
     const long nrows = (long)f.n2 * f.n3;
     const long row0 = (long)(CG_XPASS_XCD ? fft_block() : (int)blockIdx.x) * RW;
@@ -1029,8 +1036,8 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_reg(Fields f, const
                 for (int jj = 0; jj < T; jj++) {
                     const T2 z = a[u * T + jj];
                     const int n = q + E * jj;
-                    out[(r * 9 + 2 * p) * N + n] = z.x;
-                    if (p < 4) out[(r * 9 + 2 * p + 1) * N + n] = z.y;
+                    out[(r * 9 + 2 * p) * SP + n] = z.x;
+                    if (p < 4) out[(r * 9 + 2 * p + 1) * SP + n] = z.y;
                 }
         }
     __syncthreads();
@@ -1045,7 +1052,7 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_reg(Fields f, const
         const long v = (in ? row : nrows - 1) * N + i;
         double d[9], q9[9];
         for (int c = 0; c < 9; c++) {
-            d[c] = (double)out[(rr * 9 + c) * N + i] * out_scale;
+            d[c] = (double)out[(rr * 9 + c) * SP + i] * out_scale;
             if (in) f.dLref[c * f.ld + v] = d[c];
         }
         mm3(d, Fp[q], q9);
