@@ -852,13 +852,18 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_reg(Fields f, const
     __syncthreads();
     fft_dist<N, T, false>(a, s, t, NS, ex, tw);
     __syncthreads();
+    // the transformed signals, SX = N + CG_XPAD apart (the nine components of a frequency
+    // read five different signals: padded, they fall in different LDS banks)
+    constexpr int SX = N + CG_XPAD;
+    static_assert(5 * RW * SX <= N * NS, "padded signals fit the exchange");
+    if (act)
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-        const int q = t + T * u;
-        if (U == 1 || q < E)
+        for (int u = 0; u < U; u++) {
+            const int q = t + T * u;
+            if (U == 1 || q < E)
 #pragma unroll
-            for (int j = 0; j < T; j++) ex[s * N + q + E * j] = a[u * T + j];
-    }
+                for (int j = 0; j < T; j++) ex[s * SX + q + E * j] = a[u * T + j];
+        }
     __syncthreads();
     T2* spec = spec_of<R>(f);
     constexpr int w = (N / 2 + 1) * 9;  // h1 = n1/2 + 1 complex values of 9 components a row
@@ -871,7 +876,7 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_reg(Fields f, const
         const long row = row0 + rr;
         if (row >= nrows) continue;
         const int sp = rr * 5 + c / 2;
-        const T2 Z = ex[sp * N + kx], Zm = ex[sp * N + (N - kx) % N];
+        const T2 Z = ex[sp * SX + kx], Zm = ex[sp * SX + (N - kx) % N];
         T2 o;
         if ((c & 1) == 0) o = T2{(R)0.5 * (Z.x + Zm.x), (R)0.5 * (Z.y - Zm.y)};  // (Z + conj Zm)/2
         else o = T2{(R)0.5 * (Z.y + Zm.y), (R)-0.5 * (Z.x - Zm.x)};           // (Z - conj Zm)/(2i)
