@@ -16,12 +16,13 @@ import threading
 from . import driver
 
 
-def run_batch(problems, steps=None, workers=4, precision=0, solvers=None):
+def run_batch(problems, steps=None, workers=4, precision=0, solvers=None, on_done=None):
     """Run `steps` accepted increments (all of them when None) of every problem.
 
     Returns the per-problem lists of row dicts, in input order.  The work queue is ordered
     largest RVE first.  `solvers`, if given, are already-created :class:`driver.Solver`
-    objects (one per problem, closed here)."""
+    objects (one per problem, closed here).  `on_done(i, solver)`, if given, is called when
+    RVE i has finished, before its context is closed (e.g. to read its fields)."""
     n = len(problems)
     results = [None] * n
     order = sorted(range(n), key=lambda i: -problems[i].nvox)
@@ -40,6 +41,8 @@ def run_batch(problems, steps=None, workers=4, precision=0, solvers=None):
             try:
                 s = solvers[i] if solvers is not None else driver.Solver(problems[i], precision=precision)
                 results[i] = s.run(max_steps=steps)
+                if on_done is not None:
+                    on_done(i, s)
             except BaseException as e:  # reported after the pool drains
                 errors.append((i, e))
             finally:

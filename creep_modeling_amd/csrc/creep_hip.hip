@@ -189,6 +189,22 @@ static int ensure_part(cg_ctx* c, size_t n) {
     return 0;
 }
 
+// The singular-system flag the kernels raise (R_FLAG_SING) as the reference's `stop 1`:
+// returns 1 with the voxel's 1-based (i, j, k) and clears the flag, or 0.
+static int singular_from(cg_ctx* c, const double* h, const char* where) {
+    if (h[R_FLAG_SING] == 0.0) return 0;
+    const long v = (long)h[R_FLAG_VOX];
+    launch_fill(c->f.red + R_FLAG_VOX, 0.0, 2, c->s);
+    return fail(c, 1, "Singular matrix in lu_inverse / lu_eqsystem (%s) at voxel (%ld,%ld,%ld)", where,
+                v % c->n1 + 1, (v / c->n1) % c->n2 + 1, v / ((long)c->n1 * c->n2) + 1);
+}
+static int check_singular(cg_ctx* c, const char* where) {
+    HIPCHK(c, hipMemcpyAsync(c->h_red + R_FLAG_VOX, c->f.red + R_FLAG_VOX, 2 * sizeof(double),
+                             hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return singular_from(c, c->h_red, where);
+}
+
 // reduce nvals partial columns into h_red[0..nvals) (synchronous)
 static int reduce_to_host(cg_ctx* c, int nblocks, int nvals, double* host_out) {
     launch_reduce_final(c->f.part, nblocks, nvals, c->f.red, c->s);
@@ -405,6 +421,7 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     rc |= dalloc(c, &f.xnode, 3 * (size_t)(f.n1 + 1) * (f.n2 + 1) * c->n3node);
     rc |= dalloc(c, &f.red, 256);
     if (rc) return rc;
+    launch_fill(f.red, 0.0, 256, c->s);  // the status slots start clear
     if (ensure_part(c, (size_t)((V + 63) / 64 + 64) * 81)) return -3;  // the solve may run 64-thread blocks
     HIPCHK(c, hipHostMalloc(&c->h_red, 256 * sizeof(double)));
     if (dalloc(c, &c->d_sgsum, 16) || dalloc(c, &c->d_fluct, 16)) return -3;
@@ -451,6 +468,7 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     double sums[36];
     if (reduce_to_host(c, nb, 36, sums)) return -2;
     hipFree(d_eul); hipFree(d_hp); hipFree(d_gr);
+    if (int e = check_singular(c, "data_grain: sg66 = cg66^-1")) return e;
     // wphsol / wphgas (IO_functions.f90:347-363) and micro-level defaults (types.f90:338-341)
     c->wphsol = 0.0; c->wphgas = 0.0;
     const double wgt = 1.0 / (double)V;
@@ -572,10 +590,11 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     launch_reduce_final_cm(f.part, nb, R_NSOLVE, f.red, c->s);
     // next pass' <sigma> = sum_all w sigma
     launch_copy(c->d_sgsum, f.red + R_SGALL, 9, c->s);
-    HIPCHK(c, hipMemcpyAsync(c->h_red, f.red, R_NSOLVE * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->h_red, f.red, R_NCOPY * sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
     HIPCHK(c, hipGetLastError());
     tcollect(c, 0); tcollect(c, 1); tcollect(c, 2);
+    if (int e = singular_from(c, c->h_red, "update_schmid / calc_gas_compl / solve_res")) return e;
     c->sgsum_valid = 1;
     c->lold_same = 1;
     const double* r = c->h_red;
@@ -645,6 +664,10 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     erre = erre / c->lvmavg;
     errs = errs / c->svmavg;
     errsbc = errsbc / c->svmavg;
+    // A diverged pass leaves NaN or Inf in the error sums.  The reference's `< 0` test
+    // (various_functions.f90:1357) lets NaN through and the loop then runs to itmax on
+    // garbage; here it ends the run with the non-finite code.
+    const bool finite = std::isfinite(erre) && std::isfinite(errs) && std::isfinite(errsbc);
     if (out) {
         out->errs = errs; out->erre = erre; out->errsbc = errsbc;
         memcpy(out->velgradavg, c->velgradavg, sizeof out->velgradavg);
@@ -658,6 +681,9 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
         out->outer = (int64_t)r[R_OUTER];
     }
     if (!ok) return fail(c, 1, "Failed system solve in lu_eqsystem (state_6x6_evpsc)");
+    if (!finite)
+        return fail(c, 2, "non-finite fixed-point errors (errs=%g erre=%g errsbc=%g) at pass %d", errs, erre, errsbc,
+                    it->iter);
     return 0;
 }
 
@@ -824,7 +850,7 @@ int cg_end_increment(cg_ctx* c, const cg_process* p, int scan_diagnostics, cg_in
     HIPCHK(c, hipStreamSynchronize(c->s));
     HIPCHK(c, hipGetLastError());
     tcollect(c, 3);
-    return 0;
+    return check_singular(c, "update_defgrad / update_elstic_defgrad / update_el_stiff / update_tensor_config");
 }
 
 int cg_calc_c0(cg_ctx* c, const cg_process* p) {
@@ -837,6 +863,7 @@ int cg_calc_c0(cg_ctx* c, const cg_process* p) {
     if (reduce_to_host(c, nb, 72, r)) return -2;
     tstop(c, 3);
     tcollect(c, 3);
+    if (int e = check_singular(c, "calc_c0")) return e;
     double dsde[36], dP[36], a66[36];
     for (int i = 0; i < 36; i++) {
         dsde[i] = r[i] / c->wphcsol;

@@ -75,6 +75,11 @@ enum {
     R_DLSOL = 50, R_WSOL = 51, R_DLGAS = 52, R_WGAS = 53,
     R_NSOLVE = 54
 };
+// Status slots of Fields::red, above every reduction's output range (at most 81 values): a
+// kernel that meets an exactly singular system (the reference's lu_inverse / lu_eqsystem
+// `stop 1`, tensor_functions.f90:323-334, 353-356) stores the voxel index and a 1.  A pass
+// copies red[0, R_NCOPY) to the host in one transfer, so the flag travels with its sums.
+enum { R_FLAG_VOX = 126, R_FLAG_SING = 127, R_NCOPY = 128 };
 
 // ---- solver_kernels.hip ----
 void launch_reduce_final(const double* part, int nblocks, int nvals, double* out, hipStream_t s);
@@ -106,6 +111,7 @@ void launch_snapshot(const Fields& f, int restore, hipStream_t s);
 void launch_init_voxels(const Fields& f, const PhaseTables& pt, const double* euler, const int* grain,
                         const double* hp_grain, const double* udot, double tdot, hipStream_t s, int* nb);
 void launch_copy(double* dst, const double* src, size_t n, hipStream_t s);
+void launch_fill(double* dst, double val, size_t n, hipStream_t s);
 void launch_init_xnode(const Fields& f, int n3node, hipStream_t s);
 void launch_xnode(const Fields& f, const double* dv, const double* lavg, double tdot, int ideriv, int n3node,
                   hipStream_t s);

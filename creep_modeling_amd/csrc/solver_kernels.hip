@@ -43,6 +43,15 @@ __device__ __forceinline__ int uidx(int i) {
 
 static inline int nblk(long V) { return (int)((V + BS - 1) / BS); }
 
+// An exactly singular per-voxel system: the reference stops with status 1 there
+// (lu_inverse / lu_eqsystem, tensor_functions.f90:323-334, 353-356).  Every thread that
+// meets one stores the same flag (plain stores, no ordering needed); the host reads it
+// with the pass' or the update's sums and returns the numerical-failure code.
+__device__ __forceinline__ void flag_singular(const Fields& f, long v) {
+    f.red[R_FLAG_VOX] = (double)v;
+    f.red[R_FLAG_SING] = 1.0;
+}
+
 // |x|^(n-1) for an integer n (exponentiation by squaring) or through pow().
 template <bool INTPOW>
 __device__ __forceinline__ double powm1(double x, double n) {
@@ -970,7 +979,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 :
                                 for (int k = 0; k < 5; k++) s += cm[6 * i + k] * dd[sym5(k, j)];
                             Jm[6 * i + j] = s;
                         }
-                    solve6(Jm, res);
+                    if (!solve6(Jm, res)) flag_singular(f, v);  // lu_eqsystem (:1287)
                     double d1 = 0.0;
                     for (int c = 0; c < 6; c++) {
                         sg[c] -= res[c];
@@ -1095,7 +1104,7 @@ __global__ __launch_bounds__(BS) void k_update_schmid(Fields f, PhaseTables pt) 
     double Fe[9], ag[9], A[9], Ai[9];
     for (int c = 0; c < 9; c++) { Fe[c] = f.Fe[c * f.ld + v]; ag[c] = f.ag[c * f.ld + v]; }
     mm3(Fe, ag, A);
-    inv3(A, Ai);
+    if (inv3(A, Ai) == 0.0) flag_singular(f, v);  // lu_inverse of A (:518-520)
     for (int c = 0; c < 9; c++) {
         f.schA[c * f.ld + v] = A[c];
         f.schA[(9 + c) * f.ld + v] = Ai[c];
@@ -1179,7 +1188,7 @@ __global__ __launch_bounds__(BS) void k_gas_compl(Fields f, double tdot, double 
         a[i] = (i % 7 == 0 ? 1.0 : 0.0) + f.c66m[(long)i * f.ld + v] / tdot * complgas;
         if (modal) a[i] -= f.cgm[(long)i * f.ld + v];
     }
-    inv6(a);
+    if (!inv6(a)) flag_singular(f, v);
     for (int i = 0; i < 36; i++) f.cgas[(long)i * f.ld + v] = a[i];
 }
 void launch_gas_compl(const Fields& f, double tdot, double complgas, int modal, hipStream_t s) {
@@ -1364,7 +1373,7 @@ __global__ __launch_bounds__(BS) void k_update_defgrad(Fields f, double tdot) {
         // determinant33 (tensor_functions.f90:442-456) term order
         const double J = F[0] * F[4] * F[8] + F[1] * F[5] * F[6] + F[2] * F[3] * F[7] - F[6] * F[4] * F[2] -
                          F[7] * F[5] * F[0] - F[8] * F[3] * F[1];
-        inv3(F, Fi);
+        if (inv3(F, Fi) == 0.0) flag_singular(f, v);
         const double wg = 1.0 / (double)V;
         for (int c = 0; c < 9; c++) {
             f.F[c * f.ld + v] = F[c];
@@ -1403,8 +1412,8 @@ __global__ __launch_bounds__(BS) void k_elastic_stiff(Fields f, PhaseTables pt) 
     const PhaseDev& P = pt.ph[f.phase[v]];
     double F[9], Fp[9], Fpi[9], Feo[9], Feoi[9], Fe[9], Fei[9];
     for (int c = 0; c < 9; c++) { F[c] = f.F[c * f.ld + v]; Fp[c] = f.Fp[c * f.ld + v]; Feo[c] = f.Fe[c * f.ld + v]; }
-    inv3(Feo, Feoi);
-    inv3(Fp, Fpi);
+    bool ok = inv3(Feo, Feoi) != 0.0;
+    ok = (inv3(Fp, Fpi) != 0.0) && ok;
     mm3(F, Fpi, Fe);
     mm3(Fe, Feoi, Fei);
     for (int c = 0; c < 9; c++) { f.Fe[c * f.ld + v] = Fe[c]; f.Feinc[c * f.ld + v] = Fei[c]; }
@@ -1415,7 +1424,8 @@ __global__ __launch_bounds__(BS) void k_elastic_stiff(Fields f, PhaseTables pt) 
     congr6(T, P.c66cr, c66);
     const double dFe = det3(Fe);
     for (int i = 0; i < 36; i++) c66[i] /= dFe;
-    inv6(c66);
+    ok = inv6(c66) && ok;
+    if (!ok) flag_singular(f, v);
     for (int i = 0; i < 36; i++) f.sg66[(long)i * f.ld + v] = c66[i];
 }
 void launch_elastic_stiff(const Fields& f, const PhaseTables& pt, hipStream_t s) {
@@ -1430,7 +1440,7 @@ __global__ __launch_bounds__(BS) void k_tensor_config_a(Fields f) {
     if (v < V) {
         double Fn[9], Fni[9], L[9];
         for (int c = 0; c < 9; c++) { Fn[c] = f.Finc[c * f.ld + v]; L[c] = f.L[c * f.ld + v]; }
-        inv3(Fn, Fni);
+        if (inv3(Fn, Fni) == 0.0) flag_singular(f, v);
         if (!f.gas[v]) {
             double R[9], lam[3], Fe[9];
             polar3(Fn, R, lam);
@@ -1623,7 +1633,7 @@ __global__ __launch_bounds__(BS) void k_calc_c0(Fields f, PhaseTables pt, double
                 if (i < 5 && j < 5) e += dd[sym5(i, j)];
                 T[6 * i + j] = e;
             }
-        inv6(T);
+        if (!inv6(T)) flag_singular(f, v);
         double Fi[9], N[36], dP[36];
         for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * f.ld + v];
         push_matrix(Fi, N);
@@ -1927,7 +1937,7 @@ __global__ __launch_bounds__(BS) void k_init_voxels(Fields f, PhaseTables pt, co
             rot_matrix(ag, T);
             congr6(T, P.c66cr, c66);
             for (int i = 0; i < 36; i++) { s66[i] = c66[i]; acc[i] = c66[i] * wg; }
-            inv6(s66);
+            if (!inv6(s66)) flag_singular(f, v);  // sg66 = cg66^-1 (data_grain)
             for (int i = 0; i < 36; i++) f.sg66[(long)i * f.ld + v] = s66[i];
             // initial_guess: sigma = cg66 (udot6 tdot)
             double u6[6], s6[6];
@@ -1969,6 +1979,14 @@ void launch_init_voxels(const Fields& f, const PhaseTables& pt, const double* eu
 }
 void launch_copy(double* dst, const double* src, size_t n, hipStream_t s) {
     hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, s);
+}
+__global__ __launch_bounds__(BS) void k_fill(double* __restrict__ dst, double val, long n) {
+    const long i = (long)blockIdx.x * BS + threadIdx.x;
+    if (i < n) dst[i] = val;
+}
+void launch_fill(double* dst, double val, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill, dim3(nblk((long)n)), dim3(BS), 0, s, dst, val, (long)n);
 }
 
 }  // namespace cg

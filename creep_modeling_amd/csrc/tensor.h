@@ -446,8 +446,11 @@ CG_HD void expm3(const double* A0, double* E) {
                           1323241920.0,        40840800.0,          960960.0,           16380.0,
                           182.0,               1.0};
     const double theta13 = 5.371920351148152;
+    // a finite norm needs at most 1022 squarings; a non-finite one (a diverged increment)
+    // would turn the count into an unbounded loop, so it is capped (the result is then
+    // non-finite, which the caller's error checks report)
     int s = 0;
-    if (n1 > theta13) s = (int)ceil(log2(n1 / theta13));
+    if (n1 > theta13) s = (int)fmin(ceil(log2(n1 / theta13)), 1100.0);
     double sc = ldexp(1.0, -s);
     double A[9], A2[9], A4[9], A6[9], U[9], V[9], T[9];
     for (int i = 0; i < 9; i++) A[i] = A0[i] * sc;

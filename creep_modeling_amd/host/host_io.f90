@@ -40,6 +40,17 @@ contains
         stop 1
     end subroutine fatal
 
+    ! A failed library call: status 1 for a singular system (the reference's `stop 1` in
+    ! lu_inverse / lu_eqsystem, tensor_functions.f90:323-334, 353-356) and for runtime
+    ! errors, status 2 for non-finite fixed-point errors.
+    subroutine fatal_rc(msg, rc)
+        character(len=*), intent(in) :: msg
+        integer, intent(in) :: rc
+        write (*, '(A)') trim(msg)
+        if (rc == 2) stop 2
+        stop 1
+    end subroutine fatal_rc
+
     subroutine load_input(fname, inp)
         character(len=*), intent(in) :: fname
         type(input_t), intent(out) :: inp

@@ -125,7 +125,7 @@ program lsevpfft_host
     rc = cg_set_voxels(ctx, inp%euler, inp%grain, inp%ph)
     if (rc /= 0) call fatal('cg_set_voxels: '//trim(last_error(ctx)))
     rc = cg_init(ctx, inp%proc(1))
-    if (rc /= 0) call fatal('cg_init: '//trim(last_error(ctx)))
+    if (rc /= 0) call fatal_rc('cg_init: '//trim(last_error(ctx)), rc)
     call write_vtk_cell(ctx, inp, 0, output_dir)
 
     call system_clock(start, cnt_rate)
@@ -164,7 +164,7 @@ program lsevpfft_host
                     end if
                 end if
                 rc = cg_begin_increment(ctx, p, merge(1, 0, (imicro == 1 .and. ipr == 1) .or. p%iupdate == 1))
-                if (rc /= 0) call fatal('cg_begin_increment: '//trim(last_error(ctx)))
+                if (rc /= 0) call fatal_rc('cg_begin_increment: '//trim(last_error(ctx)), rc)
                 did_converge = .false.
                 accept_step = .false.
                 st%iter = 0
@@ -172,7 +172,7 @@ program lsevpfft_host
                 do while (.not. (did_converge .or. accept_step))
                     call write_status(.false.)
                     rc = cg_iterate(ctx, p, st, io)
-                    if (rc /= 0) call fatal('cg_iterate: '//trim(last_error(ctx)))
+                    if (rc /= 0) call fatal_rc('cg_iterate: '//trim(last_error(ctx)), rc)
                     if (p%ictrl == 0) then
                         vs = 0.5d0 * (reshape(io%velgradavg, [3, 3]) + transpose(reshape(io%velgradavg, [3, 3])))
                         h = (vs(1, 1) + vs(2, 2) + vs(3, 3)) / 3.d0
@@ -224,7 +224,7 @@ program lsevpfft_host
                 end do
                 last_iter = st%iter
                 rc = cg_end_increment(ctx, p, 1, ic)
-                if (rc /= 0) call fatal('cg_end_increment: '//trim(last_error(ctx)))
+                if (rc /= 0) call fatal_rc('cg_end_increment: '//trim(last_error(ctx)), rc)
                 tdot_used = p%tdot
                 if (p%ictrl == -1) then
                     proc_time_done = proc_time_done + tdot_used
@@ -280,7 +280,7 @@ program lsevpfft_host
                 end if
                 if (did_converge) nconverged = nconverged + 1
                 rc = cg_calc_c0(ctx, p)
-                if (rc /= 0) call fatal('cg_calc_c0: '//trim(last_error(ctx)))
+                if (rc /= 0) call fatal_rc('cg_calc_c0: '//trim(last_error(ctx)), rc)
             end associate
             imacroloop = imacroloop + 1
         end do

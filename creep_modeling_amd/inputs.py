@@ -182,7 +182,7 @@ def voronoi_rve(n, ngrains, seed, phase=1):
     plane = np.stack([ii.ravel(), jj.ravel(), np.zeros(n1 * n2)], axis=1).astype(np.float64)
     for k in range(n3):
         plane[:, 2] = k
-        _, idx = tree.query(plane, k=1)
+        _, idx = tree.query(plane, k=1, workers=-1)  # independent queries: same result on any thread count
         grain[k * n1 * n2:(k + 1) * n1 * n2] = idx.astype(np.int32) + 1
     euler = geul[grain - 1]
     return Microstructure((n1, n2, n3), grain, np.full(V, phase, np.int32), euler, geul)

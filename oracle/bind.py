@@ -66,6 +66,14 @@ class OrcProcess(C.Structure):
     ]
 
 
+class OracleStop(RuntimeError):
+    """The restated reference stopped (status `code`, e.g. 1 for a singular system)."""
+
+    def __init__(self, code, msg):
+        super().__init__("stop %d: %s" % (code, msg))
+        self.code = code
+
+
 def build(force=False):
     if force or not LIB.exists() or LIB.stat().st_mtime < (HERE / "evpfft_oracle.c").stat().st_mtime:
         subprocess.run(["make", "-C", str(HERE), "-s"], check=True)
@@ -174,10 +182,14 @@ class Oracle:
         rc = L.orc_init(self.h)
         if rc < 0:
             raise RuntimeError("orc_init: %s" % L.orc_error(self.h).decode())
+        if rc > 0:  # the reference's `stop 1` (singular system at initialisation)
+            raise OracleStop(rc, L.orc_error(self.h).decode())
 
     def step(self):
         row = OrcRow()
         rc = self.L.orc_step(self.h, C.byref(row))
+        if rc == -7:  # singular system met during the increment (reference: stop 1)
+            raise OracleStop(1, self.L.orc_error(self.h).decode())
         if rc < 0:
             raise RuntimeError("orc_step: %s" % self.L.orc_error(self.h).decode())
         return row if rc == 1 else None

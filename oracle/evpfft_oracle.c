@@ -310,7 +310,8 @@ static void expm_pade13(const double *m, double *e) {
     double nrm = l1_norm33(m);
     if (nrm <= 0.0) { memcpy(e, ID3, sizeof A); return; }
     int s = 0;
-    if (nrm > theta13) s = (int)ceil(log(nrm / theta13) / log(2.0));
+    /* capped for a non-finite norm, which would make the squaring loop unbounded */
+    if (nrm > theta13) s = (int)fmin(ceil(log(nrm / theta13) / log(2.0)), 1100.0);
     double sc = pow(2.0, s);
     for (int i = 0; i < 9; i++) A[i] = m[i] / sc;
     mm33(A, A, A2);
@@ -485,6 +486,7 @@ orc_ctx *orc_create(int n1, int n2, int n3, int nph, double complgas, int modal,
     c->proc_cap = 8;
     c->proc = calloc(c->proc_cap, sizeof(orc_process));
     init_basis();
+    g_singular = 0; /* a new run starts clean (the reference process would have stopped) */
     return c;
 }
 
@@ -630,10 +632,12 @@ int orc_init(orc_ctx *c) {
     memcpy(c->defgradinvavgcs, ID3, sizeof ID3);
     memcpy(c->defgradinvavgcg, ID3, sizeof ID3);
 
-    /* data_grain: ag, cg66, sg66 */
+    /* data_grain: ag, cg66, sg66 (independent per voxel) */
+    for (long v = 0; v < V; v++)
+        if (c->phase[v] < 1 || c->phase[v] > c->nph) { snprintf(c->err, sizeof c->err, "phase id out of range"); return -3; }
+    #pragma omp parallel for schedule(static)
     for (long v = 0; v < V; v++) {
         int jph = c->phase[v];
-        if (jph < 1 || jph > c->nph) { snprintf(c->err, sizeof c->err, "phase id out of range"); return -3; }
         phase_t *P = &c->ph[jph - 1];
         c->gas[v] = (unsigned char)(P->igas != 0);
         if (P->igas) continue;
