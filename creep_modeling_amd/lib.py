@@ -91,6 +91,14 @@ def load():
         return _lib
     if not LIBPATH.exists():
         raise HipError("%s is missing: run __graft_entry__.build() first" % LIBPATH)
+    # PyTorch-ROCm ships its own HIP runtime under the same soname.  Loaded first, it is the
+    # one the library binds to; loaded after the library's (/opt/rocm) copy, a second runtime
+    # would enter the process and could not open the device.  One runtime per process either
+    # way, so torch tensors and the library's contexts can be used together.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(LIBPATH))
     vp = C.c_void_p
     L.cg_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,

@@ -120,6 +120,9 @@ def lib():
         L.orc_set_ipr.argtypes = [C.c_void_p, C.c_int]
         L.orc_calc_c0.argtypes = [C.c_void_p]
         L.orc_kinematic.argtypes = [C.c_void_p, C.c_double, _DBL]
+        L.orc_stage.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
+        L.orc_set_macro.argtypes = [C.c_void_p, C.c_char_p, _DBL]
+        L.orc_get_lavg.argtypes = [C.c_void_p, _DBL]
         _lib = L
     return _lib
 
@@ -215,6 +218,32 @@ class Oracle:
         a = np.ascontiguousarray(arr, np.float64)
         if self.L.orc_set_field(self.h, name.encode(), dp(a)):
             raise KeyError(name)
+
+    def stage(self, name, tdot=1.0):
+        """Run one restated procedure of the increment on the current fields."""
+        if self.L.orc_stage(self.h, name.encode(), float(tdot)):
+            raise KeyError(name)
+
+    def set_macro(self, name, arr):
+        a = np.ascontiguousarray(arr, np.float64).ravel()
+        if self.L.orc_set_macro(self.h, name.encode(), dp(a)):
+            raise KeyError(name)
+
+    def lavg(self):
+        out = np.zeros(27)
+        self.L.orc_get_lavg(self.h, dp(out))
+        return out[:9].reshape(3, 3), out[9:18].reshape(3, 3), out[18:].reshape(3, 3)
+
+    def macro(self):
+        m = self.field("macro")
+        keys = (("c0", 81), ("s0", 81), ("de_ds", 81), ("c066", 36), ("goperr0", 81), ("sgavg", 9),
+                ("velgradavg", 9), ("dvelgradavg", 9), ("defgradinvavgc", 9), ("defgradavg", 9))
+        out, o = {}, 0
+        for k, n in keys:
+            out[k] = m[o:o + n].copy()
+            o += n
+        out.update(wphcsol=m[o], wphcgas=m[o + 1], lvmavg=m[o + 2], svmavg=m[o + 3], wphsol=m[o + 7])
+        return out
 
     def close(self):
         if self.h:

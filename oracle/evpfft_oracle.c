@@ -1241,6 +1241,44 @@ static void calc_velgradavg_corr(orc_ctx *c) {
 }
 
 /* one fixed-point pass (LS-EVPFFT.f90:238-358) */
+/* correction_to_current (various_functions.f90:1062-1139): dL_fluct = dvelgradref F^-1, its
+   weighted average, dudot = (dvelgradavg - <dL_fluct>) <F^-1>^-1, L += (dvelgradref + dudot) F^-1,
+   and <L> over all / gas / solid voxels */
+static void correction_to_current(orc_ctx *c) {
+    double fl[9];
+    {
+        double red_[9] = {0};
+        PLANE_REDUCE(c, 9, {
+            double t9[9];
+            mm33(c->dvelgradref + 9 * v, c->defgradinv + 9 * v, t9);
+            for (int t = 0; t < 9; t++) acc[t] += c->wgtc[v] * t9[t];
+        });
+        memcpy(fl, red_, sizeof fl);
+    }
+    double finv[9], dud[9], dm[9];
+    memcpy(finv, c->defgradinvavgc, sizeof finv);
+    lu_inverse(finv, 3);
+    for (int t = 0; t < 9; t++) dm[t] = c->dvelgradavg[t] - fl[t];
+    mm33(dm, finv, dud);
+    {
+        double red_[27] = {0};
+        PLANE_REDUCE(c, 27, {
+            double t9[9], a9[9];
+            for (int t = 0; t < 9; t++) a9[t] = c->dvelgradref[9 * v + t] + dud[t];
+            mm33(a9, c->defgradinv + 9 * v, t9);
+            for (int t = 0; t < 9; t++) c->velgrad[9 * v + t] += t9[t];
+            for (int t = 0; t < 9; t++) acc[t] += c->velgrad[9 * v + t] * c->wgtc[v];
+            if (c->gas[v]) for (int t = 0; t < 9; t++) acc[9 + t] += c->velgrad[9 * v + t] * c->wgtc[v];
+            else for (int t = 0; t < 9; t++) acc[18 + t] += c->velgrad[9 * v + t] * c->wgtc[v];
+        });
+        for (int t = 0; t < 9; t++) {
+            c->velgradavg[t] = red_[t];
+            c->velgradavggas[t] = c->wphcgas > 0.0 ? red_[9 + t] / c->wphcgas : red_[9 + t];
+            c->velgradavgsol[t] = red_[18 + t] / c->wphcsol;
+        }
+    }
+}
+
 static int fixed_point_pass(orc_ctx *c, double *mix) {
     long V = c->V;
     /* <sigma> over all voxels and PK1 (:238-269) */
@@ -1275,39 +1313,8 @@ static int fixed_point_pass(orc_ctx *c, double *mix) {
     for (long v = 0; v < V; v++)
         for (int t = 0; t < 9; t++) c->dvelgradref[9 * v + t] = creal(c->four[(size_t)t * V + v]) / (double)V;
 
-    /* correction_to_current (various_functions.f90:1062-1139) */
-    double fl[9];
-    {
-        double red_[9] = {0};
-        PLANE_REDUCE(c, 9, {
-            double t9[9];
-            mm33(c->dvelgradref + 9 * v, c->defgradinv + 9 * v, t9);
-            for (int t = 0; t < 9; t++) acc[t] += c->wgtc[v] * t9[t];
-        });
-        memcpy(fl, red_, sizeof fl);
-    }
-    double finv[9], dud[9], dm[9];
-    memcpy(finv, c->defgradinvavgc, sizeof finv);
-    lu_inverse(finv, 3);
-    for (int t = 0; t < 9; t++) dm[t] = c->dvelgradavg[t] - fl[t];
-    mm33(dm, finv, dud);
-    {
-        double red_[27] = {0};
-        PLANE_REDUCE(c, 27, {
-            double t9[9], a9[9];
-            for (int t = 0; t < 9; t++) a9[t] = c->dvelgradref[9 * v + t] + dud[t];
-            mm33(a9, c->defgradinv + 9 * v, t9);
-            for (int t = 0; t < 9; t++) c->velgrad[9 * v + t] += t9[t];
-            for (int t = 0; t < 9; t++) acc[t] += c->velgrad[9 * v + t] * c->wgtc[v];
-            if (c->gas[v]) for (int t = 0; t < 9; t++) acc[9 + t] += c->velgrad[9 * v + t] * c->wgtc[v];
-            else for (int t = 0; t < 9; t++) acc[18 + t] += c->velgrad[9 * v + t] * c->wgtc[v];
-        });
-        for (int t = 0; t < 9; t++) {
-            c->velgradavg[t] = red_[t];
-            c->velgradavggas[t] = c->wphcgas > 0.0 ? red_[9 + t] / c->wphcgas : red_[9 + t];
-            c->velgradavgsol[t] = red_[18 + t] / c->wphcsol;
-        }
-    }
+    correction_to_current(c);
+
     /* dL_iter and mixing (LS-EVPFFT.f90:277-327) */
     double Lscale = c->lvmavg > 2.2250738585072014e-308 ? c->lvmavg : 2.2250738585072014e-308;
     double m = *mix;
@@ -2187,4 +2194,48 @@ int orc_calc_c0(orc_ctx *c) { calc_c0(c); return 0; }
  * min detF_next; returns nsub */
 int orc_kinematic(orc_ctx *c, double tdot, double *out4) {
     return kinematic_substep(c, tdot, out4, out4 + 1, out4 + 2, out4 + 3);
+}
+
+/* Run one named stage of the increment on the current fields (closed-form tests of each
+ * restated procedure): "correction_to_current", "calc_c066mod", "calc_gas_compl",
+ * "update_plastic_defgrad", "update_grid_velgrad_node", "update_defgrad",
+ * "update_elstic_defgrad", "update_el_stiff", "update_tensor_config", "update_schmid".
+ * Stages that read process data use process ipr (orc_set_ipr) and tdot. */
+int orc_stage(orc_ctx *c, const char *name, double tdot) {
+    if (c->ipr < 0) c->ipr = 0;
+    orc_process *P = &c->proc[c->ipr];
+    double save = P->tdot;
+    P->tdot = tdot;
+    int rc = 0;
+    if (!strcmp(name, "correction_to_current")) correction_to_current(c);
+    else if (!strcmp(name, "calc_c066mod")) calc_c066mod(c);
+    else if (!strcmp(name, "calc_gas_compl")) calc_gas_compl(c);
+    else if (!strcmp(name, "update_plastic_defgrad")) update_plastic_defgrad(c, tdot);
+    else if (!strcmp(name, "update_grid_velgrad_node")) update_grid_velgrad_node(c, tdot);
+    else if (!strcmp(name, "update_defgrad")) update_defgrad(c, tdot);
+    else if (!strcmp(name, "update_elstic_defgrad")) update_elstic_defgrad(c);
+    else if (!strcmp(name, "update_el_stiff")) update_el_stiff(c);
+    else if (!strcmp(name, "update_tensor_config")) update_tensor_config(c);
+    else if (!strcmp(name, "update_schmid")) update_schmid(c);
+    else rc = -1;
+    P->tdot = save;
+    return rc;
+}
+/* Set one macroscopic quantity of micro_type: "velgradavg", "dvelgradavg", "defgradinvavgc"
+ * (9 each, row-major), "Goperr0" (81) or "wph" (wphcsol, wphcgas). */
+int orc_set_macro(orc_ctx *c, const char *name, const double *v) {
+    if (!strcmp(name, "velgradavg")) memcpy(c->velgradavg, v, 72);
+    else if (!strcmp(name, "dvelgradavg")) memcpy(c->dvelgradavg, v, 72);
+    else if (!strcmp(name, "defgradinvavgc")) memcpy(c->defgradinvavgc, v, 72);
+    else if (!strcmp(name, "Goperr0")) memcpy(c->Goperr0, v, 81 * 8);
+    else if (!strcmp(name, "wph")) { c->wphcsol = v[0]; c->wphcgas = v[1]; }
+    else return -1;
+    return 0;
+}
+/* Macroscopic averages of correction_to_current: velgradavg, velgradavgsol, velgradavggas (27). */
+int orc_get_lavg(orc_ctx *c, double *out27) {
+    memcpy(out27, c->velgradavg, 72);
+    memcpy(out27 + 9, c->velgradavgsol, 72);
+    memcpy(out27 + 18, c->velgradavggas, 72);
+    return 0;
 }

@@ -113,6 +113,9 @@ int orc_set_c066(orc_ctx *c, const double *c066);
 int orc_set_ipr(orc_ctx *c, int ipr0);
 int orc_calc_c0(orc_ctx *c);
 int orc_kinematic(orc_ctx *c, double tdot, double *out4);
+int orc_stage(orc_ctx *c, const char *name, double tdot);
+int orc_set_macro(orc_ctx *c, const char *name, const double *v);
+int orc_get_lavg(orc_ctx *c, double *out27);
 
 #ifdef __cplusplus
 }
