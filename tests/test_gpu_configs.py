@@ -30,6 +30,14 @@ def test_config0_rve_ten_increments_match_oracle():
     # the pass-count criterion bites: all but the first increment converge before itmax
     assert sum(ro["passes"] < p.processes[0].itmax for ro, _ in rows) >= 9
     check_fields(fo, fg)
+    # the device's work counters (Newton steps, slip-law evaluations, hardening iterations,
+    # summed over voxels and passes: the flop model of the constitutive roofline) against the
+    # oracle's; a voxel whose Newton loop stops one step earlier or later at the tolerance
+    # changes them by one
+    for ro, rg in rows:
+        for k in ("nr_steps", "evals", "outer"):
+            print("%s oracle %d gpu %d" % (k, ro[k], rg[k]), flush=True)
+            assert abs(ro[k] - rg[k]) <= 1e-4 * ro[k], (k, ro[k], rg[k])
 
 
 def test_config1_rve_128_matches_oracle():
