@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -214,7 +215,63 @@ static int reduce_to_host(cg_ctx* c, int nblocks, int nvals, double* host_out) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Device scratch of the no-allocation entry point (cg_kinematic_substep_noalloc): the
+// block partials of its reductions, 4 per 256-voxel block, for grids of at most
+// NOALLOC_MAX_VOX voxels (256^3): 2 MiB on every GPU the library runs on, held until the
+// process ends.  Host code reaches it only through the per-device address that
+// hipGetSymbolAddress returns (the array's name is a host-side stand-in).  Every call on a
+// GPU shares that GPU's array: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr long NOALLOC_MAX_VOX = 1L << 24;
+constexpr long NOALLOC_PART = 4 * (NOALLOC_MAX_VOX / 256);
+static __device__ __align__(256) double scratch[NOALLOC_PART];
+constexpr int NOALLOC_MAX_DEV = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[NOALLOC_MAX_DEV];
+static thread_local char g_noctx_err[256] = "";
+
+static int noctx_fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_noctx_err, sizeof g_noctx_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+// The current device's copy of `scratch`, resolved once per device with that device current.
+static int scratch_on_current_device(double** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEV)
+        return noctx_fail(-4, "no usable current device (%d)", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return noctx_fail(-2, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 extern "C" {
+
+int cg_kinematic_substep_noalloc(long nvox, const double* velgrad, long ld, const double* detF, double tdot,
+                                 double* out5, void* stream) {
+    if (nvox < 1 || ld < nvox || !velgrad || !detF || !out5)
+        return noctx_fail(-1, "cg_kinematic_substep_noalloc: bad arguments (nvox=%ld, ld=%ld)", nvox, ld);
+    if (nvox > NOALLOC_MAX_VOX)
+        return noctx_fail(-6, "cg_kinematic_substep_noalloc: %ld voxels exceed the scratch capacity of %ld voxels",
+                          nvox, NOALLOC_MAX_VOX);
+    double* sp = nullptr;
+    if (int e = scratch_on_current_device(&sp)) return e;
+    launch_kinematic_raw(velgrad, ld, detF, nvox, tdot, sp, out5, (hipStream_t)stream);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return noctx_fail(-2, "cg_kinematic_substep_noalloc: %s", hipGetErrorString(e));
+    return 0;
+}
 
 int cg_create(int n1, int n2, int n3, int nph, double complgas, int modal, int do_mix, int precision,
               cg_ctx** out) {
@@ -271,7 +328,7 @@ int cg_destroy(cg_ctx* c) {
     return 0;
 }
 
-const char* cg_last_error(cg_ctx* c) { return c ? c->err : "null context"; }
+const char* cg_last_error(cg_ctx* c) { return c ? c->err : g_noctx_err; }
 
 int cg_set_phase(cg_ctx* c, int iph, int igas, int nsyst, int nmodes, const int* nsm, const double* dnca,
                  const double* dbca, const double* tau, const double* thet, const double* nrs,
@@ -704,13 +761,13 @@ int cg_kinematic_substep(cg_ctx* c, double tdot, double* out4, int* nsub) {
     int nb;
     if (ensure_part(c, (size_t)((c->V + 63) / 64 + 64) * 81)) return -3;
     launch_kinematic(c->f, tdot, c->s, &nb);
-    double r[4];
-    HIPCHK(c, hipMemcpyAsync(c->h_red, c->f.red, 4 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    double r[5];
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->f.red, 5 * sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
     memcpy(r, c->h_red, sizeof r);
-    if (out4) memcpy(out4, r, sizeof r);
-    double worst = std::max(1.0, std::max(r[0] / 5.0e-2, std::max(r[1] / 2.0, r[2] / 5.0e-2)));
-    if (nsub) *nsub = std::max(1, (int)ceil(worst));
+    if (out4) memcpy(out4, r, 4 * sizeof(double));
+    // r[4] = max(1, ceiling(worst ratio)), formed on the device; a non-finite ratio gives 1
+    if (nsub) *nsub = (r[4] >= 1.0 && r[4] < 2.0e9) ? (int)r[4] : 1;
     return 0;
 }
 

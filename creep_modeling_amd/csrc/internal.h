@@ -107,6 +107,11 @@ void launch_diagnostics(const Fields& f, double lfloor, int scan_cond, hipStream
 void launch_calc_c0(const Fields& f, const PhaseTables& pt, double tdot, double tdotref, hipStream_t s, int* nb,
                     int int_pow);
 void launch_kinematic(const Fields& f, double tdot, hipStream_t s, int* nb);
+// the same kernels on raw device pointers; part holds kinematic_scratch_doubles(V) partials and
+// red receives 5 values (max volinc, rotang, stretchinc, min detF_next, nsub)
+void launch_kinematic_raw(const double* L, long ld, const double* J, long V, double tdot, double* part, double* red,
+                          hipStream_t s);
+long kinematic_scratch_doubles(long V);
 void launch_snapshot(const Fields& f, int restore, hipStream_t s);
 void launch_init_voxels(const Fields& f, const PhaseTables& pt, const double* euler, const int* grain,
                         const double* hp_grain, const double* udot, double tdot, hipStream_t s, int* nb);

@@ -1673,31 +1673,47 @@ __device__ __forceinline__ void kin_voxel(const double* Finc, double* vol, doubl
     }
     *str = sqrt(s);
 }
-__global__ __launch_bounds__(BS) void k_kinematic(Fields f, double tdot) {
-    const long V = f.V;
+// L: 9 columns of stride ld; J: V values; part: 4 nblk(V) block partials; red: 5 results
+// (max volinc, max rotang, max stretchinc, min detF_next, nsub)
+__global__ __launch_bounds__(BS) void k_kinematic(const double* __restrict__ L, long ld, const double* __restrict__ J,
+                                                  long V, double tdot, double* __restrict__ part) {
     const long v = (long)blockIdx.x * BS + threadIdx.x;
     double mx[3] = {0.0, 0.0, 0.0}, mn[1] = {1.7976931348623157e308};
     if (v < V) {
         double d[9], E[9];
-        for (int c = 0; c < 9; c++) d[c] = f.L[c * f.ld + v] * tdot;
+        for (int c = 0; c < 9; c++) d[c] = L[c * ld + v] * tdot;
         expm3(d, E);
         const double Jinc = E[0] * E[4] * E[8] + E[1] * E[5] * E[6] + E[2] * E[3] * E[7] - E[6] * E[4] * E[2] -
                             E[7] * E[5] * E[0] - E[8] * E[3] * E[1];
         kin_voxel(E, &mx[0], &mx[1], &mx[2]);
-        mn[0] = f.J[v] * Jinc;
+        mn[0] = J[v] * Jinc;
     }
     const int nb = gridDim.x;
-    block_reduce_store<3, BS, OpMax>(mx, f.part);
+    block_reduce_store<3, BS, OpMax>(mx, part);
     __syncthreads();
-    block_reduce_store<1, BS, OpMin>(mn, f.part + 3 * (size_t)nb);
+    block_reduce_store<1, BS, OpMin>(mn, part + 3 * (size_t)nb);
+}
+// kinematic_substep (various_functions.f90:2759-2760): nsub = ceiling of the largest ratio of
+// an increment to its limit (5e-2 volume, 2 degrees, 5e-2 stretch), at least 1
+__global__ void k_kinematic_nsub(double* red) {
+    if (threadIdx.x == 0) {
+        const double worst = fmax(1.0, fmax(red[0] / 5.0e-2, fmax(red[1] / 2.0, red[2] / 5.0e-2)));
+        red[4] = fmax(1.0, ceil(worst));
+    }
+}
+void launch_kinematic_raw(const double* L, long ld, const double* J, long V, double tdot, double* part, double* red,
+                          hipStream_t s) {
+    const int nb = nblk(V);
+    hipLaunchKernelGGL(k_kinematic, dim3(nb), dim3(BS), 0, s, L, ld, J, V, tdot, part);
+    hipLaunchKernelGGL(k_reduce_final_op<OpMax>, dim3(3), dim3(BS), 0, s, part, nb, 3, red);
+    hipLaunchKernelGGL(k_reduce_final_op<OpMin>, dim3(1), dim3(BS), 0, s, part + 3 * (size_t)nb, nb, 1, red + 3);
+    hipLaunchKernelGGL(k_kinematic_nsub, dim3(1), dim3(64), 0, s, red);
 }
 void launch_kinematic(const Fields& f, double tdot, hipStream_t s, int* nb) {
     *nb = nblk(f.V);
-    hipLaunchKernelGGL(k_kinematic, dim3(*nb), dim3(BS), 0, s, f, tdot);
-    hipLaunchKernelGGL(k_reduce_final_op<OpMax>, dim3(3), dim3(BS), 0, s, f.part, *nb, 3, f.red);
-    hipLaunchKernelGGL(k_reduce_final_op<OpMin>, dim3(1), dim3(BS), 0, s, f.part + 3 * (size_t)(*nb), *nb, 1,
-                       f.red + 3);
+    launch_kinematic_raw(f.L, f.ld, f.J, f.V, tdot, f.part, f.red, s);
 }
+long kinematic_scratch_doubles(long V) { return 4L * nblk(V); }
 
 // ---------------------------------------------------------------------------
 // update_diagnostics (types.f90:345-758)

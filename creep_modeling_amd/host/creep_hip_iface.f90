@@ -102,6 +102,14 @@ module creep_hip_iface
             real(c_double) :: out4(4)
             integer(c_int) :: nsub
         end function
+        ! device pointers in and out, nothing allocated or synchronised (callers that cannot allocate)
+        integer(c_int) function cg_kinematic_substep_noalloc(nvox, velgrad, ld, detf, tdot, out5, stream) &
+            bind(C, name='cg_kinematic_substep_noalloc')
+            import :: c_int, c_long, c_double, c_ptr
+            integer(c_long), value :: nvox, ld
+            type(c_ptr), value :: velgrad, detf, out5, stream
+            real(c_double), value :: tdot
+        end function
         integer(c_int) function cg_end_increment(ctx, p, scan, out) bind(C, name='cg_end_increment')
             import :: c_int, c_ptr, cg_process, cg_incr_out
             type(c_ptr), value :: ctx

@@ -21,7 +21,7 @@ EXPORTS = (
     "cg_create", "cg_destroy", "cg_last_error", "cg_set_phase", "cg_set_voxels", "cg_init",
     "cg_begin_increment", "cg_iterate", "cg_rollback", "cg_kinematic_substep", "cg_end_increment",
     "cg_calc_c0", "cg_field_ncomp", "cg_get_field", "cg_set_field", "cg_get_macro", "cg_get_stats",
-    "cg_set_timing",
+    "cg_set_timing", "cg_kinematic_substep_noalloc",
 )
 
 
@@ -121,6 +121,7 @@ def load():
     L.cg_get_macro.argtypes = [vp, _D, _D, _D, _D, _D]
     L.cg_get_stats.argtypes = [vp, C.POINTER(CgStats), C.c_int]
     L.cg_set_timing.argtypes = [vp, C.c_int]
+    L.cg_kinematic_substep_noalloc.argtypes = [C.c_long, vp, C.c_long, vp, C.c_double, vp, vp]
     _lib = L
     return L
 

@@ -120,6 +120,19 @@ int cg_rollback(cg_ctx *ctx, cg_process *p);
 /* kinematic_increments + kinematic_substep (various_functions.f90:2745-2837):
  * out[4] = max volinc, max rotang (deg), max stretchinc, min detF_next; returns nsub in *nsub. */
 int cg_kinematic_substep(cg_ctx *ctx, double tdot, double *out4, int *nsub);
+/* The kinematic check of cg_kinematic_substep without a context and without allocating: no
+ * counterpart in the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Inputs and outputs are
+ * device pointers: velgrad = 9 columns of stride ld (row-major 3x3 components, column c at
+ * velgrad + c*ld), detF = nvox values; out5 (device) receives max volinc, max rotang (deg),
+ * max stretchinc, min detF_next and nsub (as a double), exactly as cg_kinematic_substep
+ * returns them for the same fields.  Work is only enqueued on `stream` (a hipStream_t, NULL
+ * for the default stream): nothing is synchronised, allocated or freed.  The block partials
+ * live in a fixed device array of 2 MiB per GPU, so nvox <= 2^24 (256^3); larger grids return
+ * -6.  Calls that overlap on one GPU share that array and race.  Errors: -1 arguments, -2 HIP,
+ * -4 no device, -6 too large; cg_last_error(NULL) gives the message (per host thread). */
+int cg_kinematic_substep_noalloc(long nvox, const double *velgrad, long ld, const double *detF, double tdot,
+                                 double *out5, void *stream);
 /* Accepted-increment updates (LS-EVPFFT.f90:431-523 and types.f90:345-758):
  * accumulation, update_plastic_defgrad, update_grid_velgrad_node, update_defgrad,
  * update_elstic_defgrad, update_el_stiff, update_tensor_config (iupdate=1),
