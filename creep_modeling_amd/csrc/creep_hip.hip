@@ -1029,6 +1029,13 @@ int cg_set_field(cg_ctx* c, const char* name, const double* in, size_t n) {
         launch_copy(c->f.Lold, c->f.L, 9 * c->f.ld, c->s);
         c->lold_same = 0;
     }
+    // The symmetric solve of a context without the modified-AL term rebuilds the Schmid tensors
+    // from A = Fe ag (schA, written by update_schmid) and never reads sch: a host write to sch
+    // would be silently ignored there, so it is refused.  Writes to defgrade and ag reach the
+    // Schmid tensors at the next update_schmid (cg_begin_increment with do_schmid), as in the
+    // reference, where sch changes only in update_schmid.
+    if (d.p == c->f.sch && !c->modal)
+        return fail(c, -1, "sch is rebuilt from defgrade and ag by update_schmid in this context; write those instead");
     if (c->sym_tau && (d.p == c->f.crss || d.p == c->f.ttau)) {
         // the kernels of a context whose forward and backward strengths are equal read one
         // column of each pair

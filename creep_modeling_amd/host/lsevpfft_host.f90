@@ -21,7 +21,7 @@ program lsevpfft_host
     type(cg_incr_out) :: ic
     integer(c_int) :: rc, nsub_c
     integer :: i, iph, ipr, imicro, imacroloop, nconverged, total_macrosteps, nsteps_left, nsteps_total_new
-    integer :: nthreads, tmp_int, precision, nsub, last_iter
+    integer :: nthreads, tmp_int, precision, nsub, last_iter, npass
     integer(8) :: nv, start, cnt_rate, now
     logical :: skip_next, modal, do_mix, did_converge, accept_step
     character(len=256) :: arg, fft_input_file, output_dir, fftw_save_path
@@ -169,10 +169,17 @@ program lsevpfft_host
                 accept_step = .false.
                 st%iter = 0
                 st%mix = 1.d0
+                npass = 0
                 do while (.not. (did_converge .or. accept_step))
                     call write_status(.false.)
                     rc = cg_iterate(ctx, p, st, io)
                     if (rc /= 0) call fatal_rc('cg_iterate: '//trim(last_error(ctx)), rc)
+                    ! The reference's loop has no exit but convergence when ictrl /= -1 or
+                    ! tdot > tdotmin (LS-EVPFFT.f90:361-429): an increment that never converges
+                    ! would run forever; it is ended after 100 itmax + 1000 passes.
+                    npass = npass + 1
+                    if (npass > 100 * max(p%itmax, 1) + 1000) &
+                        call fatal('fixed-point loop did not terminate (no convergence in 100 itmax + 1000 passes)')
                     if (p%ictrl == 0) then
                         vs = 0.5d0 * (reshape(io%velgradavg, [3, 3]) + transpose(reshape(io%velgradavg, [3, 3])))
                         h = (vs(1, 1) + vs(2, 2) + vs(3, 3)) / 3.d0

@@ -138,6 +138,10 @@ class CreepProcess:
     iwstep: int = -1
     xc0: float = 2.0
     igamma: int = 0
+    #: "creep": sigma33 imposed, the other stresses zero (the creep template,
+    #: ``utils/sim_cases/creep_calibration.py``); "tension": the velocity gradient L33 = udot33
+    #: imposed with sigma33 free (the tension template, ``utils/sim_cases/tension.py:33-47``)
+    control: str = "creep"
 
 
 # ---------------------------------------------------------------------------
@@ -264,12 +268,13 @@ def write_fft_in(path, n, microstructure_path, cupl2_path, cuel_path, processes,
         "%d                      number of processes (nproc)" % len(processes),
     ]
     for p in processes:
+        tension = p.control == "tension"
         lines += [
             "*INFORMATION ABOUT TEST CONDITIONS",
             "* boundary conditions",
             "    0       1       1           iudot",
             "    1       0       1",
-            "    1       1       0",
+            "    1       1       %d" % (1 if tension else 0),
             "",
             "    0.0     0.0      0.0          udot",
             "    0.0     0.0      0.0",
@@ -277,7 +282,7 @@ def write_fft_in(path, n, microstructure_path, cupl2_path, cuel_path, processes,
             "",
             "    1       0        0           iscau",
             "            1        0",
-            "                     1",
+            "                     %d" % (0 if tension else 1),
             "",
             "    %r      0.       0.          scauchy" % float(p.s11),
             "            %r       0." % float(p.s22),
