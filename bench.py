@@ -27,21 +27,26 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by what the
-# same counters read for a 1 GiB device copy); the file records the commands and raw counts.
-TRAFFIC_FILE = ROOT / "profiles" / "traffic_fcc128.json"
+# Hardware counters of the timed region of `bench.py --steps 20 --warmup 5` (rocprofv3 --pmc runs
+# of the FP64 form; per-pass kernels averaged over their last dispatches, which are the timed
+# increments'): HBM bytes per launch from FETCH_SIZE / WRITE_SIZE (scaled by what the same
+# counters read for a 1 GiB device copy) and FP64 instruction counts; the file records the
+# commands and raw counts.
+COUNTER_FILE = ROOT / "profiles" / "counters_fcc128_timed.json"
 
 
-def measured_traffic(stage, config, n, grains):
-    """HBM bytes per launch of `stage` for this workload from the committed counter runs, or None."""
+def measured(stage, key, args):
+    """A counter-derived value of `stage` for this workload and timed region, or None."""
     try:
-        d = json.loads(TRAFFIC_FILE.read_text())
+        d = json.loads(COUNTER_FILE.read_text())
     except (OSError, ValueError):
         return None
     e = d.get("stages", {}).get(stage)
-    if not e or d.get("config") != config or d.get("n") != n or d.get("grains") != grains:
+    w = d.get("workload", {})
+    if not e or (w.get("config"), w.get("n"), w.get("grains"), w.get("steps"), w.get("warmup")) != \
+            (args.config, args.n, args.grains, args.steps, args.warmup):
         return None
-    return e["bytes_per_launch"]
+    return e.get(key)
 
 
 # algorithmic bytes per voxel-iteration of the FFT + Green stage: read the 9-component
@@ -89,15 +94,41 @@ def run_gpu(problem, precision, steps, warmup):
     return rows, st, t1 - t0
 
 
-def cpu_baseline(problem, max_passes=3):
-    """The C/OpenMP oracle on the first increment of the same RVE, capped at max_passes passes."""
+def host_cpu():
+    """The host's processor as lscpu reports it (model, sockets, cores per socket, threads per core)."""
+    info = {}
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except Exception:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    return {"model": info.get("Model name"), "sockets": info.get("Socket(s)"),
+            "cores_per_socket": info.get("Core(s) per socket"), "threads_per_core": info.get("Thread(s) per core"),
+            "cpus_online": info.get("CPU(s)"), "cpus_usable": affinity}
+
+
+def omp_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def cpu_baseline(problem, max_passes):
+    """The C/OpenMP oracle (a port of the reference loop) on the first increment of the same RVE,
+    capped at max_passes fixed-point passes (accepted at itmax)."""
     from oracle import bind
 
     p = problem.processes[0]
     saved = (p.itmax, p.tdotmin)
     p.itmax = max_passes
     p.tdotmin = p.tdot  # accept at itmax
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = omp_threads()
     o = bind.Oracle(problem, nthreads=threads)
     t0 = time.perf_counter()
     r = o.step()
@@ -105,8 +136,68 @@ def cpu_baseline(problem, max_passes=3):
     o.close()
     p.itmax, p.tdotmin = saved
     return dict(value=r.passes * problem.nvox / (t1 - t0), unit="voxel-iters/s", cores=threads, kind="port",
-                sample="first increment of the %dx%dx%d RVE capped at %d fixed-point passes (incl. per-increment "
-                       "setup and updates), %.1f s" % (*problem.n, r.passes, t1 - t0))
+                sample="first increment of the %dx%dx%d RVE capped at %d fixed-point passes (incl. the increment's "
+                       "setup and accepted-step updates), %.1f s on %d OpenMP threads" % (*problem.n, r.passes, t1 - t0,
+                                                                                      threads),
+                host=host_cpu())
+
+
+def cpu_config0(args):
+    """The CPU baseline plan's full run: the oracle on the 32^3 / 64-grain FCC creep RVE, all 50
+    increments of 1 s (sigma33 = 300 MPa), on OMP_NUM_THREADS threads."""
+    from creep_modeling_amd import inputs, parse
+    from oracle import bind
+
+    ms, mat, procs = inputs.config_case(0)
+    problem = parse.problem_from_case(ms, mat, procs)
+    threads = omp_threads()
+    o = bind.Oracle(problem, nthreads=threads)
+    t0 = time.perf_counter()
+    passes = incr = 0
+    while True:
+        r = o.step()
+        if r is None:
+            break
+        passes += r.passes
+        incr += 1
+        print("increment %d: %d passes, %.1f s" % (incr, r.passes, time.perf_counter() - t0), file=sys.stderr,
+              flush=True)
+    T = time.perf_counter() - t0
+    o.close()
+    print(json.dumps({"metric": "voxel-iters/s, CPU oracle (C/OpenMP port of the reference loop), 32^3 / 64 grains "
+                                "FCC creep, 50 increments", "value": passes * problem.nvox / T, "unit": "voxel-iters/s",
+                      "rve_increments_per_s": incr / T, "passes": passes, "increments": incr, "seconds": T,
+                      "threads": threads, "host": host_cpu()}))
+
+
+def checksums():
+    """sha256 of the microstructure files (the reference's .dat text) of every workload."""
+    import hashlib
+
+    from creep_modeling_amd import inputs
+
+    def digest(ms):
+        n1, n2, n3 = ms.n
+        h = hashlib.sha256()
+        V = n1 * n2 * n3
+        for k0 in range(0, n3):  # one z-plane at a time
+            lines = []
+            for v in range(k0 * n1 * n2, (k0 + 1) * n1 * n2):
+                e = ms.euler[v]
+                lines.append("%.6f %.6f %.6f %d %d %d %d %d\n" % (e[0], e[1], e[2], v % n1 + 1, (v // n1) % n2 + 1,
+                                                                   v // (n1 * n2) + 1, ms.grain[v], ms.phase[v]))
+            h.update("".join(lines).encode())
+        assert V > 0
+        return h.hexdigest()
+
+    out = {}
+    for cfg, kw in ((0, {}), (1, {}), (2, {"n": 256, "ngrains": 4096}), (3, {})):
+        ms, _, _ = inputs.config_case(cfg, **kw)
+        out["config%d %dx%dx%d/%d grains" % (cfg, *ms.n, int(ms.grain.max()))] = digest(ms)
+        print(cfg, file=sys.stderr, flush=True)
+    sweep = [digest(c[0]) for _, c in inputs.sweep_cases(64, nsteps=1)]
+    out["config4 64 RVEs (sha256 of the 64 digests in order)"] = hashlib.sha256("".join(sweep).encode()).hexdigest()
+    print(json.dumps(out, indent=1))
 
 
 def bench_sweep(args):
@@ -170,8 +261,8 @@ def copy_calibration(gib=1.0, reps=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3, 4),
                     help="1: FCC n=10 (the headline workload); 2: FCC n=15; 3: BCC 48 systems, biaxial, dt=0.5 s; "
                          "4: batched sweep of independent FCC RVEs (32^3-96^3)")
@@ -182,12 +273,22 @@ def main():
     ap.add_argument("--s33", type=float, default=None, help="creep stress sigma33 in MPa (config default 300)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--cpu-passes", type=int, default=2)
+    ap.add_argument("--cpu-passes", type=int, default=8,
+                    help="fixed-point passes of the CPU oracle's bounded sample (first increment)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (counter calibration) and exit")
+    ap.add_argument("--cpu-config0", action="store_true",
+                    help="run only the CPU oracle on the 32^3 / 64-grain RVE, all 50 increments, and exit")
+    ap.add_argument("--checksums", action="store_true", help="print the sha256 of every workload's microstructure")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.cpu_config0:
+        cpu_config0(args)
+        return
+    if args.checksums:
+        checksums()
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the multi-GPU layout (independent RVEs per GPU, or z-slab "
@@ -222,12 +323,19 @@ def main():
     solve_tflops = flops / (st.t_ms[1] * 1e-3) / 1e12 if st.t_ms[1] > 0 else 0.0
     dominant_solve = t_solve >= t_fft
     if dominant_solve:
-        roof = dict(bound="mfma", kernel="k_correct_solve (constitutive Newton, FP64 vector ALU)",
+        # the constitutive kernel runs on the FP64 vector ALU (no matrix-core instruction): its peak is
+        # the FP64 vector rate; achieved = the flop model (counted Newton steps, evaluations and
+        # hardening calls) / the kernel's average HIP-event duration
+        roof = dict(bound="fp64_valu", kernel="k_correct_solve (constitutive Newton, FP64 vector ALU)",
                     achieved=solve_tflops, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=solve_tflops / FP64_PEAK_TFLOPS,
-                    traffic=measured_traffic("k_correct_solve", args.config, args.n, args.grains), avg_ms=t_solve)
+                    traffic=measured("k_correct_solve", "bytes_per_launch", args), avg_ms=t_solve)
+        cf = measured("k_correct_solve", "fp64_flops_per_launch", args)
+        # FP64 instructions counted by the hardware (64 lanes per issued instruction, FMA = 2) over
+        # the same launches, against the live average duration
+        roof["counter_fp64_tflops"] = cf / (t_solve * 1e-3) / 1e12 if cf else None
     else:
         roof = dict(bound="hbm", kernel="FFT + Green stage", achieved=fft_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=fft_gbs / HBM_PEAK_GBS, traffic=measured_traffic("fft_green", args.config, args.n, args.grains),
+                    frac=fft_gbs / HBM_PEAK_GBS, traffic=measured("fft_green", "bytes_per_launch", args),
                     avg_ms=t_fft)
     out = {
         "metric": "voxel-iters/s at 128^3 creep RVE (mixed vs FP64 GPU) + % HBM roofline",
@@ -254,7 +362,7 @@ def main():
                                "per_increment_updates_ms": st.t_ms[3] / max(args.steps, 1)},
         "fft_green_roofline": {"bound": "hbm", "achieved": fft_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": fft_gbs / HBM_PEAK_GBS, "alg_bytes_per_voxel": fft_alg_bytes(args.n),
-                               "traffic": measured_traffic("fft_green", args.config, args.n, args.grains)},
+                               "traffic": measured("fft_green", "bytes_per_launch", args)},
         "constitutive": {"tflops": solve_tflops, "frac_fp64_peak": solve_tflops / FP64_PEAK_TFLOPS,
                          "nr_steps_per_voxel_iter": st.nr_steps / max(st.voxel_iters, 1),
                          "evals_per_voxel_iter": st.evals / max(st.voxel_iters, 1),

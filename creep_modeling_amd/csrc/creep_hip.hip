@@ -494,28 +494,21 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
             if (c->grain[v] < 1) return fail(c, -1, "grain id out of bounds for Hall-Petch init");
             nvox[(size_t)c->grain[v] * c->nph + c->phase[v] - 1]++;
         }
-    // Hall-Petch shift per grain id: 252.8/sqrt(max((64 N)^(1/3), 12)) (IO_functions.f90:107-169);
-    // a grain spread over several phases uses its per-phase size, so tabulate per (grain, phase)
-    if (c->nph > 1) {
-        for (int g = 1; g <= gmax; g++) {
-            int nonzero = 0;
-            for (int p = 0; p < c->nph; p++) nonzero += nvox[(size_t)g * c->nph + p] > 0;
-            if (nonzero > 1) return fail(c, -1, "grain %d spans several phases (unsupported)", g);
-        }
-    }
-    std::vector<double> hpg(gmax + 1, 0.0);
+    // Hall-Petch shift 252.8/sqrt(max((64 N)^(1/3), 12)) with N the voxel count of the voxel's
+    // (grain, phase) pair (IO_functions.f90:107-169: a grain id spread over several phases has
+    // one size per phase), tabulated per voxel
+    std::vector<double> hpg((size_t)(gmax + 1) * c->nph, 0.0), hpv(V, 0.0);
     const double hpk = 632.0 / 2.5, vvol = 64.0, dmin = 3.0 * pow(vvol, 1.0 / 3.0);
-    for (int g = 1; g <= gmax; g++) {
-        long n = 0;
-        for (int p = 0; p < c->nph; p++) n += nvox[(size_t)g * c->nph + p];
-        if (n > 0) hpg[g] = hpk / sqrt(std::max(pow((double)n * vvol, 1.0 / 3.0), dmin));
-    }
+    for (size_t q = 0; q < hpg.size(); q++)
+        if (nvox[q] > 0) hpg[q] = hpk / sqrt(std::max(pow((double)nvox[q] * vvol, 1.0 / 3.0), dmin));
+    for (long v = 0; v < V; v++)
+        if (!hgas[v]) hpv[v] = hpg[(size_t)c->grain[v] * c->nph + c->phase[v] - 1];
     double *d_eul = nullptr, *d_hp = nullptr;
     int* d_gr = nullptr;
-    if (dalloc(c, &d_eul, 3 * V) || dalloc(c, &d_hp, gmax + 1)) return -3;
+    if (dalloc(c, &d_eul, 3 * V) || dalloc(c, &d_hp, V)) return -3;
     HIPCHK(c, hipMalloc(&d_gr, V * sizeof(int)));
     HIPCHK(c, hipMemcpy(d_eul, c->euler.data(), 3 * V * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(d_hp, hpg.data(), (gmax + 1) * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d_hp, hpv.data(), V * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d_gr, c->grain.data(), V * sizeof(int), hipMemcpyHostToDevice));
     PhaseTables pt{c->d_ph, c->d_tab};
     int nb;

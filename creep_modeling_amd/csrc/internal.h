@@ -114,7 +114,7 @@ void launch_kinematic_raw(const double* L, long ld, const double* J, long V, dou
 long kinematic_scratch_doubles(long V);
 void launch_snapshot(const Fields& f, int restore, hipStream_t s);
 void launch_init_voxels(const Fields& f, const PhaseTables& pt, const double* euler, const int* grain,
-                        const double* hp_grain, const double* udot, double tdot, hipStream_t s, int* nb);
+                        const double* hp_voxel, const double* udot, double tdot, hipStream_t s, int* nb);
 void launch_copy(double* dst, const double* src, size_t n, hipStream_t s);
 void launch_fill(double* dst, double val, size_t n, hipStream_t s);
 void launch_init_xnode(const Fields& f, int n3node, hipStream_t s);

@@ -1968,7 +1968,7 @@ __global__ __launch_bounds__(BS) void k_init_voxels(Fields f, PhaseTables pt, co
             const double* tau = pt.data + P.off_tau;
             const double* hpf = pt.data + P.off_hpfac;
             const double* db = pt.data + P.off_dbca;
-            const double hps = hp[grain[v]];
+            const double hps = hp[v];  // Hall-Petch shift of this voxel's (grain, phase)
             for (int is = 0; is < P.nsyst; is++) {
                 const double by = ag[3] * db[3 * is] + ag[4] * db[3 * is + 1] + ag[5] * db[3 * is + 2];
                 const double dd = hpf[is] * sqrt(fabs(by));
@@ -1986,12 +1986,12 @@ __global__ __launch_bounds__(BS) void k_init_voxels(Fields f, PhaseTables pt, co
     block_reduce_store<36, BS>(acc, f.part);
 }
 void launch_init_voxels(const Fields& f, const PhaseTables& pt, const double* euler, const int* grain,
-                        const double* hp_grain, const double* udot, double tdot, hipStream_t s, int* nb) {
+                        const double* hp_voxel, const double* udot, double tdot, hipStream_t s, int* nb) {
     InitArgs a;
     for (int i = 0; i < 9; i++) a.udot[i] = udot[i];
     a.tdot = tdot;
     *nb = nblk(f.V);
-    hipLaunchKernelGGL(k_init_voxels, dim3(*nb), dim3(BS), 0, s, f, pt, euler, grain, hp_grain, a);
+    hipLaunchKernelGGL(k_init_voxels, dim3(*nb), dim3(BS), 0, s, f, pt, euler, grain, hp_voxel, a);
 }
 void launch_copy(double* dst, const double* src, size_t n, hipStream_t s) {
     hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, s);

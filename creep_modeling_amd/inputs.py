@@ -371,6 +371,8 @@ def config_case(config, n=None, ngrains=None, nsteps=None, s33=None, nrs=None):
     Keyword overrides shrink a workload for tests.
     """
     seed = 1000 + config
+    if config == 2 and n in (None, 128) and ngrains in (None, 512):
+        seed = 1001  # the 128^3 input of the n = 15 workload is the 128^3 / 512-grain RVE of config 1
     if config in (0, 1, 2):
         dn, dg, ds = {0: (32, 64, 50), 1: (128, 512, 200), 2: (128, 512, 200)}[config]
         mat = fcc_material(n=15.0 if config == 2 else 10.0)
