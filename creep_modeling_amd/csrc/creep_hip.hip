@@ -305,7 +305,7 @@ static void free_fields(cg_ctx* c) {
     Fields& f = c->f;
     void* ptrs[] = {f.phase, f.gas, f.w, f.sg, f.sgt, f.L, f.Lold, f.dLref, f.F, f.Fi, f.J, f.Fe, f.Fp, f.ag,
                     f.sg66, f.c66m, f.cgm, f.cgas, f.sop, f.sch, f.schA, f.crss, f.ttau, f.taun, f.gdot, f.gacc, f.edotp, f.ept, f.dgs,
-                    f.epvm, f.Finc, f.Feinc, f.s_L, f.s_Lold, f.s_sg, f.s_edotp, f.s_gdot, f.s_ttau, f.spec,
+                    f.epvm, f.Finc, f.Feinc, f.eth, f.s_L, f.s_Lold, f.s_sg, f.s_edotp, f.s_gdot, f.s_ttau, f.spec,
                     f.specf, f.xnode, f.part, f.red};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -622,6 +622,9 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     pc.mix = it->mix; pc.lscale = std::max(c->lvmavg, 2.2250738585072014e-308);
     pc.modal = c->modal; pc.do_mix = c->do_mix; pc.kin_ctrl = (p->iscau[2] == 0);
     pc.lold_same = c->lold_same; pc.iuphard = p->iuphard;
+    // ithermo = 1: the eigenstrain rate acts in the first increment of the process only
+    // (various_functions.f90:1226: imicro == 1 .and. ithermo == 1)
+    pc.thermo = (f.eth != nullptr && p->ithermo == 1 && it->imicro == 1) ? 1 : 0;
     double finv_inv[9];
     inv3(c->defgradinvavgc, finv_inv);
     PhaseTables pt{c->d_ph, c->d_tab};
@@ -1051,6 +1054,23 @@ int cg_set_field(cg_ctx* c, const char* name, const double* in, size_t n) {
     for (int k = 0; k < d.stored; k++)
         HIPCHK(c, hipMemcpy(d.p + (size_t)k * c->f.ld, h.data() + (size_t)k * V, V * sizeof(double), hipMemcpyHostToDevice));
     c->sgsum_valid = 0;
+    return 0;
+}
+
+int cg_set_thermo(cg_ctx* c, const double* eth) {
+    if (!c || !c->initialised || !eth) return -1;
+    Fields& f = c->f;
+    const long V = c->V;
+    if (!f.eth && dalloc(c, &f.eth, 6 * f.ld)) return -3;
+    std::vector<double> h((size_t)6 * V, 0.0);
+    for (long v = 0; v < V; v++) {
+        double s6[6];
+        to6(eth + 9 * v, s6);  // chg_basis(edotth, eth, ..., 2, 6): the symmetric part in the b-basis
+        for (int k = 0; k < 6; k++) h[(size_t)k * V + v] = c->ph[c->phase[v] - 1].igas ? 0.0 : s6[k];
+    }
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    for (int k = 0; k < 6; k++)
+        HIPCHK(c, hipMemcpy(f.eth + (size_t)k * f.ld, h.data() + (size_t)k * V, V * sizeof(double), hipMemcpyHostToDevice));
     return 0;
 }
 

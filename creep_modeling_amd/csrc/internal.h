@@ -47,6 +47,7 @@ struct Fields {
     double *schA;         // 18: A = Fe ag and A^-1 of update_schmid (sch_s = sym(A b_s x A^-T n_s))
     double *edotp, *ept, *dgs, *epvm;   // 6,6,6,1
     double *Finc, *Feinc;               // 9,9
+    double *eth;          // 6 (b-basis): eigenstrain rate of the voxel's grain (ithermo = 1), or null
     double *s_L, *s_Lold, *s_sg, *s_edotp, *s_gdot, *s_ttau;  // snapshots
     double2* spec;        // FP64 half spectrum, [point][9] (point = (k*n2+j)*h1+kx)
     float2* specf;        // mixed form: FP32 half spectrum
@@ -59,6 +60,7 @@ struct Fields {
 struct PassConst {
     double tdot, complgas, error, erroral, mix, lscale;
     int modal, do_mix, kin_ctrl, lold_same, iuphard;
+    int thermo;           // the eigenstrain rate f.eth enters the solve (first increment, ithermo = 1)
 };
 
 // reduction slots written by the fused correction + constitutive kernel

@@ -616,6 +616,10 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
             sg[c] = lam[c];  // the solve starts from lambda
         }
         to6(Lmix, D);
+        // ithermo = 1, first increment of the process: edot gains the eigenstrain rate
+        // (various_functions.f90:1226-1227, 1265); in R = sg - lambda + cm (edot - D) that is D - eth
+        if (pc.thermo)
+            for (int c = 0; c < 6; c++) D[c] -= x.ld(f.eth, c);
         // the operators of this increment (k_solve_ops): the L D L^T factors of cm^-1 (unit
         // L below the diagonal, 1/d on it) -> LDS, for the residual norm |cm R'|; cm^-1 =
         // L D L^T rebuilt here; A = cm^-1 + S, b = cm^-1 lambda + D + S sgt, S = sym(s66)/tdot
@@ -887,6 +891,8 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 :
                 double L[9];
                 for (int c = 0; c < 9; c++) L[c] = x.ld(f.L, c);
                 to6(L, D);
+                if (pc.thermo)  // eigenstrain rate of the first increment (ithermo = 1)
+                    for (int c = 0; c < 6; c++) D[c] -= x.ld(f.eth, c);
             }
             for (int c = 0; c < 6; c++) {
                 lam[c] = x.ld(f.sg, c);

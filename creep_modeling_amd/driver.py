@@ -105,7 +105,7 @@ class Solver:
             P.tdot = min(self.tdot_plan, rem) if rem > 0.0 else self.tdot_plan
         do_schmid = int((self.imicro == 1 and self.ipr == 0) or P.iupdate == 1)
         self._chk(L.cg_begin_increment(h, C.byref(P), do_schmid), "cg_begin_increment")
-        st = _lib.CgIterState(0, 1.0)
+        st = _lib.CgIterState(0, 1.0, self.imicro)
         out = _lib.CgIterOut()
         did_converge = accept = False
         passes = rollbacks = nr = ev = ou = 0

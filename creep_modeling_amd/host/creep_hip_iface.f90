@@ -22,6 +22,7 @@ module creep_hip_iface
     type, bind(C) :: cg_iter_state
         integer(c_int) :: iter
         real(c_double) :: mix
+        integer(c_int) :: imicro
     end type cg_iter_state
 
     type, bind(C) :: cg_iter_out
@@ -101,6 +102,11 @@ module creep_hip_iface
             real(c_double), value :: tdot
             real(c_double) :: out4(4)
             integer(c_int) :: nsub
+        end function
+        integer(c_int) function cg_set_thermo(ctx, eth) bind(C, name='cg_set_thermo')
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            real(c_double) :: eth(*)
         end function
         ! device pointers in and out, nothing allocated or synchronised (callers that cannot allocate)
         integer(c_int) function cg_kinematic_substep_noalloc(nvox, velgrad, ld, detf, tdot, out5, stream) &

@@ -32,6 +32,12 @@ module host_io
         type(cg_process), allocatable :: proc(:)
     end type input_t
 
+    ! eigenstrain rates of ithermo = 1 (IO_functions.f90:316-341): one 3x3 tensor per grain from
+    ! the file the last such process names (every process assigns the grain table to the voxels,
+    ! and the table holds what was read last)
+    real(8), allocatable :: eth_grain(:, :, :)   ! (3, 3, ngrains)
+    logical :: have_thermo = .false.
+
 contains
 
     subroutine fatal(msg)
@@ -354,11 +360,32 @@ contains
         if (.not. (p%igamma == 0 .or. p%igamma == 1 .or. p%igamma == 2)) &
             call fatal('igamma must be 0 (continuous), 1 (discrete), or 2 (modified)')
         read (11, *) p%ithermo
-        if (p%ithermo == 1) call fatal('ithermo=1 (eigenstrain file) is not supported')
+        if (p%ithermo == 1) call read_thermo()
         if (irecover /= 0 .and. irecover /= 1) call fatal('irecover must be 0 or 1')
         if (isave /= 0 .and. isave /= 1) call fatal('isave must be 0 or 1')
         if (iwtex /= 0 .and. iwtex /= 1) call fatal('iwtex must be 0 or 1')
     end subroutine read_process
+
+    ! the eigenstrain file: ngrains, then eth11 eth22 eth33 eth23 eth31 eth12 per grain
+    subroutine read_thermo()
+        character(len=256) :: filethermo
+        integer :: ngr, ig
+        real(8) :: e(6)
+        read (11, '(a)') filethermo
+        open (unit=49, file=trim(adjustl(filethermo)), status='old')
+        read (49, *) ngr
+        if (allocated(eth_grain)) deallocate (eth_grain)
+        allocate (eth_grain(3, 3, ngr))
+        do ig = 1, ngr
+            read (49, *) e
+            eth_grain(1, 1, ig) = e(1); eth_grain(2, 2, ig) = e(2); eth_grain(3, 3, ig) = e(3)
+            eth_grain(2, 3, ig) = e(4); eth_grain(3, 2, ig) = e(4)
+            eth_grain(3, 1, ig) = e(5); eth_grain(1, 3, ig) = e(5)
+            eth_grain(1, 2, ig) = e(6); eth_grain(2, 1, ig) = e(6)
+        end do
+        close (49)
+        have_thermo = .true.
+    end subroutine read_thermo
 
     ! results.csv (IO_functions.f90:1237-1303)
     subroutine write_results(dir, init, iter, imacro, ipr, sim_time, wall_time, tdot, tdot_ref, o)

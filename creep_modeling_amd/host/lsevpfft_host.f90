@@ -126,6 +126,7 @@ program lsevpfft_host
     if (rc /= 0) call fatal('cg_set_voxels: '//trim(last_error(ctx)))
     rc = cg_init(ctx, inp%proc(1))
     if (rc /= 0) call fatal_rc('cg_init: '//trim(last_error(ctx)), rc)
+    if (have_thermo) call set_thermo()
     call write_vtk_cell(ctx, inp, 0, output_dir)
 
     call system_clock(start, cnt_rate)
@@ -169,6 +170,7 @@ program lsevpfft_host
                 accept_step = .false.
                 st%iter = 0
                 st%mix = 1.d0
+                st%imicro = imicro
                 npass = 0
                 do while (.not. (did_converge .or. accept_step))
                     call write_status(.false.)
@@ -326,6 +328,27 @@ contains
         rc = cg_set_phase(ctx, iph, 0, ns, p%nmodes, p%nsm, dn, db, tau, th, p%nrs, p%gamd0, p%hpfac, hard, cflat)
         deallocate (cflat, dn, db, tau, th, hard)
     end subroutine set_phase
+
+    ! per-voxel eigenstrain rates from the grain table (micro%voxel%eth = eth(:,:,grain))
+    subroutine set_thermo()
+        real(c_double), allocatable :: e(:)
+        integer(8) :: v
+        integer :: g, r, c
+        allocate (e(9 * nv))
+        e = 0.d0
+        do v = 1, nv
+            g = inp%grain(v)
+            if (g < 1 .or. g > size(eth_grain, 3)) call fatal('grain id without an eigenstrain rate in the ithermo file')
+            do r = 1, 3
+                do c = 1, 3
+                    e(9 * (v - 1) + 3 * (r - 1) + c) = eth_grain(r, c, g)
+                end do
+            end do
+        end do
+        rc = cg_set_thermo(ctx, e)
+        if (rc /= 0) call fatal('cg_set_thermo: '//trim(last_error(ctx)))
+        deallocate (e)
+    end subroutine set_thermo
 
     subroutine replan(p, tdot_new)
         type(cg_process), intent(inout) :: p

@@ -142,6 +142,10 @@ class CreepProcess:
     #: ``utils/sim_cases/creep_calibration.py``); "tension": the velocity gradient L33 = udot33
     #: imposed with sigma33 free (the tension template, ``utils/sim_cases/tension.py:33-47``)
     control: str = "creep"
+    #: 1: an eigenstrain rate per grain acts in the first increment (IO_functions.f90:316-341);
+    #: thermo_file names the file (written by write_thermo)
+    ithermo: int = 0
+    thermo_file: str = ""
 
 
 # ---------------------------------------------------------------------------
@@ -302,8 +306,20 @@ def write_fft_in(path, n, microstructure_path, cupl2_path, cuel_path, processes,
             "0               IWTEX",
             "%d %d            IWFIELDS,IWSTEP" % (p.iwfields, p.iwstep),
             "%r %d            XC0, IGAMMA" % (float(p.xc0), p.igamma),
-            "0               ITHERMO",
+            "%d               ITHERMO" % p.ithermo,
         ]
+        if p.ithermo == 1:
+            lines.append(str(p.thermo_file))
+    Path(path).write_text("\n".join(lines) + "\n")
+
+
+def write_thermo(path, eth):
+    """Eigenstrain file of ithermo = 1: the number of grains, then eth11 eth22 eth33 eth23 eth31
+    eth12 per grain; eth is (ngrains, 3, 3) (symmetric part used)."""
+    eth = np.asarray(eth, float)
+    lines = ["%d" % eth.shape[0]]
+    for e in eth:
+        lines.append(" ".join(repr(float(x)) for x in (e[0, 0], e[1, 1], e[2, 2], e[1, 2], e[2, 0], e[0, 1])))
     Path(path).write_text("\n".join(lines) + "\n")
 
 

@@ -21,7 +21,7 @@ EXPORTS = (
     "cg_create", "cg_destroy", "cg_last_error", "cg_set_phase", "cg_set_voxels", "cg_init",
     "cg_begin_increment", "cg_iterate", "cg_rollback", "cg_kinematic_substep", "cg_end_increment",
     "cg_calc_c0", "cg_field_ncomp", "cg_get_field", "cg_set_field", "cg_get_macro", "cg_get_stats",
-    "cg_set_timing", "cg_kinematic_substep_noalloc",
+    "cg_set_timing", "cg_kinematic_substep_noalloc", "cg_set_thermo",
 )
 
 
@@ -38,7 +38,7 @@ class CgProcess(C.Structure):
 
 
 class CgIterState(C.Structure):
-    _fields_ = [("iter", C.c_int), ("mix", C.c_double)]
+    _fields_ = [("iter", C.c_int), ("mix", C.c_double), ("imicro", C.c_int)]
 
 
 class CgIterOut(C.Structure):
@@ -122,6 +122,7 @@ def load():
     L.cg_get_stats.argtypes = [vp, C.POINTER(CgStats), C.c_int]
     L.cg_set_timing.argtypes = [vp, C.c_int]
     L.cg_kinematic_substep_noalloc.argtypes = [C.c_long, vp, C.c_long, vp, C.c_double, vp, vp]
+    L.cg_set_thermo.argtypes = [vp, _D]
     _lib = L
     return L
 
@@ -178,6 +179,9 @@ class Context:
         self._chk(L.cg_set_voxels(h, dp(eul), ip(gr), ip(phv)))
         self._p1 = process_struct(problem.processes[0])
         self._chk(L.cg_init(h, C.byref(self._p1)))
+        eth = getattr(problem, "eth", None)
+        if eth is not None:  # ithermo = 1 in some process: per-voxel eigenstrain rate
+            self._chk(L.cg_set_thermo(h, dp(np.ascontiguousarray(eth, np.float64).reshape(-1))), "cg_set_thermo")
 
     def _chk(self, rc, what=""):
         if rc != 0:

@@ -107,6 +107,7 @@ class Process:
     xc0: float
     igamma: int
     ithermo: int
+    eth_grain: np.ndarray = None  # (ngrains + 1, 3, 3) eigenstrain rates of ithermo = 1 (row 0 unused)
 
 
 @dataclass
@@ -119,6 +120,7 @@ class Problem:
     complgas: float
     idispini: int
     processes: list
+    eth: np.ndarray = None  # (V, 3, 3) per-voxel eigenstrain rate (some process has ithermo = 1)
 
     @property
     def nvox(self):
@@ -271,7 +273,7 @@ def read_microstructure(path, n):
     return euler, grain, phase
 
 
-def _read_process(r):
+def _read_process(r, resolve):
     r.skip()
     r.skip()
     iudot = np.array([r.values(3, int) for _ in range(3)], dtype=np.int32)
@@ -339,11 +341,36 @@ def _read_process(r):
     if igamma not in (0, 1, 2):
         raise InputError("igamma must be 0 (continuous), 1 (discrete), or 2 (modified)")
     ithermo = r.values(1, int)[0]
-    if ithermo == 1:
-        raise InputError("ithermo=1 (eigenstrain file) is not supported")
+    eth_grain = read_thermo(resolve(r.line())) if ithermo == 1 else None
     return Process(iudot, udot, idsim, iscau, sc, dsim, tdot, tdotref, tdotmin, devmmx, ictrl,
                    devmapp, nsteps, error, erroral, itmax, irecover, isave, iupdate, iuphard, iwtex,
-                   iwfields, iwstep, xc0, igamma, ithermo)
+                   iwfields, iwstep, xc0, igamma, ithermo, eth_grain)
+
+
+def read_thermo(path):
+    """The eigenstrain file of ithermo = 1 (IO_functions.f90:316-332): the number of grains,
+    then eth11 eth22 eth33 eth23 eth31 eth12 per grain.  Returns (ngrains + 1, 3, 3), row 0
+    unused (grain ids are 1-based)."""
+    r = _Reader(path)
+    ngr = r.values(1, int)[0]
+    eth = np.zeros((ngr + 1, 3, 3))
+    for g in range(1, ngr + 1):
+        e = r.values(6)
+        eth[g] = [[e[0], e[5], e[4]], [e[5], e[1], e[3]], [e[4], e[3], e[2]]]
+    return eth
+
+
+def voxel_eth(procs, grain):
+    """Per-voxel eigenstrain rate: every process assigns the grain table to the voxels
+    (IO_functions.f90:334-341) and the table holds the file read last, so the last process
+    with ithermo = 1 decides.  None if no process has ithermo = 1."""
+    tabs = [p.eth_grain for p in procs if p.eth_grain is not None]
+    if not tabs:
+        return None
+    tab = tabs[-1]
+    if grain.max() >= tab.shape[0]:
+        raise InputError("grain id %d has no eigenstrain rate in the ithermo file" % grain.max())
+    return tab[grain]
 
 
 def read_fft_in(path):
@@ -387,8 +414,8 @@ def read_fft_in(path):
     nproc = r.values(1, int)[0]
     if nproc < 1:
         raise InputError("cannot have less than one processes")
-    procs = [_read_process(r) for _ in range(nproc)]
-    return Problem(n, phases, grain, phase, euler, complgas, idispini, procs)
+    procs = [_read_process(r, resolve) for _ in range(nproc)]
+    return Problem(n, phases, grain, phase, euler, complgas, idispini, procs, voxel_eth(procs, grain))
 
 
 def problem_from_case(ms, mat, processes, complgas=0.01):
@@ -411,6 +438,7 @@ def problem_from_case(ms, mat, processes, complgas=0.01):
         for _ in range(15):
             r.skip()
         nproc = r.values(1, int)[0]
-        procs = [_read_process(r) for _ in range(nproc)]
-    return Problem(tuple(ms.n), [ph], ms.grain.astype(np.int32), ms.phase.astype(np.int32),
-                   np.asarray(ms.euler, np.float64), complgas, 0, procs)
+        procs = [_read_process(r, lambda name: Path(name.strip())) for _ in range(nproc)]
+    grain = ms.grain.astype(np.int32)
+    return Problem(tuple(ms.n), [ph], grain, ms.phase.astype(np.int32), np.asarray(ms.euler, np.float64), complgas, 0,
+                   procs, voxel_eth(procs, grain))

@@ -55,6 +55,7 @@ typedef struct {
 typedef struct {
     int iter;      /* 0 at the start of an increment or after a rollback */
     double mix;    /* relaxation factor applied at the next pass           */
+    int imicro;    /* increment number within the process (1 = first)    */
 } cg_iter_state;
 
 /* Results of one fixed-point pass (LS-EVPFFT.f90:238-358). */
@@ -149,6 +150,11 @@ int cg_calc_c0(cg_ctx *ctx, const cg_process *p);
 int cg_field_ncomp(cg_ctx *ctx, const char *name);
 int cg_get_field(cg_ctx *ctx, const char *name, double *out, size_t n);
 int cg_set_field(cg_ctx *ctx, const char *name, const double *in, size_t n);
+/* The eigenstrain rate of ithermo = 1 (IO_functions.f90:316-341: one 3x3 tensor per grain,
+ * assigned to the grain's voxels; here given per voxel, (V,9) voxel-major, row-major 3x3).  It
+ * enters solve_res (various_functions.f90:1226-1227, 1265) in the first increment
+ * (cg_iter_state.imicro == 1) of every process whose ithermo is 1. */
+int cg_set_thermo(cg_ctx *ctx, const double *eth);
 /* Macroscopic operators: c0[81], s0[81], de_ds[81], c066[36], goperr0[81] (any may be NULL). */
 int cg_get_macro(cg_ctx *ctx, double *c0, double *s0, double *de_ds, double *c066, double *goperr0);
 

@@ -187,6 +187,9 @@ class Oracle:
             raise RuntimeError("orc_init: %s" % L.orc_error(self.h).decode())
         if rc > 0:  # the reference's `stop 1` (singular system at initialisation)
             raise OracleStop(rc, L.orc_error(self.h).decode())
+        eth = getattr(problem, "eth", None)
+        if eth is not None:  # ithermo = 1: per-voxel eigenstrain rate
+            self.set_field("eth", np.ascontiguousarray(eth, np.float64).reshape(-1))
 
     def step(self):
         row = OrcRow()

@@ -445,6 +445,7 @@ struct orc_ctx {
     double *defgrade, *defgradp, *ag, *cg66, *sg66, *c066mod, *cgm, *cgas;
     double *sch, *crss, *trialtau, *gamdot, *gacumgr;
     double *edotp, *ept, *disgradsym, *epvm, *defgradinc, *defgradeinc;
+    double *eth; /* eigenstrain rate of the voxel's grain (ithermo = 1), 3x3 */
     double *s_velgrad, *s_velgradold, *s_sg, *s_edotp, *s_gamdot, *s_trialtau;
     double *Goper; /* V*81 */
     double *xnode; /* 3*(n1+1)*(n2+1)*(n3node) */
@@ -496,7 +497,7 @@ static void freeall(orc_ctx *c) {
                     c->defgradp, c->ag, c->cg66, c->sg66, c->c066mod, c->cgm, c->cgas, c->sch,
                     c->crss, c->trialtau, c->gamdot, c->gacumgr, c->edotp, c->ept, c->disgradsym,
                     c->epvm, c->defgradinc, c->defgradeinc, c->s_velgrad, c->s_velgradold, c->s_sg,
-                    c->s_edotp, c->s_gamdot, c->s_trialtau, c->Goper, c->xnode, c->four};
+                    c->s_edotp, c->s_gamdot, c->s_trialtau, c->Goper, c->xnode, c->four, c->eth};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
 }
 void orc_destroy(orc_ctx *c) {
@@ -569,7 +570,7 @@ static int alloc_state(orc_ctx *c) {
     AL(c->cgas, 36 * V);
     AL(c->sch, (size_t)5 * ns * V); AL(c->crss, (size_t)2 * ns * V); AL(c->trialtau, (size_t)2 * ns * V);
     AL(c->gamdot, (size_t)ns * V); AL(c->gacumgr, V);
-    AL(c->edotp, 9 * V); AL(c->ept, 9 * V); AL(c->disgradsym, 9 * V); AL(c->epvm, V);
+    AL(c->edotp, 9 * V); AL(c->ept, 9 * V); AL(c->disgradsym, 9 * V); AL(c->epvm, V); AL(c->eth, 9 * V);
     AL(c->defgradinc, 9 * V); AL(c->defgradeinc, 9 * V);
     AL(c->s_velgrad, 9 * V); AL(c->s_velgradold, 9 * V); AL(c->s_sg, 9 * V); AL(c->s_edotp, 9 * V);
     AL(c->s_gamdot, (size_t)ns * V); AL(c->s_trialtau, (size_t)2 * ns * V);
@@ -1020,6 +1021,11 @@ static vox_res solve_voxel(orc_ctx *c, long v) {
         double errtau = 2.0 * tol;
         double edotp[6], ded[36], edote[6], edot[6], dedot[36], res[6], jac[36], sgold[6] = {0};
         double dsgnorm = 0, ddgnorm = 0;
+        /* eigenstrain rate of the first increment of a process with ithermo = 1
+           (various_functions.f90:1226-1227, added to edot at :1265) */
+        const int thermo = c->imicro == 1 && Pr->ithermo == 1;
+        double edotth[6] = {0, 0, 0, 0, 0, 0};
+        if (thermo) B2(edotth, c->eth + 9 * v);
         while (itertau < 10 && errtau > tol) {
             itertau++;
             out.outer++;
@@ -1039,6 +1045,7 @@ static vox_res solve_voxel(orc_ctx *c, long v) {
                         edote[a] = s;
                     }
                     for (int a = 0; a < 6; a++) edot[a] = edotp[a] + edote[a];
+                    if (thermo) for (int a = 0; a < 6; a++) edot[a] += edotth[a];
                     for (int t = 0; t < 36; t++) dedot[t] = ded[t] + s66[t] / tdot;
                     for (int a = 0; a < 6; a++) {
                         double s = 0;
@@ -2088,6 +2095,7 @@ static double *field_ptr(orc_ctx *c, const char *name, int *ncomp) {
         {"cgm", c->cgm, 36}, {"cgas", c->cgas, 36}, {"sch", c->sch, 5 * c->ns}, {"crss", c->crss, 2 * c->ns},
         {"trialtau", c->trialtau, 2 * c->ns}, {"gamdot", c->gamdot, c->ns}, {"gacumgr", c->gacumgr, 1},
         {"edotp", c->edotp, 9}, {"ept", c->ept, 9}, {"disgradsym", c->disgradsym, 9}, {"epvm", c->epvm, 1},
+        {"eth", c->eth, 9},
         {"defgradinc", c->defgradinc, 9}, {"defgradeinc", c->defgradeinc, 9},
     };
     for (size_t i = 0; i < sizeof T / sizeof T[0]; i++)
