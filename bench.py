@@ -215,7 +215,7 @@ def bench_sweep(args):
         solvers = [driver.Solver(p, precision=prec, scan_cond=False) for p in problems]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        rows = batch.run_batch(problems, workers=args.workers, solvers=solvers)
+        rows = batch.run_batch(problems, workers=args.workers, solvers=solvers, engine=args.engine)
         torch.cuda.synchronize()
         T = time.perf_counter() - t0
         vi = sum(sum(r["passes"] for r in rr) * p.nvox for rr, p in zip(rows, problems))
@@ -228,9 +228,11 @@ def bench_sweep(args):
                 "vs_baseline": None, "dtype": "f64",
                 "data": "synthetic (seeded periodic Voronoi RVEs, uniform SO(3) orientations)",
                 "config": {"workload": "%d FCC RVEs, edges in {32,48,64,96}, 32-512 grains, sigma33 200-400 MPa, "
-                                       "n 8-14, Arrhenius-scaled gamma0 (475/525/575 C); %d increments each, "
-                                       "%d RVEs in flight; step = one increment of every RVE"
-                                       % (args.rves, args.warmup + args.steps, args.workers),
+                                       "n 8-14, Arrhenius-scaled gamma0 (475/525/575 C); %d increments each; %s; "
+                                       "step = one increment of every RVE"
+                                       % (args.rves, args.warmup + args.steps,
+                                          "one constitutive launch per round over all RVEs" if args.engine == "shared"
+                                          else "%d RVEs in flight on host threads" % args.workers),
                            "voxels": sum(p.nvox for p in problems),
                            "passes": sum(sum(r["passes"] for r in rr) for rr in rows)},
             }
@@ -267,7 +269,9 @@ def main():
                     help="1: FCC n=10 (the headline workload); 2: FCC n=15; 3: BCC 48 systems, biaxial, dt=0.5 s; "
                          "4: batched sweep of independent FCC RVEs (32^3-96^3)")
     ap.add_argument("--rves", type=int, default=64, help="config 4: number of RVEs in the sweep")
-    ap.add_argument("--workers", type=int, default=8, help="config 4: RVEs in flight at a time")
+    ap.add_argument("--workers", type=int, default=8, help="config 4, thread engine: RVEs in flight at a time")
+    ap.add_argument("--engine", default="shared", choices=("shared", "threads"),
+                    help="config 4: shared constitutive launch over all RVEs (default) or a host-thread pool")
     ap.add_argument("--n", type=int, default=None, help="grid edge (default 128 for configs 1-2, 256 for 3)")
     ap.add_argument("--grains", type=int, default=None, help="Voronoi grains (default 512, 2048 for config 3)")
     ap.add_argument("--s33", type=float, default=None, help="creep stress sigma33 in MPa (config default 300)")

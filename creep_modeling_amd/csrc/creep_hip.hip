@@ -591,10 +591,14 @@ int cg_begin_increment(cg_ctx* c, const cg_process* p, int do_schmid) {
     return 0;
 }
 
-int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
-    if (!c || !c->initialised || !p || !it) return -1;
+}  // extern "C"
+
+// One fixed-point pass is split in three so that several RVEs can share the constitutive
+// launch (cg_batch_iterate): pass_begin enqueues the work before it on the context's stream
+// and fills the solve's arguments; pass_reduce enqueues the reductions of the solve's sums
+// and their copy to the host; pass_finish (after the stream has finished) is the host part.
+static void pass_begin(cg_ctx* c, cg_process* p, cg_iter_state* it, SolveArgs& sa) {
     Fields& f = c->f;
-    const long V = c->V;
     if (it->iter == 0) {  // LS-EVPFFT.f90:222-231
         if (c->ngas) launch_gas_compl(f, p->tdot, c->complgas, c->modal, c->s);
         memset(c->dvelgradavg, 0, sizeof c->dvelgradavg);
@@ -625,10 +629,7 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     // ithermo = 1: the eigenstrain rate acts in the first increment of the process only
     // (various_functions.f90:1226: imicro == 1 .and. ithermo == 1)
     pc.thermo = (f.eth != nullptr && p->ithermo == 1 && it->imicro == 1) ? 1 : 0;
-    double finv_inv[9];
-    inv3(c->defgradinvavgc, finv_inv);
     PhaseTables pt{c->d_ph, c->d_tab};
-    tstart(c, 1);
     if (!c->taun_valid) {
         launch_taun(f, pt, c->sym_tau, c->s);
         c->taun_valid = 1;
@@ -638,16 +639,24 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
         c->sop_valid = 1;
         c->sop_tdot = p->tdot;
     }
-    launch_correct_solve_full(f, pt, pc, c->dvelgradavg, finv_inv, c->d_fluct, c->s, &nb, c->int_pow, c->sym_tau, c->small_ns);
-    tstop(c, 1);
-    launch_reduce_final_cm(f.part, nb, R_NSOLVE, f.red, c->s);
+    sa.pc = pc;
+    for (int i = 0; i < 9; i++) sa.dvelgradavg[i] = c->dvelgradavg[i];
+    inv3(c->defgradinvavgc, sa.finv_inv);
+    sa.fluct = c->d_fluct;
+}
+
+static void pass_reduce(cg_ctx* c) {
+    Fields& f = c->f;
+    launch_reduce_final_cm(f.part, solve_blocks(c->V), R_NSOLVE, f.red, c->s);
     // next pass' <sigma> = sum_all w sigma
     launch_copy(c->d_sgsum, f.red + R_SGALL, 9, c->s);
-    HIPCHK(c, hipMemcpyAsync(c->h_red, f.red, R_NCOPY * sizeof(double), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    HIPCHK(c, hipGetLastError());
-    tcollect(c, 0); tcollect(c, 1); tcollect(c, 2);
+    hipMemcpyAsync(c->h_red, f.red, R_NCOPY * sizeof(double), hipMemcpyDeviceToHost, c->s);
+}
+
+static int pass_finish(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
+    const long V = c->V;
     if (int e = singular_from(c, c->h_red, "update_schmid / calc_gas_compl / solve_res")) return e;
+    const bool kin_ctrl = (p->iscau[2] == 0);
     c->sgsum_valid = 1;
     c->lold_same = 1;
     const double* r = c->h_red;
@@ -664,7 +673,7 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     }
     // dL_iter -> mix for the next pass (LS-EVPFFT.f90:313-327)
     double dl = 0.0;
-    if (!pc.kin_ctrl) {
+    if (!kin_ctrl) {
         const double ds = r[R_WSOL] > 0.0 ? sqrt(r[R_DLSOL] / r[R_WSOL]) : 0.0;
         const double dg = r[R_WGAS] > 0.0 ? sqrt(r[R_DLGAS] / r[R_WGAS]) : 0.0;
         dl = std::max(ds, dg);
@@ -738,6 +747,199 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
         return fail(c, 2, "non-finite fixed-point errors (errs=%g erre=%g errsbc=%g) at pass %d", errs, erre, errsbc,
                     it->iter);
     return 0;
+}
+
+extern "C" {
+
+int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
+    if (!c || !c->initialised || !p || !it) return -1;
+    SolveArgs sa;
+    pass_begin(c, p, it, sa);
+    PhaseTables pt{c->d_ph, c->d_tab};
+    int nb;
+    tstart(c, 1);
+    launch_correct_solve_full(c->f, pt, sa.pc, sa.dvelgradavg, sa.finv_inv, sa.fluct, c->s, &nb, c->int_pow,
+                              c->sym_tau, c->small_ns);
+    tstop(c, 1);
+    pass_reduce(c);
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    HIPCHK(c, hipGetLastError());
+    tcollect(c, 0); tcollect(c, 1); tcollect(c, 2);
+    return pass_finish(c, p, it, out);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched passes: several RVEs share the constitutive launch
+// ---------------------------------------------------------------------------
+struct cg_batch {
+    std::vector<cg_ctx*> ctx;
+    hipStream_t s = nullptr;          // the shared constitutive launches
+    hipEvent_t ev_solve = nullptr;
+    std::vector<hipEvent_t> ev_pre;   // per RVE: its pre-solve work is enqueued
+    SolveWork* h_work = nullptr;      // pinned host copy of the work table
+    SolveWork* d_work = nullptr;
+    int* d_blk_rve = nullptr;         // RVE of every block of the launches, groups one after another
+    size_t blk_cap = 0;
+    std::vector<int> table_active;    // active set d_blk_rve was built for
+    std::vector<int> key;             // kernel instance of every RVE: modal, integer exponents, equal strengths, <= 12 systems
+    std::vector<int> grp_key, grp_off, grp_nblk;  // launches of the current table
+    // the pass in flight (cg_batch_launch -> cg_batch_wait)
+    std::vector<cg_process*> pend_p;
+    cg_iter_state* pend_st = nullptr;
+    cg_iter_out* pend_out = nullptr;
+    std::vector<int> pend_active;
+};
+
+static int solve_key(const cg_ctx* c) {
+    return (c->modal ? 8 : 0) + (c->int_pow ? 4 : 0) + (c->sym_tau ? 2 : 0) + (c->small_ns ? 1 : 0);
+}
+
+// d_blk_rve and the RVEs' first blocks for the active set: one launch per kernel instance
+static int batch_table(cg_batch* b, const int* active) {
+    const int n = (int)b->ctx.size();
+    std::vector<int> act(active, active + n);
+    for (int& a : act) a = a ? 1 : 0;
+    if (act == b->table_active) return 0;
+    std::vector<int> blk;
+    b->grp_key.clear(); b->grp_off.clear(); b->grp_nblk.clear();
+    for (int k = 0; k < 16; k++) {
+        const size_t off = blk.size();
+        for (int i = 0; i < n; i++) {
+            if (!act[i] || b->key[i] != k) continue;
+            const long nb = solve_blocks(b->ctx[i]->V);
+            b->h_work[i].blk0 = (long)(blk.size() - off);
+            b->h_work[i].nblk = nb;
+            blk.insert(blk.end(), (size_t)nb, i);
+        }
+        if (blk.size() > off) {
+            b->grp_key.push_back(k);
+            b->grp_off.push_back((int)off);
+            b->grp_nblk.push_back((int)(blk.size() - off));
+        }
+    }
+    if (blk.size() > b->blk_cap) {
+        if (b->d_blk_rve) hipFree(b->d_blk_rve);
+        b->d_blk_rve = nullptr;
+        if (hipMalloc(&b->d_blk_rve, blk.size() * sizeof(int)) != hipSuccess)
+            return noctx_fail(-3, "cg_batch_iterate: hipMalloc of the block table failed");
+        b->blk_cap = blk.size();
+    }
+    if (!blk.empty() &&
+        hipMemcpyAsync(b->d_blk_rve, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice, b->s) != hipSuccess)
+        return noctx_fail(-2, "cg_batch_iterate: block table upload failed");
+    if (hipStreamSynchronize(b->s) != hipSuccess) return noctx_fail(-2, "cg_batch_iterate: synchronisation failed");
+    b->table_active = act;
+    return 0;
+}
+
+extern "C" {
+
+int cg_batch_create(cg_ctx* const* ctxs, int n, cg_batch** out) {
+    if (!out || !ctxs || n < 1) return noctx_fail(-1, "cg_batch_create: bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; i++)
+        if (!ctxs[i] || !ctxs[i]->initialised) return noctx_fail(-1, "cg_batch_create: context %d not initialised", i);
+    cg_batch* b = new cg_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    b->ev_pre.assign(n, nullptr);
+    for (int i = 0; i < n; i++) b->key.push_back(solve_key(ctxs[i]));
+    bool ok = hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&b->ev_solve, hipEventDisableTiming) == hipSuccess &&
+              hipHostMalloc(&b->h_work, n * sizeof(SolveWork)) == hipSuccess &&
+              hipMalloc(&b->d_work, n * sizeof(SolveWork)) == hipSuccess;
+    for (int i = 0; ok && i < n; i++) ok = hipEventCreateWithFlags(&b->ev_pre[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        cg_batch_destroy(b);
+        return noctx_fail(-2, "cg_batch_create: stream, event or table allocation failed");
+    }
+    memset(b->h_work, 0, n * sizeof(SolveWork));
+    *out = b;
+    return 0;
+}
+
+int cg_batch_destroy(cg_batch* b) {
+    if (!b) return 0;
+    if (b->s) hipStreamSynchronize(b->s);
+    for (hipEvent_t e : b->ev_pre)
+        if (e) hipEventDestroy(e);
+    if (b->ev_solve) hipEventDestroy(b->ev_solve);
+    if (b->h_work) hipHostFree(b->h_work);
+    if (b->d_work) hipFree(b->d_work);
+    if (b->d_blk_rve) hipFree(b->d_blk_rve);
+    if (b->s) hipStreamDestroy(b->s);
+    delete b;
+    return 0;
+}
+
+int cg_batch_launch(cg_batch* b, cg_process* const* p, cg_iter_state* st, cg_iter_out* out, const int* active) {
+    if (!b || !p || !st || !active) return noctx_fail(-1, "cg_batch_launch: bad arguments");
+    const int n = (int)b->ctx.size();
+    int nact = 0;
+    for (int i = 0; i < n; i++)
+        if (active[i]) {
+            if (!p[i]) return noctx_fail(-1, "cg_batch_launch: no process for RVE %d", i);
+            nact++;
+        }
+    b->pend_p.assign(p, p + n);
+    b->pend_st = st;
+    b->pend_out = out;
+    b->pend_active.assign(active, active + n);
+    if (!nact) return 0;
+    if (int e = batch_table(b, active)) return e;
+    // work before the solve, on every RVE's own stream
+    for (int i = 0; i < n; i++) {
+        if (!active[i]) continue;
+        cg_ctx* c = b->ctx[i];
+        SolveArgs sa;
+        pass_begin(c, p[i], &st[i], sa);
+        b->h_work[i].f = c->f;
+        b->h_work[i].pt = PhaseTables{c->d_ph, c->d_tab};
+        b->h_work[i].a = sa;
+        hipEventRecord(b->ev_pre[i], c->s);
+        hipStreamWaitEvent(b->s, b->ev_pre[i], 0);
+    }
+    // the shared constitutive launches (one per kernel instance present)
+    if (hipMemcpyAsync(b->d_work, b->h_work, n * sizeof(SolveWork), hipMemcpyHostToDevice, b->s) != hipSuccess)
+        return noctx_fail(-2, "cg_batch_iterate: work table upload failed");
+    for (size_t g = 0; g < b->grp_key.size(); g++) {
+        const int k = b->grp_key[g];
+        launch_correct_solve_batch(b->d_work, b->d_blk_rve + b->grp_off[g], b->grp_nblk[g], (k >> 3) & 1, (k >> 2) & 1,
+                                   (k >> 1) & 1, k & 1, b->s);
+    }
+    hipEventRecord(b->ev_solve, b->s);
+    // reductions and copies back, on every RVE's stream
+    for (int i = 0; i < n; i++) {
+        if (!active[i]) continue;
+        hipStreamWaitEvent(b->ctx[i]->s, b->ev_solve, 0);
+        pass_reduce(b->ctx[i]);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return noctx_fail(-2, "cg_batch_launch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int cg_batch_wait(cg_batch* b, int* rc) {
+    if (!b || !rc) return noctx_fail(-1, "cg_batch_wait: bad arguments");
+    const int n = (int)b->ctx.size();
+    for (int i = 0; i < n; i++) rc[i] = 0;
+    if ((int)b->pend_active.size() != n) return 0;  // nothing launched
+    for (int i = 0; i < n; i++)
+        if (b->pend_active[i] && hipStreamSynchronize(b->ctx[i]->s) != hipSuccess)
+            return noctx_fail(-2, "cg_batch_wait: synchronisation of RVE %d failed", i);
+    if (hipStreamSynchronize(b->s) != hipSuccess) return noctx_fail(-2, "cg_batch_wait: synchronisation failed");
+    for (int i = 0; i < n; i++)
+        if (b->pend_active[i])
+            rc[i] = pass_finish(b->ctx[i], b->pend_p[i], &b->pend_st[i], b->pend_out ? &b->pend_out[i] : nullptr);
+    b->pend_active.clear();
+    return 0;
+}
+
+int cg_batch_iterate(cg_batch* b, cg_process* const* p, cg_iter_state* st, cg_iter_out* out, const int* active,
+                     int* rc) {
+    if (int e = cg_batch_launch(b, p, st, out, active)) return e;
+    return cg_batch_wait(b, rc);
 }
 
 int cg_rollback(cg_ctx* c, cg_process* p) {

@@ -63,6 +63,22 @@ struct PassConst {
     int thermo;           // the eigenstrain rate f.eth enters the solve (first increment, ithermo = 1)
 };
 
+// Arguments of the fused correction + constitutive kernel for one pass of one RVE.
+struct SolveArgs {
+    PassConst pc;
+    double dvelgradavg[9];
+    double finv_inv[9];
+    const double* fluct;  // 9 reduced sums of w dLref Fi
+};
+// One RVE's entry of the batched constitutive launch: its blocks are blk0 .. blk0 + nblk - 1
+// of the launch.
+struct SolveWork {
+    Fields f;
+    PhaseTables pt;
+    SolveArgs a;
+    long blk0, nblk;
+};
+
 // reduction slots written by the fused correction + constitutive kernel
 // (slots 0..22 come from the constitutive phase, 23..53 from the correction phase)
 enum {
@@ -90,6 +106,10 @@ void launch_sigma_sum(const Fields& f, hipStream_t s, int* nblocks_out);
 void launch_correct_solve_full(const Fields& f, const PhaseTables& pt, const PassConst& pc,
                                const double* dvelgradavg, const double* finv_inv, const double* fluct_dev,
                                hipStream_t s, int* nb, int int_pow, int sym_tau, int small_ns);
+// the correction + constitutive kernel over several RVEs: block b belongs to RVE blk_rve[b]
+void launch_correct_solve_batch(const SolveWork* work, const int* blk_rve, int nblocks, int modal, int int_pow,
+                                int sym_tau, int small_ns, hipStream_t s);
+int solve_blocks(long V);  // blocks of one RVE in the constitutive launch
 void launch_update_schmid(const Fields& f, const PhaseTables& pt, hipStream_t s);
 void launch_c066mod(const Fields& f, const double* c066, const double* goperr0, int modal, hipStream_t s);
 void launch_gas_compl(const Fields& f, double tdot, double complgas, int modal, hipStream_t s);

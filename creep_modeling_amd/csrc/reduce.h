@@ -106,7 +106,7 @@ __device__ __forceinline__ void block_reduce_lds(const double* acc, double* part
 // Column-major variants: block b writes its sums to part[(off + i) * gridDim.x + b], so the
 // final reduction of value i reads one contiguous column (launches with many blocks).
 template <int NV, int BS>
-__device__ __forceinline__ void block_reduce_store_cm(const double* acc, double* part, int off) {
+__device__ __forceinline__ void block_reduce_store_cm(const double* acc, double* part, int off, long blk, long nblk) {
     static_assert(NV <= BS, "NV must not exceed the block size");
     __shared__ double sm[(BS / 64) * NV];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -121,11 +121,18 @@ __device__ __forceinline__ void block_reduce_store_cm(const double* acc, double*
     if (threadIdx.x < NV) {
         double s = sm[threadIdx.x];
         for (int w = 1; w < BS / 64; w++) s += sm[w * NV + threadIdx.x];
-        part[(size_t)(off + threadIdx.x) * gridDim.x + blockIdx.x] = s;
+        part[(size_t)(off + threadIdx.x) * nblk + blk] = s;
     }
 }
 template <int NV, int BS>
-__device__ __forceinline__ void block_reduce_lds_cm(const double* acc, double* part, int off, double* sm) {
+__device__ __forceinline__ void block_reduce_store_cm(const double* acc, double* part, int off) {
+    block_reduce_store_cm<NV, BS>(acc, part, off, blockIdx.x, gridDim.x);
+}
+// (blk, nblk): this block's index and the block count of the launch it belongs to (a batched
+// launch covers several of them)
+template <int NV, int BS>
+__device__ __forceinline__ void block_reduce_lds_cm(const double* acc, double* part, int off, double* sm, long blk,
+                                                    long nblk) {
     static_assert(NV * (BS / 32) <= BS, "one thread per (value, chunk)");
     constexpr int NC = BS / 32;
     const int t = threadIdx.x;
@@ -143,8 +150,12 @@ __device__ __forceinline__ void block_reduce_lds_cm(const double* acc, double* p
     if (t < NV) {
         double r = sm[t * BS];
         for (int c = 1; c < NC; c++) r += sm[t * BS + c * 32];
-        part[(size_t)(off + t) * gridDim.x + blockIdx.x] = r;
+        part[(size_t)(off + t) * nblk + blk] = r;
     }
+}
+template <int NV, int BS>
+__device__ __forceinline__ void block_reduce_lds_cm(const double* acc, double* part, int off, double* sm) {
+    block_reduce_lds_cm<NV, BS>(acc, part, off, sm, blockIdx.x, gridDim.x);
 }
 
 }  // namespace cg

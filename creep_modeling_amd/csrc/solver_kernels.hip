@@ -189,12 +189,6 @@ void launch_sigma_sum(const Fields& f, hipStream_t s, int* nb) {
 // ---------------------------------------------------------------------------
 // Fused pass: correction_to_current + dL_iter/mixing + solve_res
 // ---------------------------------------------------------------------------
-struct SolveArgs {
-    PassConst pc;
-    double dvelgradavg[9];
-    double finv_inv[9];
-    const double* fluct;  // 9 reduced sums of w dLref Fi
-};
 
 // Deviatoric b-basis components of sym(A (b x n) A^-1) = sym((A b) x (A^-T n)) for one
 // slip system (update_schmid, various_functions.f90:472-539); A = Fe ag.
@@ -793,16 +787,17 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
     acc[R_OUTER] = (double)nout;
 }
 
+// One 64-voxel block of the correction + constitutive kernel: block blk of the nblk blocks
+// that cover this RVE.  pool: the block's LDS, per thread 21 (packed cm) + 18 (A, A^-1)
+// doubles of the symmetric solve; the block reductions reuse it when no solve is using it.
 template <bool MODAL, bool INTPOW, bool SYMTAU, bool SMALLNS>
-__global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 2))) void k_correct_solve(Fields f, PhaseTables pt, SolveArgs a) {
+__device__ __forceinline__ void correct_solve_block(const Fields& f, const PhaseTables& pt, const SolveArgs& a,
+                                                    long blk, long nblk, double* pool) {
     const long V = f.V;
-    const long v = solve_block(blockIdx.x, gridDim.x) * SBS + threadIdx.x;
+    const long v = blk * SBS + threadIdx.x;
     const bool act = v < V;
     const PassConst& pc = a.pc;
     const Vox x(v, f.ld);
-    // LDS of the symmetric solve: per thread 21 (packed cm) + 18 (A, A^-1) doubles;
-    // the block reductions reuse it when no solve is using it
-    __shared__ double pool[MODAL ? 1 : 39 * SBS];
     double* const cm_lds = pool;
     double* const sa_lds = pool + 21 * SBS;
     // ---- phase A: correction_to_current + dL_iter/mixing; its 31 sums are reduced
@@ -860,9 +855,9 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 :
             }
         }
         if (MODAL) {
-            block_reduce_store_cm<R_NSOLVE - R_NPHASEB, SBS>(accA, f.part, R_NPHASEB);
+            block_reduce_store_cm<R_NSOLVE - R_NPHASEB, SBS>(accA, f.part, R_NPHASEB, blk, nblk);
         } else {  // scratch aliased onto the solve's LDS columns, free until phase B
-            block_reduce_lds_cm<R_NSOLVE - R_NPHASEB, SBS>(accA, f.part, R_NPHASEB, pool);
+            block_reduce_lds_cm<R_NSOLVE - R_NPHASEB, SBS>(accA, f.part, R_NPHASEB, pool, blk, nblk);
             __syncthreads();
         }
     }
@@ -1063,9 +1058,29 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 :
         }
     }
     __syncthreads();
-    if (MODAL) block_reduce_store_cm<R_NPHASEB, SBS>(acc, f.part, 0);
-    else block_reduce_lds_cm<R_NPHASEB, SBS>(acc, f.part, 0, pool);  // every solve has finished
+    if (MODAL) block_reduce_store_cm<R_NPHASEB, SBS>(acc, f.part, 0, blk, nblk);
+    else block_reduce_lds_cm<R_NPHASEB, SBS>(acc, f.part, 0, pool, blk, nblk);  // every solve has finished
 }
+
+template <bool MODAL, bool INTPOW, bool SYMTAU, bool SMALLNS>
+__global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 2))) void k_correct_solve(Fields f, PhaseTables pt, SolveArgs a) {
+    __shared__ double pool[MODAL ? 1 : 39 * SBS];
+    correct_solve_block<MODAL, INTPOW, SYMTAU, SMALLNS>(f, pt, a, solve_block(blockIdx.x, gridDim.x), gridDim.x, pool);
+}
+
+// The same kernel over several RVEs in one launch: block b works for RVE blk_rve[b] (a
+// per-RVE work table: its fields, phase tables, pass constants and first block), so small
+// RVEs together fill the GPU.  Each RVE's blocks, sums and results are those of its own
+// launch of k_correct_solve.
+template <bool MODAL, bool INTPOW, bool SYMTAU, bool SMALLNS>
+__global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 : 2))) void k_correct_solve_batch(
+    const SolveWork* __restrict__ work, const int* __restrict__ blk_rve) {
+    __shared__ double pool[MODAL ? 1 : 39 * SBS];
+    const int r = __builtin_amdgcn_readfirstlane(blk_rve[blockIdx.x]);
+    const SolveWork* w = work + r;
+    correct_solve_block<MODAL, INTPOW, SYMTAU, SMALLNS>(w->f, w->pt, w->a, (long)blockIdx.x - w->blk0, w->nblk, pool);
+}
+
 
 // Public launcher with the full argument set.
 void launch_correct_solve_full(const Fields& f, const PhaseTables& pt, const PassConst& pc,
@@ -1084,6 +1099,28 @@ void launch_correct_solve_full(const Fields& f, const PhaseTables& pt, const Pas
     // the symmetric-solve kernel is instantiated per (integer exponents, equal forward and
     // backward strengths, at most 12 systems per phase)
 #define CG_SOLVE(IP, ST, SM) hipLaunchKernelGGL((k_correct_solve<false, IP, ST, SM>), dim3(*nb), dim3(SBS), 0, s, f, pt, a)
+    switch ((int_pow ? 4 : 0) + (sym_tau ? 2 : 0) + (small_ns ? 1 : 0)) {
+        case 0: CG_SOLVE(false, false, false); break;
+        case 1: CG_SOLVE(false, false, true); break;
+        case 2: CG_SOLVE(false, true, false); break;
+        case 3: CG_SOLVE(false, true, true); break;
+        case 4: CG_SOLVE(true, false, false); break;
+        case 5: CG_SOLVE(true, false, true); break;
+        case 6: CG_SOLVE(true, true, false); break;
+        default: CG_SOLVE(true, true, true); break;
+    }
+#undef CG_SOLVE
+}
+int solve_blocks(long V) { return (int)((V + SBS - 1) / SBS); }
+void launch_correct_solve_batch(const SolveWork* work, const int* blk_rve, int nblocks, int modal, int int_pow,
+                                int sym_tau, int small_ns, hipStream_t s) {
+    if (modal) {
+        if (int_pow) hipLaunchKernelGGL((k_correct_solve_batch<true, true, false, false>), dim3(nblocks), dim3(SBS), 0, s, work, blk_rve);
+        else hipLaunchKernelGGL((k_correct_solve_batch<true, false, false, false>), dim3(nblocks), dim3(SBS), 0, s, work, blk_rve);
+        return;
+    }
+#define CG_SOLVE(IP, ST, SM) \
+    hipLaunchKernelGGL((k_correct_solve_batch<false, IP, ST, SM>), dim3(nblocks), dim3(SBS), 0, s, work, blk_rve)
     switch ((int_pow ? 4 : 0) + (sym_tau ? 2 : 0) + (small_ns ? 1 : 0)) {
         case 0: CG_SOLVE(false, false, false); break;
         case 1: CG_SOLVE(false, false, true); break;

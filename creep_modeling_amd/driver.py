@@ -83,12 +83,13 @@ class Solver:
         self._chk(self.L.cg_kinematic_substep(self.h, tdot, out, C.byref(nsub)), "cg_kinematic_substep")
         return nsub.value, list(out)
 
-    def step(self):
-        """Run one accepted increment; returns a row dict (results + diagnostics) or None when done."""
+    def begin(self):
+        """Start the next increment (LS-EVPFFT.f90:196-219: process advance, Delta-t plan,
+        snapshot, update_schmid, form_G, c066mod); False when every process is done."""
         L, h = self.L, self.h
         while not self.proc_active or self.imicro >= self.procs[self.ipr].nsteps:
             if self.ipr + 1 >= len(self.procs):
-                return None
+                return False
             self.ipr += 1
             P = self.procs[self.ipr]
             self.nconverged = 0
@@ -105,62 +106,87 @@ class Solver:
             P.tdot = min(self.tdot_plan, rem) if rem > 0.0 else self.tdot_plan
         do_schmid = int((self.imicro == 1 and self.ipr == 0) or P.iupdate == 1)
         self._chk(L.cg_begin_increment(h, C.byref(P), do_schmid), "cg_begin_increment")
-        st = _lib.CgIterState(0, 1.0, self.imicro)
-        out = _lib.CgIterOut()
-        did_converge = accept = False
-        passes = rollbacks = nr = ev = ou = 0
+        self.P = P
+        self.st = _lib.CgIterState(0, 1.0, self.imicro)
+        self.out = _lib.CgIterOut()
+        self.did_converge = self.accept = False
+        self.passes = self.rollbacks = self.nr = self.ev = self.ou = 0
+        return True
+
+    def step(self):
+        """Run one accepted increment; returns a row dict (results + diagnostics) or None when done."""
+        if not self.begin():
+            return None
+        while True:
+            self._chk(self.L.cg_iterate(self.h, C.byref(self.P), C.byref(self.st), C.byref(self.out)), "cg_iterate")
+            if self.pass_done():
+                return self.finish()
+
+    def pass_done(self):
+        """The control of LS-EVPFFT.f90:358-429 after one pass (self.out): convergence, strain
+        control, Delta-t cut-back with rollback.  True when the increment's loop has ended."""
+        L, h, P, st, out = self.L, self.h, self.P, self.st, self.out
+        did_converge = self.did_converge
         cap = self.max_pass_factor * max(P.itmax, 1) + 1000
-        while not (did_converge or accept):
-            self._chk(L.cg_iterate(h, C.byref(P), C.byref(st), C.byref(out)), "cg_iterate")
-            passes += 1
-            nr += out.nr_steps
-            ev += out.evals
-            ou += out.outer
-            if passes > cap:
-                raise _lib.HipError("fixed-point loop did not terminate")
-            if P.ictrl == 0:
-                vs = np.array(out.velgradavg).reshape(3, 3)
-                vs = 0.5 * (vs + vs.T)
-                hh = np.trace(vs) / 3.0
-                d = vs - hh * np.eye(3)
-                evm = math.sqrt(2.0 / 3.0 * float((d * d).sum()))
-                P.tdot = P.devmapp / evm
-                P.tdotref = P.tdot
-            if out.errs <= P.error and out.erre <= P.error and out.errsbc <= P.error:
-                did_converge = True
-            if P.ictrl == -1:
-                if P.tdot > P.tdotmin:
-                    if (not did_converge) and st.iter == P.itmax:
-                        tdot = P.tdot
-                        nsub, _ = self._kin(tdot)
-                        tdot = tdot / nsub
+        self.passes += 1
+        passes = self.passes
+        self.nr += out.nr_steps
+        self.ev += out.evals
+        self.ou += out.outer
+        if passes > cap:
+            raise _lib.HipError("fixed-point loop did not terminate")
+        if P.ictrl == 0:
+            vs = np.array(out.velgradavg).reshape(3, 3)
+            vs = 0.5 * (vs + vs.T)
+            hh = np.trace(vs) / 3.0
+            d = vs - hh * np.eye(3)
+            evm = math.sqrt(2.0 / 3.0 * float((d * d).sum()))
+            P.tdot = P.devmapp / evm
+            P.tdotref = P.tdot
+        if out.errs <= P.error and out.erre <= P.error and out.errsbc <= P.error:
+            did_converge = True
+        if P.ictrl == -1:
+            if P.tdot > P.tdotmin:
+                if (not did_converge) and st.iter == P.itmax:
+                    tdot = P.tdot
+                    nsub, _ = self._kin(tdot)
+                    tdot = tdot / nsub
+                    tdot = max(P.tdotmin, min(tdot, 0.5 * P.tdot))
+                    nsub = math.ceil(P.tdot / tdot)
+                    if nsub > 1:
+                        self._replan(P, tdot)
+                        self.nconverged = 0
+                        st.iter = 0
+                        self._chk(L.cg_rollback(h, C.byref(P)), "cg_rollback")
+                        self.rollbacks += 1
+                elif did_converge:
+                    tdot = P.tdot
+                    nsub, kin = self._kin(tdot)
+                    tdot = tdot / nsub
+                    if kin[3] <= 1.0e-3:
                         tdot = max(P.tdotmin, min(tdot, 0.5 * P.tdot))
-                        nsub = math.ceil(P.tdot / tdot)
-                        if nsub > 1:
-                            self._replan(P, tdot)
-                            self.nconverged = 0
-                            st.iter = 0
-                            self._chk(L.cg_rollback(h, C.byref(P)), "cg_rollback")
-                            rollbacks += 1
-                    elif did_converge:
-                        tdot = P.tdot
-                        nsub, kin = self._kin(tdot)
-                        tdot = tdot / nsub
-                        if kin[3] <= 1.0e-3:
-                            tdot = max(P.tdotmin, min(tdot, 0.5 * P.tdot))
-                        else:
-                            tdot = max(P.tdotmin, tdot)
-                        nsub = math.ceil(P.tdot / tdot)
-                        if nsub > 1:
-                            self._replan(P, tdot)
-                            did_converge = False
-                            self.nconverged = 0
-                            st.iter = 0
-                            st.mix = 1.0
-                            self._chk(L.cg_rollback(h, C.byref(P)), "cg_rollback")
-                            rollbacks += 1
-                elif st.iter == P.itmax:
-                    accept = True
+                    else:
+                        tdot = max(P.tdotmin, tdot)
+                    nsub = math.ceil(P.tdot / tdot)
+                    if nsub > 1:
+                        self._replan(P, tdot)
+                        did_converge = False
+                        self.nconverged = 0
+                        st.iter = 0
+                        st.mix = 1.0
+                        self._chk(L.cg_rollback(h, C.byref(P)), "cg_rollback")
+                        self.rollbacks += 1
+            elif st.iter == P.itmax:
+                self.accept = True
+        self.did_converge = did_converge
+        return did_converge or self.accept
+
+    def finish(self):
+        """The accepted increment: updates, averages, diagnostics, Delta-t ramp, calc_c0
+        (LS-EVPFFT.f90:431-589); returns the row."""
+        L, h, P, st, out = self.L, self.h, self.P, self.st, self.out
+        did_converge = self.did_converge
+        passes, rollbacks, nr, ev, ou = self.passes, self.rollbacks, self.nr, self.ev, self.ou
         last_iter = st.iter
         inc = _lib.CgIncrOut()
         self._chk(L.cg_end_increment(h, C.byref(P), int(self.scan_cond), C.byref(inc)), "cg_end_increment")

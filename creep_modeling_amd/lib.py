@@ -21,7 +21,8 @@ EXPORTS = (
     "cg_create", "cg_destroy", "cg_last_error", "cg_set_phase", "cg_set_voxels", "cg_init",
     "cg_begin_increment", "cg_iterate", "cg_rollback", "cg_kinematic_substep", "cg_end_increment",
     "cg_calc_c0", "cg_field_ncomp", "cg_get_field", "cg_set_field", "cg_get_macro", "cg_get_stats",
-    "cg_set_timing", "cg_kinematic_substep_noalloc", "cg_set_thermo",
+    "cg_set_timing", "cg_kinematic_substep_noalloc", "cg_set_thermo", "cg_batch_create", "cg_batch_destroy",
+    "cg_batch_iterate", "cg_batch_launch", "cg_batch_wait",
 )
 
 
@@ -123,6 +124,13 @@ def load():
     L.cg_set_timing.argtypes = [vp, C.c_int]
     L.cg_kinematic_substep_noalloc.argtypes = [C.c_long, vp, C.c_long, vp, C.c_double, vp, vp]
     L.cg_set_thermo.argtypes = [vp, _D]
+    L.cg_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
+    L.cg_batch_destroy.argtypes = [vp]
+    L.cg_batch_iterate.argtypes = [vp, C.POINTER(C.POINTER(CgProcess)), C.POINTER(CgIterState), C.POINTER(CgIterOut),
+                                   _I, _I]
+    L.cg_batch_launch.argtypes = [vp, C.POINTER(C.POINTER(CgProcess)), C.POINTER(CgIterState), C.POINTER(CgIterOut),
+                                  _I]
+    L.cg_batch_wait.argtypes = [vp, _I]
     _lib = L
     return L
 

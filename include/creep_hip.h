@@ -116,6 +116,27 @@ int cg_begin_increment(cg_ctx *ctx, const cg_process *p, int do_schmid);
  * calc_velgradavg_corr, error normalisation).  Updates st->iter, st->mix and
  * p->dsim / p->scauchy. */
 int cg_iterate(cg_ctx *ctx, cg_process *p, cg_iter_state *st, cg_iter_out *out);
+/* Several RVEs in one constitutive launch.  The reference runs a sweep or a calibration as
+ * independent solver processes (utils/calibrate.py:1045-1133, utils/sweep_microstructures.py:24-29);
+ * here the RVEs' contexts are grouped and cg_batch_iterate runs one fixed-point pass of every
+ * active RVE: each RVE's spectral stage on its own stream, then one launch of the correction +
+ * constitutive kernel over a per-RVE work table (one per kernel instance present), then each
+ * RVE's reductions.  Per RVE the result is exactly that of cg_iterate (same kernels, same
+ * blocks, same reduction order); rc[i] is cg_iterate's return code for RVE i, st/out/p as
+ * there.  RVEs whose increment or run is over are left out through active[i] = 0 (they retire
+ * from the work table).  All contexts must live on the current device.  Returns <0 on a
+ * runtime error (message in cg_last_error(NULL)). */
+typedef struct cg_batch cg_batch;
+int cg_batch_create(cg_ctx *const *ctxs, int n, cg_batch **out);
+int cg_batch_destroy(cg_batch *b);
+int cg_batch_iterate(cg_batch *b, cg_process *const *p, cg_iter_state *st, cg_iter_out *out, const int *active,
+                     int *rc);
+/* cg_batch_iterate in two halves: cg_batch_launch enqueues the pass and returns at once (p, st
+ * and out must stay valid until the wait); cg_batch_wait waits for it and does the host part,
+ * filling rc.  Two batches launched in turn keep the GPU busy while the host works on the
+ * other one's results. */
+int cg_batch_launch(cg_batch *b, cg_process *const *p, cg_iter_state *st, cg_iter_out *out, const int *active);
+int cg_batch_wait(cg_batch *b, int *rc);
 /* rollback (various_functions.f90:2868-2895), restoring p->dsim / p->scauchy. */
 int cg_rollback(cg_ctx *ctx, cg_process *p);
 /* kinematic_increments + kinematic_substep (various_functions.f90:2745-2837):

@@ -1,26 +1,45 @@
 """Independent RVEs driven together (creep_modeling_amd.batch) compute exactly what each
-computes alone: contexts share nothing but the GPU."""
+computes alone: in the shared constitutive launch over the per-RVE work table (one launch for
+all RVEs of a round, RVEs retiring as their runs end) and in the thread-pool engine."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-def test_batch_matches_single_runs():
+@pytest.mark.parametrize("engine", ["shared", "threads"])
+def test_batch_matches_single_runs(engine):
     from creep_modeling_amd import batch, driver, inputs, parse
 
     cases = inputs.sweep_cases(4, nsteps=2, sizes=(16, 24))
     problems = [parse.problem_from_case(*c) for _, c in cases]
-    print("batch of", [p.n for p in problems], flush=True)
+    # uneven run lengths: RVE 1 stops after one increment and leaves the work table early
+    steps = [2, 1, 2, 2]
+    print("batch of", [p.n for p in problems], engine, flush=True)
     single = []
-    for p in problems:
+    for p, k in zip(problems, steps):
         s = driver.Solver(p)
-        single.append(s.run())
+        single.append(s.run(max_steps=k))
         s.close()
-    batched = batch.run_batch(problems, workers=4)
-    for a, b in zip(single, batched):
-        assert len(a) == len(b) == 2
+    fields = {}
+
+    def keep(i, s):
+        fields[i] = s.ctx.field("sg")
+
+    if engine == "shared":
+        solvers = [driver.Solver(p) for p in problems]
+        for s, k in zip(solvers, steps):  # per-RVE run lengths through the process tables
+            s.procs[0].nsteps = k
+        batched = batch.run_batch(problems, solvers=solvers, on_done=keep)
+    else:
+        solvers = [driver.Solver(p) for p in problems]
+        for s, k in zip(solvers, steps):
+            s.procs[0].nsteps = k
+        batched = batch.run_batch(problems, workers=4, solvers=solvers, on_done=keep, engine="threads")
+    for a, b, k in zip(single, batched, steps):
+        assert len(a) == len(b) == k
         for ra, rb in zip(a, b):
             assert ra["passes"] == rb["passes"]
-            for k in ("sgsolavg", "edotpavg", "eavg", "velgradavg"):
-                assert np.array_equal(np.asarray(ra[k]), np.asarray(rb[k])), k
+            for key in ("sgsolavg", "edotpavg", "eavg", "velgradavg", "eelavg"):
+                assert np.array_equal(np.asarray(ra[key]), np.asarray(rb[key])), key
+    assert sorted(fields) == [0, 1, 2, 3]

@@ -63,7 +63,7 @@ def test_run_batch_order_and_pool(monkeypatch):
     monkeypatch.setattr(driver, "Solver", _FakeSolver)
     _FakeSolver.peak = 0
     probs = [_P(i, nvox=(i * 7) % 5 + 1) for i in range(9)]
-    out = batch.run_batch(probs, steps=3, workers=3)
+    out = batch.run_batch(probs, steps=3, workers=3, engine="threads")
     assert [r[0]["passes"] for r in out] == list(range(9))  # input order
     assert all(len(r) == 3 for r in out)
     assert _FakeSolver.peak <= 3 and _FakeSolver.live == 0
@@ -75,5 +75,5 @@ def test_run_batch_reports_failure(monkeypatch):
     monkeypatch.setattr(driver, "Solver", _FakeSolver)
     probs = [_P(0, 1), _P(1, 1, fail=True), _P(2, 1)]
     with pytest.raises(RuntimeError, match="RVE 1"):
-        batch.run_batch(probs, workers=2)
+        batch.run_batch(probs, workers=2, engine="threads")
     assert _FakeSolver.live == 0
