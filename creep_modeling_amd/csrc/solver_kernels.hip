@@ -52,7 +52,7 @@ __device__ __forceinline__ void flag_singular(const Fields& f, long v) {
     f.red[R_FLAG_SING] = 1.0;
 }
 
-// |x|^(n-1) for an integer n (exponentiation by squaring) or through pow().
+// |x|^(n-1) for an integer n (exponentiation by squaring) or through xpow (exp(y ln x)).
 template <bool INTPOW>
 __device__ __forceinline__ double powm1(double x, double n) {
     if (INTPOW) {
@@ -65,7 +65,7 @@ __device__ __forceinline__ double powm1(double x, double n) {
         }
         return r;
     } else {
-        return pow(x, n - 1.0);
+        return xpow(x, n - 1.0);
     }
 }
 
@@ -325,6 +325,9 @@ __device__ __forceinline__ void slip_eval_ab(const Fields& f, TB tab, const PH& 
                 dd[12] += s3 * t3; dd[13] += s3 * t4;
                 dd[14] += s4 * t4;
             }
+            // one system at a time: interleaving the unrolled systems raises the register
+            // pressure past the two-wave budget (spills), and gains nothing at two waves per SIMD
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 }

@@ -39,6 +39,64 @@ constexpr double R6 = 0.40824829046386301637;   // 1/sqrt6
 constexpr double R23 = 0.81649658092772603273;  // 2/sqrt6
 constexpr double PI_D = 3.14159265358979323846;
 
+// x^y for x >= 0 and a finite y > 0 (the power law's |rss/tau|^(n-1) with a non-integer n), as
+// exp(y ln x) in FP64 with no table and no double-double steps (about 45 instructions; the
+// library pow keeps extra precision and spills registers in the constitutive kernel):
+//   ln x:  x = m 2^e with m in [1/sqrt2, sqrt2), ln m = 2 atanh(s), s = (m-1)/(m+1),
+//          |s| <= 0.1716, by its Taylor series through s^23 (the next term is below 2^-60);
+//   exp t: t = k ln2 + r, |r| <= ln2/2, k integer, ln2 in two parts (the first with 11 trailing
+//          zero bits, so k ln2_hi is exact), e^r by its Taylor series through r^13/13! (the
+//          next term is below 2^-57), then scaled by 2^k.
+// The result is within a few ulp of x^y for |y ln x| up to ~40 (relative error ~ |y ln x| eps);
+// x = 0 gives 0, y ln x < -745.2 gives 0.
+CG_HD double xpow(double x, double y) {
+    if (!(x > 0.0)) return x == 0.0 ? 0.0 : x;  // 0 -> 0; NaN propagates
+    int e;
+    double m = frexp(x, &e);  // m in [0.5, 1)
+    if (m < 0.70710678118654752440) {
+        m *= 2.0;
+        e -= 1;
+    }
+    const double s = (m - 1.0) * recip(m + 1.0);
+    const double z = s * s;
+    double p = 1.0 / 23.0;
+    p = fma(p, z, 1.0 / 21.0);
+    p = fma(p, z, 1.0 / 19.0);
+    p = fma(p, z, 1.0 / 17.0);
+    p = fma(p, z, 1.0 / 15.0);
+    p = fma(p, z, 1.0 / 13.0);
+    p = fma(p, z, 1.0 / 11.0);
+    p = fma(p, z, 1.0 / 9.0);
+    p = fma(p, z, 1.0 / 7.0);
+    p = fma(p, z, 1.0 / 5.0);
+    p = fma(p, z, 1.0 / 3.0);
+    const double lnm = 2.0 * s + 2.0 * s * (z * p);
+    constexpr double LN2 = 0.69314718055994530942;
+    constexpr double LN2_HI = 6.93147180369123816490e-01;  // ln2 rounded to 42 significant bits
+    constexpr double LN2_LO = 1.90821492927058770002e-10;  // ln2 - LN2_HI
+    const double t = y * fma((double)e, LN2, lnm);
+    if (t < -745.2) return 0.0;
+    if (t > 709.8) return HUGE_VAL;
+    const double k = rint(t * (1.0 / LN2));
+    double r = fma(-k, LN2_HI, t);
+    r = fma(-k, LN2_LO, r);
+    double q = 1.0 / 6227020800.0;  // 1/13!
+    q = fma(q, r, 1.0 / 479001600.0);
+    q = fma(q, r, 1.0 / 39916800.0);
+    q = fma(q, r, 1.0 / 3628800.0);
+    q = fma(q, r, 1.0 / 362880.0);
+    q = fma(q, r, 1.0 / 40320.0);
+    q = fma(q, r, 1.0 / 5040.0);
+    q = fma(q, r, 1.0 / 720.0);
+    q = fma(q, r, 1.0 / 120.0);
+    q = fma(q, r, 1.0 / 24.0);
+    q = fma(q, r, 1.0 / 6.0);
+    q = fma(q, r, 0.5);
+    q = fma(q, r, 1.0);
+    q = fma(q, r, 1.0);
+    return ldexp(q, (int)k);
+}
+
 // ---- b-basis <-> 3x3 ------------------------------------------------------
 // 6-vector of the symmetric part of a general 3x3 tensor.
 CG_HD void to6(const double* a, double* s) {
