@@ -157,6 +157,10 @@ struct cg_ctx {
     int timing = 0;
     hipEvent_t ev[8];
     cg_stats st;
+    // FFT shape tuning: done at the first pass (or given by cg_set_fft_tuning), cached in the
+    // tuning file when one is set
+    int fft_tuned = 0;
+    std::string tune_path;
 };
 
 static int fail(cg_ctx* c, int code, const char* fmt, ...) {
@@ -593,6 +597,77 @@ int cg_begin_increment(cg_ctx* c, const cg_process* p, int do_schmid) {
 
 }  // extern "C"
 
+// The FFT shape tuning of a context, done before its first pass.  The tuning file (the
+// reference's FFTW wisdom file, --fftw_save, fourier_functions.f90:35-66, 106-127) holds one
+// line per (device, grid, precision, Green-operator kind):
+//   creep_hip_fft_tune <arch> <CUs> <n1> <n2> <n3> <precision> <igamma>0> : <xcd x5> <yhalf x2> <kt1>
+// A matching line is used as it is; otherwise the shapes are timed and the line appended.
+static std::string tune_key(const cg_ctx* c, int igamma) {
+    hipDeviceProp_t pr;
+    int dev = 0;
+    hipGetDevice(&dev);
+    char arch[256] = "unknown";
+    int cus = 0;
+    if (hipGetDeviceProperties(&pr, dev) == hipSuccess) {
+        snprintf(arch, sizeof arch, "%s", pr.gcnArchName);
+        for (char* q = arch; *q; q++)
+            if (*q == ' ' || *q == ':') *q = '_';
+        cus = pr.multiProcessorCount;
+    }
+    char k[512];
+    snprintf(k, sizeof k, "creep_hip_fft_tune %s %d %d %d %d %d %d", arch, cus, c->n1, c->n2, c->n3, c->mixed,
+             igamma > 0 ? 1 : 0);
+    return k;
+}
+static bool tune_read(const std::string& path, const std::string& key, FftTune& t) {
+    FILE* fp = fopen(path.c_str(), "r");
+    if (!fp) return false;
+    char line[1024];
+    bool found = false;
+    while (!found && fgets(line, sizeof line, fp)) {
+        const size_t kl = key.size();
+        if (strncmp(line, key.c_str(), kl) != 0 || strncmp(line + kl, " : ", 3) != 0) continue;
+        FftTune u;
+        int n = sscanf(line + kl + 3, "%d %d %d %d %d %d %d %d", &u.xcd[0], &u.xcd[1], &u.xcd[2], &u.xcd[3],
+                       &u.xcd[4], &u.yhalf[0], &u.yhalf[1], &u.kt1);
+        if (n == 8) {
+            u.tuned = 1;
+            t = u;
+            found = true;
+        }
+    }
+    fclose(fp);
+    return found;
+}
+static void tune_write(const std::string& path, const std::string& key, const FftTune& t) {
+    FILE* fp = fopen(path.c_str(), "a");
+    if (!fp) return;  // an unwritable cache only costs the tuning of the next run
+    fprintf(fp, "%s : %d %d %d %d %d %d %d %d\n", key.c_str(), t.xcd[0], t.xcd[1], t.xcd[2], t.xcd[3], t.xcd[4],
+            t.yhalf[0], t.yhalf[1], t.kt1);
+    fclose(fp);
+}
+static int ensure_fft_tuned(cg_ctx* c, const cg_process* p) {
+    if (c->fft_tuned) return 0;
+    c->fft_tuned = 1;
+    if (!fft_tunable(c->f, c->fft)) return 0;
+    const std::string key = tune_key(c, p->igamma);
+    FftTune t;
+    if (!c->tune_path.empty() && tune_read(c->tune_path, key, t)) {
+        fft_set_tune(c->fft, t);
+        return 0;
+    }
+    if (!c->sgsum_valid) {  // the x forward pass reads <sigma>
+        int nb;
+        launch_sigma_sum(c->f, c->s, &nb);
+        launch_reduce_final(c->f.part, nb, 9, c->d_sgsum, c->s);
+    }
+    const double scale = c->mixed ? std::max(c->svmavg, 1.0) : 1.0;
+    if (fft_tune(c->f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, scale, c->s) != 0)
+        return fail(c, -2, "FFT shape tuning failed");
+    if (!c->tune_path.empty()) tune_write(c->tune_path, key, fft_get_tune(c->fft));
+    return 0;
+}
+
 // One fixed-point pass is split in three so that several RVEs can share the constitutive
 // launch (cg_batch_iterate): pass_begin enqueues the work before it on the context's stream
 // and fills the solve's arguments; pass_reduce enqueues the reductions of the solve's sums
@@ -753,6 +828,7 @@ extern "C" {
 
 int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     if (!c || !c->initialised || !p || !it) return -1;
+    if (int e = ensure_fft_tuned(c, p)) return e;
     SolveArgs sa;
     pass_begin(c, p, it, sa);
     PhaseTables pt{c->d_ph, c->d_tab};
@@ -882,6 +958,10 @@ int cg_batch_launch(cg_batch* b, cg_process* const* p, cg_iter_state* st, cg_ite
             if (!p[i]) return noctx_fail(-1, "cg_batch_launch: no process for RVE %d", i);
             nact++;
         }
+    for (int i = 0; i < n; i++)
+        if (active[i])
+            if (int e = ensure_fft_tuned(b->ctx[i], p[i]))
+                return noctx_fail(e, "cg_batch_launch: RVE %d: %s", i, b->ctx[i]->err);
     b->pend_p.assign(p, p + n);
     b->pend_st = st;
     b->pend_out = out;
@@ -1296,6 +1376,40 @@ int cg_get_stats(cg_ctx* c, cg_stats* st, int reset) {
 int cg_set_timing(cg_ctx* c, int on) {
     if (!c) return -1;
     c->timing = on ? 1 : 0;
+    return 0;
+}
+
+int cg_set_tuning_file(cg_ctx* c, const char* path) {
+    if (!c) return -1;
+    c->tune_path = path ? path : "";
+    return 0;
+}
+
+int cg_get_fft_tuning(cg_ctx* c, int* shape8, double* us5) {
+    if (!c || !c->fft) return -1;
+    const FftTune t = fft_get_tune(c->fft);
+    if (shape8) {
+        for (int k = 0; k < 5; k++) shape8[k] = t.xcd[k];
+        shape8[5] = t.yhalf[0];
+        shape8[6] = t.yhalf[1];
+        shape8[7] = t.kt1;
+    }
+    if (us5)
+        for (int k = 0; k < 5; k++) us5[k] = t.us[k];
+    return t.tuned;
+}
+
+int cg_set_fft_tuning(cg_ctx* c, const int* shape8) {
+    if (!c || !c->fft || !shape8) return -1;
+    FftTune t;
+    for (int k = 0; k < 8; k++)
+        if (shape8[k] != 0 && shape8[k] != 1) return fail(c, -1, "cg_set_fft_tuning: entries are 0 or 1");
+    for (int k = 0; k < 5; k++) t.xcd[k] = shape8[k];
+    t.yhalf[0] = shape8[5];
+    t.yhalf[1] = shape8[6];
+    t.kt1 = shape8[7];
+    fft_set_tune(c->fft, t);
+    c->fft_tuned = 1;  // given, not timed
     return 0;
 }
 

@@ -473,6 +473,7 @@ struct FftCtx {
     float2* tw32;
     int rows, ty, tz;
     int use_reg;  // register-FFT kernels for the per-pass stage (all axis lengths supported)
+    FftTune tune;  // block shapes and placement of the register passes
 };
 
 FftCtx* fft_create(const Fields& f, char* err, int errlen) {
@@ -549,7 +550,12 @@ static int run_green(const Fields& f, FftCtx* c, const double* sgsum, const Gree
 bool fft_reg_supported(const Fields& f);
 template <typename R>
 int run_green_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
-                  const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, double scale, hipStream_t s);
+                  const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, double scale,
+                  const FftTune& t, hipStream_t s);
+template <typename R>
+int tune_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
+             const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, double scale, FftTune& t,
+             hipStream_t s);
 
 int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
               int mixed, double scale, hipStream_t s) {
@@ -560,13 +566,30 @@ int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double*
         const Plan3& pl = c->plan;
         if (mixed)
             return run_green_reg<float>(f, c->tw32 + pl.ax[0].tw_off, c->tw32 + pl.ax[1].tw_off,
-                                        c->tw32 + pl.ax[2].tw_off, sgsum_dev, g, scale, s);
+                                        c->tw32 + pl.ax[2].tw_off, sgsum_dev, g, scale, c->tune, s);
         return run_green_reg<double>(f, c->tw64 + pl.ax[0].tw_off, c->tw64 + pl.ax[1].tw_off,
-                                     c->tw64 + pl.ax[2].tw_off, sgsum_dev, g, 1.0, s);
+                                     c->tw64 + pl.ax[2].tw_off, sgsum_dev, g, 1.0, c->tune, s);
     }
     if (mixed) return run_green<float>(f, c, sgsum_dev, g, scale, s);
     return run_green<double>(f, c, sgsum_dev, g, 1.0, s);
 }
+
+int fft_tune(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
+             int mixed, double scale, hipStream_t s) {
+    if (!(c->use_reg && fft_reg_supported(f))) return 0;  // the LDS kernels have no shapes to pick
+    GreenDev g;
+    for (int i = 0; i < 81; i++) { g.c0[i] = c0[i]; g.s0[i] = s0[i]; }
+    g.igamma = igamma;
+    const Plan3& pl = c->plan;
+    if (mixed)
+        return tune_reg<float>(f, c->tw32 + pl.ax[0].tw_off, c->tw32 + pl.ax[1].tw_off, c->tw32 + pl.ax[2].tw_off,
+                               sgsum_dev, g, scale, c->tune, s);
+    return tune_reg<double>(f, c->tw64 + pl.ax[0].tw_off, c->tw64 + pl.ax[1].tw_off, c->tw64 + pl.ax[2].tw_off,
+                            sgsum_dev, g, 1.0, c->tune, s);
+}
+FftTune fft_get_tune(const FftCtx* c) { return c->tune; }
+void fft_set_tune(FftCtx* c, const FftTune& t) { c->tune = t; }
+bool fft_tunable(const Fields& f, const FftCtx* c) { return c->use_reg && fft_reg_supported(f); }
 
 }  // namespace cg
 
@@ -746,19 +769,29 @@ int fft_integrate(const Fields& f, FftCtx* c, int ideriv, double* dv, hipStream_
 
 namespace cg {
 
-constexpr int YBS = 256;  // threads per block: y passes
+constexpr int YBS = 256;  // threads per block of the y passes (the tuner may halve it)
 
 
 template <int N> struct RegDims {
     static constexpr int T = Shape<N>::T, E = N / T, U = (E + T - 1) / T;
     static constexpr int AN = U * T > E ? U * T : E;  // register slots per thread
-    static constexpr int NS = YBS / T;                 // signals per y block
+    static constexpr int NS = YBS / T;                 // signals per y block of YBS threads
     // z pass: KT whole kx columns of 9 components per block, 24 signals' worth of threads
     // (216) for T = 4 and 8, one column (144 threads) for T = 16
     static constexpr int KT = T >= 16 ? 1 : 24 / T;    // kx columns per z block
     static constexpr int NSZ = 9 * KT;                 // signals per z block
     static constexpr int ZB = NSZ * T;                 // threads per z block
     static_assert(KT >= 1, "z block shape");
+};
+// y pass with YB threads per block (YBS or YBS / 2)
+template <int N, int YB> struct YDims {
+    static constexpr int T = Shape<N>::T, NS = YB / T;
+};
+// z pass with KT kx columns per block, or one column (KT1)
+template <int N, bool KT1> struct ZDims {
+    static constexpr int T = Shape<N>::T;
+    static constexpr int KT = KT1 ? 1 : RegDims<N>::KT;
+    static constexpr int NSZ = 9 * KT, ZB = NSZ * T;
 };
 // x passes: XB threads per block, RW rows of 9 components (5 signals a row).  FP64 rows run
 // in one-wave blocks (more blocks per CU, and their barriers hold one wave); FP32 rows and
@@ -771,21 +804,16 @@ template <int N, typename R> struct XDims {
     static_assert(RW >= 1, "x block shape");
 };
 
-// Block b of a launch runs on XCD b % 8.  The y and z passes give each XCD a contiguous eighth
-// of their tiles, so neighbouring tiles (which share the 128-byte lines at their edges) meet in
-// the same L2 (3% off the FP64 stage at 128^3, 5% off the FP32 one).  CG_FFT_XCD=0: in order.
-#ifndef CG_FFT_XCD
-#define CG_FFT_XCD 1
-#endif
+// Block b of a launch runs on XCD b % 8.  With xcd set, a pass gives each XCD a contiguous eighth
+// of its tiles or rows, so neighbouring tiles (which share the 128-byte lines at their edges) meet
+// in the same L2 (3% off the FP64 stage at 128^3, 5% off the FP32 one); else tiles go in launch
+// order.  The tuner picks it per pass; the results do not depend on it.
 #ifndef CG_XPAD
 #define CG_XPAD 1
 #endif
-#ifndef CG_XPASS_XCD
-#define CG_XPASS_XCD 1  // the x passes too (their spectrum rows share edge lines; 2% off the FP32 stage)
-#endif
-__device__ __forceinline__ int fft_block() {
+__device__ __forceinline__ int fft_block(int xcd) {
     const int b = blockIdx.x, nb = gridDim.x;
-    if (!CG_FFT_XCD) return b;
+    if (!xcd) return b;
     const int q = nb / 8;
     return b < 8 * q ? (b % 8) * q + b / 8 : b;
 }
@@ -799,7 +827,7 @@ template <typename R> __device__ __forceinline__ typename Cx<R>::T* spec_of(cons
 // separated with X[N-k] afterwards.
 template <int N, typename R>
 __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
-                                                   const double* __restrict__ sgsum, double inv_scale) {
+                                                   const double* __restrict__ sgsum, double inv_scale, int xcd) {
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
     using X = XDims<N, R>;
@@ -812,7 +840,8 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_reg(Fields f, const
     static_assert(RW * 9 * SP * sizeof(R) <= N * NS * sizeof(T2), "stage fits the exchange");
     const long V = f.V;
     const long nrows = (long)f.n2 * f.n3;
-    const long row0 = (long)(CG_XPASS_XCD ? fft_block() : (int)blockIdx.x) * RW;
+    const int lb = fft_block(xcd);  // the block's rows (and, in the x inverse, its partial-sum slot)
+    const long row0 = (long)lb * RW;
     double sa[9];
     for (int c = 0; c < 9; c++) sa[c] = sgsum[c];
     // each thread's voxels: every load is issued before the products (rows past the grid
@@ -885,13 +914,13 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_reg(Fields f, const
 }
 
 // y pass over every (kx, c) column of one z plane
-template <int N, bool INV, typename R>
-__global__ __launch_bounds__(YBS) void k_y_reg(Fields f, const typename Cx<R>::T* __restrict__ tw, int ntile) {
+template <int N, bool INV, typename R, int YB>
+__global__ __launch_bounds__(YB) void k_y_reg(Fields f, const typename Cx<R>::T* __restrict__ tw, int ntile, int xcd) {
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
-    constexpr int T = D::T, E = D::E, U = D::U, NS = D::NS;
+    constexpr int T = D::T, E = D::E, U = D::U, NS = YDims<N, YB>::NS;
     __shared__ T2 ex[N * NS];
-    const int bid = fft_block();
+    const int bid = fft_block(xcd);
     const int k = bid / ntile, tile = bid - k * ntile;
     const int ncol = f.h1 * 9;
     const int s = threadIdx.x % NS, t = threadIdx.x / NS;
@@ -913,15 +942,16 @@ __global__ __launch_bounds__(YBS) void k_y_reg(Fields f, const typename Cx<R>::T
 }
 
 // z pass with the Green operator: KT whole kx columns (9 components each) of one y row
-template <int N, typename R, bool GENERAL>
-__global__ __launch_bounds__(RegDims<N>::ZB) void k_z_green_reg(Fields f, const typename Cx<R>::T* __restrict__ tw, GreenDev g,
-                                                     int ntile) {
+template <int N, typename R, bool GENERAL, bool KT1>
+__global__ __launch_bounds__((ZDims<N, KT1>::ZB)) void k_z_green_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
+                                                                    GreenDev g, int ntile, int xcd) {
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
-    constexpr int T = D::T, E = D::E, U = D::U, NS = D::NSZ, KT = D::KT;
+    using Z = ZDims<N, KT1>;
+    constexpr int T = D::T, E = D::E, U = D::U, NS = Z::NSZ, KT = Z::KT;
     __shared__ T2 ex[N * NS];
     const int h1 = f.h1;
-    const int bid = fft_block();
+    const int bid = fft_block(xcd);
     const int j = bid / ntile, tile = bid - j * ntile;
     const int kx0 = tile * KT;
     const int KTe = min(KT, h1 - kx0);
@@ -942,7 +972,7 @@ __global__ __launch_bounds__(RegDims<N>::ZB) void k_z_green_reg(Fields f, const 
             for (int jj = 0; jj < T; jj++) ex[(q + E * jj) * NS + s] = a[u * T + jj];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < KTe * N; e += D::ZB) {
+    for (int e = threadIdx.x; e < KTe * N; e += Z::ZB) {
         const int kz = e / KTe, kxl = e - kz * KTe;
         green_apply<R, GENERAL>(g, kx0 + kxl, j, kz, f.n1, f.n2, N, ex + kz * NS + kxl * 9, 1);
     }
@@ -966,7 +996,7 @@ __global__ __launch_bounds__(RegDims<N>::ZB) void k_z_green_reg(Fields f, const 
 // c2c inverse does); block partial sums of w dLref F^-1.
 template <int N, typename R>
 __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
-                                                   double out_scale) {
+                                                   double out_scale, int xcd) {
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
     using X = XDims<N, R>;
@@ -978,7 +1008,8 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_reg(Fields f, const
     static_assert(RW * 9 * SP * sizeof(R) <= N * NS * sizeof(T2), "output rows fit the exchange");
     const long V = f.V;
     const long nrows = (long)f.n2 * f.n3;
-    const long row0 = (long)(CG_XPASS_XCD ? fft_block() : (int)blockIdx.x) * RW;
+    const int lb = fft_block(xcd);  // the block's rows (and, in the x inverse, its partial-sum slot)
+    const long row0 = (long)lb * RW;
     constexpr int w = H * 9;
     const T2* spec = spec_of<R>(f);
     // the block's spectrum rows: every load is issued before the first is written to LDS
@@ -1063,36 +1094,58 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_reg(Fields f, const
         mm3(d, Fp[q], q9);
         for (int c = 0; c < 9; c++) acc[c] += q9[c] * wp[q];
     }
-    block_reduce_store_cm<9, XBS>(acc, f.part, 0);
+    // partials in row order whatever the block placement, so the reduced sums do not depend on it
+    block_reduce_store_cm<9, XBS>(acc, f.part, 0, lb, gridDim.x);
 }
 
 static bool reg_size(int n) { return n == 16 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128 || n == 256; }
 bool fft_reg_supported(const Fields& f) { return reg_size(f.n1) && reg_size(f.n2) && reg_size(f.n3); }
 
 template <int N, typename R> static void launch_x_fwd_reg(const Fields& f, const typename Cx<R>::T* tw, const double* sgsum,
-                                                          double inv_scale, hipStream_t s) {
+                                                          double inv_scale, int xcd, hipStream_t s) {
     const long nrows = (long)f.n2 * f.n3;
     using X = XDims<N, R>;
     const int nb = (int)((nrows + X::RW - 1) / X::RW);
-    hipLaunchKernelGGL((k_x_fwd_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, f, tw, sgsum, inv_scale);
+    hipLaunchKernelGGL((k_x_fwd_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, f, tw, sgsum, inv_scale, xcd);
 }
 template <int N, typename R> static int launch_x_inv_reg(const Fields& f, const typename Cx<R>::T* tw, double out_scale,
-                                                         hipStream_t s) {
+                                                         int xcd, hipStream_t s) {
     const long nrows = (long)f.n2 * f.n3;
     using X = XDims<N, R>;
     const int nb = (int)((nrows + X::RW - 1) / X::RW);
-    hipLaunchKernelGGL((k_x_inv_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, f, tw, out_scale);
+    hipLaunchKernelGGL((k_x_inv_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, f, tw, out_scale, xcd);
     return nb;
 }
-template <int N, bool INV, typename R> static void launch_y_reg(const Fields& f, const typename Cx<R>::T* tw, hipStream_t s) {
-    const int ntile = (f.h1 * 9 + RegDims<N>::NS - 1) / RegDims<N>::NS;
-    hipLaunchKernelGGL((k_y_reg<N, INV, R>), dim3(f.n3 * ntile), dim3(YBS), 0, s, f, tw, ntile);
+template <int N, bool INV, typename R, int YB>
+static void launch_y_reg_b(const Fields& f, const typename Cx<R>::T* tw, int xcd, hipStream_t s) {
+    constexpr int NS = YDims<N, YB>::NS;
+    const int ntile = (f.h1 * 9 + NS - 1) / NS;
+    hipLaunchKernelGGL((k_y_reg<N, INV, R, YB>), dim3(f.n3 * ntile), dim3(YB), 0, s, f, tw, ntile, xcd);
 }
-template <int N, typename R> static void launch_z_reg(const Fields& f, const typename Cx<R>::T* tw, const GreenDev& g,
-                                                      hipStream_t s) {
-    const int ntile = (f.h1 + RegDims<N>::KT - 1) / RegDims<N>::KT;
-    if (g.igamma == 0) hipLaunchKernelGGL((k_z_green_reg<N, R, false>), dim3(f.n2 * ntile), dim3(RegDims<N>::ZB), 0, s, f, tw, g, ntile);
-    else hipLaunchKernelGGL((k_z_green_reg<N, R, true>), dim3(f.n2 * ntile), dim3(RegDims<N>::ZB), 0, s, f, tw, g, ntile);
+template <int N, bool INV, typename R>
+static void launch_y_reg(const Fields& f, const typename Cx<R>::T* tw, int half, int xcd, hipStream_t s) {
+    if (half && YBS / 2 >= 2 * Shape<N>::T) launch_y_reg_b<N, INV, R, YBS / 2>(f, tw, xcd, s);
+    else launch_y_reg_b<N, INV, R, YBS>(f, tw, xcd, s);
+}
+template <int N, typename R, bool KT1>
+static void launch_z_reg_b(const Fields& f, const typename Cx<R>::T* tw, const GreenDev& g, int xcd, hipStream_t s) {
+    using Z = ZDims<N, KT1>;
+    const int ntile = (f.h1 + Z::KT - 1) / Z::KT;
+    if (g.igamma == 0)
+        hipLaunchKernelGGL((k_z_green_reg<N, R, false, KT1>), dim3(f.n2 * ntile), dim3(Z::ZB), 0, s, f, tw, g, ntile, xcd);
+    else
+        hipLaunchKernelGGL((k_z_green_reg<N, R, true, KT1>), dim3(f.n2 * ntile), dim3(Z::ZB), 0, s, f, tw, g, ntile, xcd);
+}
+template <int N, typename R>
+static void launch_z_reg(const Fields& f, const typename Cx<R>::T* tw, const GreenDev& g, int kt1, int xcd,
+                         hipStream_t s) {
+    if constexpr (RegDims<N>::KT > 1) {
+        if (kt1) {
+            launch_z_reg_b<N, R, true>(f, tw, g, xcd, s);
+            return;
+        }
+    }
+    launch_z_reg_b<N, R, false>(f, tw, g, xcd, s);
 }
 
 #define CG_SWITCH_N(n, CALL)                                             \
@@ -1107,20 +1160,25 @@ template <int N, typename R> static void launch_z_reg(const Fields& f, const typ
         default: break;                                                  \
     }
 
+// One stage (0 x forward, 1 y forward, 2 z + Gamma, 3 y inverse, 4 x inverse) of the register
+// form with the block shape and placement of t; returns the x inverse's block count.
 template <typename R>
-int run_green_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
-                  const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, double scale, hipStream_t s) {
+static int reg_stage(int stage, const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
+                     const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, double scale,
+                     const FftTune& t, hipStream_t s) {
     int nb = -1;
-#define CG_XF(N) launch_x_fwd_reg<N, R>(f, tw1, sgsum, 1.0 / scale, s)
-#define CG_YF(N) launch_y_reg<N, false, R>(f, tw2, s)
-#define CG_Z(N) launch_z_reg<N, R>(f, tw3, g, s)
-#define CG_YI(N) launch_y_reg<N, true, R>(f, tw2, s)
-#define CG_XI(N) nb = launch_x_inv_reg<N, R>(f, tw1, scale / (double)f.V, s)
-    CG_SWITCH_N(f.n1, CG_XF)
-    CG_SWITCH_N(f.n2, CG_YF)
-    CG_SWITCH_N(f.n3, CG_Z)
-    CG_SWITCH_N(f.n2, CG_YI)
-    CG_SWITCH_N(f.n1, CG_XI)
+#define CG_XF(N) launch_x_fwd_reg<N, R>(f, tw1, sgsum, 1.0 / scale, t.xcd[0], s)
+#define CG_YF(N) launch_y_reg<N, false, R>(f, tw2, t.yhalf[0], t.xcd[1], s)
+#define CG_Z(N) launch_z_reg<N, R>(f, tw3, g, t.kt1, t.xcd[2], s)
+#define CG_YI(N) launch_y_reg<N, true, R>(f, tw2, t.yhalf[1], t.xcd[3], s)
+#define CG_XI(N) nb = launch_x_inv_reg<N, R>(f, tw1, scale / (double)f.V, t.xcd[4], s)
+    switch (stage) {
+        case 0: CG_SWITCH_N(f.n1, CG_XF) break;
+        case 1: CG_SWITCH_N(f.n2, CG_YF) break;
+        case 2: CG_SWITCH_N(f.n3, CG_Z) break;
+        case 3: CG_SWITCH_N(f.n2, CG_YI) break;
+        default: CG_SWITCH_N(f.n1, CG_XI) break;
+    }
 #undef CG_XF
 #undef CG_YF
 #undef CG_Z
@@ -1128,9 +1186,70 @@ int run_green_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename 
 #undef CG_XI
     return nb;
 }
+
+template <typename R>
+int run_green_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
+                  const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, double scale,
+                  const FftTune& t, hipStream_t s) {
+    int nb = -1;
+    for (int st = 0; st < 5; st++) nb = reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, scale, t, s);
+    return nb;
+}
+
+// The autotuner: each stage's candidate shapes (block placement on or off; the y passes with
+// 256 or 128 threads a block; the z pass with its default kx columns or one) timed on the
+// context's own stream, two warm-up and five timed launches each, the fastest kept.  Every
+// candidate computes the same values in the same order, so the choice changes time only.
+template <typename R>
+int tune_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
+                    const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, double scale, FftTune& t,
+                    hipStream_t s) {
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return -1;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return -1;
+    }
+    int rc = 0;
+    for (int st = 0; st < 5 && !rc; st++) {
+        const int nshape = (st == 1 || st == 3 || st == 2) ? 2 : 1;
+        float best = 1e30f;
+        FftTune keep = t;
+        for (int sh = 0; sh < nshape && !rc; sh++)
+            for (int x = 1; x >= 0 && !rc; x--) {
+                FftTune c = keep;
+                c.xcd[st] = x;
+                if (st == 1) c.yhalf[0] = sh;
+                if (st == 3) c.yhalf[1] = sh;
+                if (st == 2) c.kt1 = sh;
+                for (int w = 0; w < 2; w++) reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, scale, c, s);
+                hipEventRecord(e0, s);
+                for (int w = 0; w < 5; w++) reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, scale, c, s);
+                hipEventRecord(e1, s);
+                float ms = 0.0f;
+                if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                    rc = -2;
+                    break;
+                }
+                if (ms < best) {
+                    best = ms;
+                    t = c;
+                }
+            }
+        t.us[st] = best * 1000.0f / 5.0f;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    t.tuned = rc ? 0 : 1;
+    return rc;
+}
 template int run_green_reg<double>(const Fields&, const double2*, const double2*, const double2*, const double*,
-                                   const GreenDev&, double, hipStream_t);
+                                   const GreenDev&, double, const FftTune&, hipStream_t);
 template int run_green_reg<float>(const Fields&, const float2*, const float2*, const float2*, const double*,
-                                  const GreenDev&, double, hipStream_t);
+                                  const GreenDev&, double, const FftTune&, hipStream_t);
+template int tune_reg<double>(const Fields&, const double2*, const double2*, const double2*, const double*,
+                              const GreenDev&, double, FftTune&, hipStream_t);
+template int tune_reg<float>(const Fields&, const float2*, const float2*, const float2*, const double*,
+                             const GreenDev&, double, FftTune&, hipStream_t);
 
 }  // namespace cg

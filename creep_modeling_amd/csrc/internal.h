@@ -145,12 +145,29 @@ void launch_xnode(const Fields& f, const double* dv, const double* lavg, double 
 
 // ---- fft_kernels.hip ----
 struct FftCtx;
+// Block shapes and placement of the register FFT passes, picked by the autotuner (stages:
+// 0 x forward, 1 y forward, 2 z + Gamma, 3 y inverse, 4 x inverse); none changes the results.
+struct FftTune {
+    int xcd[5] = {1, 1, 1, 1, 1};   // blocks placed so each XCD gets a contiguous share of the tiles
+    int yhalf[2] = {0, 0};          // y passes (forward, inverse) with half-size blocks
+    int kt1 = 0;                    // z pass with one kx column a block
+    int tuned = 0;                  // 1: picked by timing (or read from a tuning file)
+    float us[5] = {0, 0, 0, 0, 0};  // time per launch of the chosen shapes (microseconds)
+};
 FftCtx* fft_create(const Fields& f, char* err, int errlen);
 void fft_destroy(FftCtx* c);
 // PK1 -> Gamma convolution -> dLref; writes the 9 block partial sums of
 // w dLref F^-1 to f.part, column-major (part[i * nblocks + b]), and returns the number of blocks.
 int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
               int mixed, double scale, hipStream_t s);
+// Times the candidate shapes of each register pass on stream s (the passes overwrite the
+// spectrum, ΔL^ref and the block partials, which the next fft_green recomputes) and keeps the
+// fastest; 0 on success or when the lengths use the LDS kernels.
+int fft_tune(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
+             int mixed, double scale, hipStream_t s);
+FftTune fft_get_tune(const FftCtx* c);
+void fft_set_tune(FftCtx* c, const FftTune& t);
+bool fft_tunable(const Fields& f, const FftCtx* c);
 // spectral integration of L F for the nodal update: dv (3 x V) and 9 block partial
 // sums of L F / V into f.part; returns the number of blocks.
 int fft_integrate(const Fields& f, FftCtx* c, int ideriv, double* dv, hipStream_t s);

@@ -103,6 +103,11 @@ module creep_hip_iface
             real(c_double) :: out4(4)
             integer(c_int) :: nsub
         end function
+        integer(c_int) function cg_set_tuning_file(ctx, path) bind(C, name='cg_set_tuning_file')
+            import :: c_int, c_ptr, c_char
+            type(c_ptr), value :: ctx
+            character(kind=c_char) :: path(*)
+        end function
         integer(c_int) function cg_set_thermo(ctx, eth) bind(C, name='cg_set_thermo')
             import :: c_int, c_double, c_ptr
             type(c_ptr), value :: ctx

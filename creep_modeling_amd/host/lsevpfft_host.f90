@@ -88,7 +88,7 @@ program lsevpfft_host
             write (*, '(A)') '--in FILE | name of input file (fft.in by default)'
             write (*, '(A)') '--out DIR | output directory (evpfft_outputs by default)'
             write (*, '(A)') '--nthreads n | host threads (the field work runs on the GPU)'
-            write (*, '(A)') '--fftw_save FILE | accepted for compatibility (no FFT planning cache is needed)'
+            write (*, '(A)') '--fftw_save FILE | FFT shape tuning cache (read if it holds this grid, else written)'
             write (*, "(A)") '--modal 0-1 | toggle modified AL (default 1)'
             write (*, "(A)") '--mix 0-1 | velocity-gradient mixing (default 1)'
             write (*, "(A)") '--precision fp64|mixed | arithmetic of the FFT/Green stage (default fp64)'
@@ -126,6 +126,11 @@ program lsevpfft_host
     if (rc /= 0) call fatal('cg_set_voxels: '//trim(last_error(ctx)))
     rc = cg_init(ctx, inp%proc(1))
     if (rc /= 0) call fatal_rc('cg_init: '//trim(last_error(ctx)), rc)
+    ! the FFT shape tuning of the first pass is cached where the reference keeps its FFTW wisdom
+    if (len_trim(fftw_save_path) > 0) then
+        rc = cg_set_tuning_file(ctx, trim(fftw_save_path)//c_null_char)
+        if (rc /= 0) call fatal('cg_set_tuning_file: '//trim(last_error(ctx)))
+    end if
     if (have_thermo) call set_thermo()
     call write_vtk_cell(ctx, inp, 0, output_dir)
 

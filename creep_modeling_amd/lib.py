@@ -21,8 +21,8 @@ EXPORTS = (
     "cg_create", "cg_destroy", "cg_last_error", "cg_set_phase", "cg_set_voxels", "cg_init",
     "cg_begin_increment", "cg_iterate", "cg_rollback", "cg_kinematic_substep", "cg_end_increment",
     "cg_calc_c0", "cg_field_ncomp", "cg_get_field", "cg_set_field", "cg_get_macro", "cg_get_stats",
-    "cg_set_timing", "cg_kinematic_substep_noalloc", "cg_set_thermo", "cg_batch_create", "cg_batch_destroy",
-    "cg_batch_iterate", "cg_batch_launch", "cg_batch_wait",
+    "cg_set_timing", "cg_set_tuning_file", "cg_get_fft_tuning", "cg_set_fft_tuning", "cg_kinematic_substep_noalloc",
+    "cg_set_thermo", "cg_batch_create", "cg_batch_destroy", "cg_batch_iterate", "cg_batch_launch", "cg_batch_wait",
 )
 
 
@@ -122,6 +122,9 @@ def load():
     L.cg_get_macro.argtypes = [vp, _D, _D, _D, _D, _D]
     L.cg_get_stats.argtypes = [vp, C.POINTER(CgStats), C.c_int]
     L.cg_set_timing.argtypes = [vp, C.c_int]
+    L.cg_set_tuning_file.argtypes = [vp, C.c_char_p]
+    L.cg_get_fft_tuning.argtypes = [vp, _I, _D]
+    L.cg_set_fft_tuning.argtypes = [vp, _I]
     L.cg_kinematic_substep_noalloc.argtypes = [C.c_long, vp, C.c_long, vp, C.c_double, vp, vp]
     L.cg_set_thermo.argtypes = [vp, _D]
     L.cg_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
@@ -223,6 +226,24 @@ class Context:
         s = CgStats()
         self.L.cg_get_stats(self.h, C.byref(s), int(reset))
         return s
+
+    def set_tuning_file(self, path):
+        """FFT shape tuning cache (the reference's FFTW wisdom file); before the first pass."""
+        self._chk(self.L.cg_set_tuning_file(self.h, str(path).encode()), "cg_set_tuning_file")
+
+    def fft_tuning(self):
+        """(tuned, shapes[8], us[5]) of the FFT passes (see cg_get_fft_tuning)."""
+        sh = np.zeros(8, np.int32)
+        us = np.zeros(5)
+        rc = self.L.cg_get_fft_tuning(self.h, ip(sh), dp(us))
+        if rc < 0:
+            raise HipError("cg_get_fft_tuning rc=%d" % rc)
+        return bool(rc), sh.tolist(), us.tolist()
+
+    def set_fft_tuning(self, shapes):
+        """Fix the FFT shapes (8 entries of 0/1) and skip the tuning."""
+        sh = np.ascontiguousarray(shapes, np.int32)
+        self._chk(self.L.cg_set_fft_tuning(self.h, ip(sh)), "cg_set_fft_tuning")
 
     def close(self):
         if getattr(self, "h", None):

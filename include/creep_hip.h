@@ -192,6 +192,23 @@ int cg_get_stats(cg_ctx *ctx, cg_stats *st, int reset);
 /* Enable (1) / disable (0) the per-stage HIP-event timing (default off). */
 int cg_set_timing(cg_ctx *ctx, int on);
 
+/* FFT shape tuning.  Before a context's first pass the block shapes and block placement of its
+ * register FFT passes are timed on the device and the fastest kept (every candidate computes
+ * the same values in the same order).  Replaces the FFTW planning of init_fftw /
+ * init_fftw_wisdom_path (fourier_functions.f90:35-66, 68-194): with a tuning file (the path of
+ * --fftw_save), a line for the same device, grid, precision and Green-operator kind is read
+ * instead of timing, and a new result is appended.  Call before the first cg_iterate. */
+int cg_set_tuning_file(cg_ctx *ctx, const char *path);
+/* The shapes in use: shape8 = {x forward, y forward, z, y inverse, x inverse block placement
+ * (1 XCD-contiguous, 0 launch order), y forward half blocks, y inverse half blocks, z one kx
+ * column}; us5 = time per launch of each chosen pass (microseconds, 0 if not timed).  Returns 1
+ * if the shapes were tuned or read, 0 if they are the defaults, <0 on error.  No reference
+ * counterpart (inspection). */
+int cg_get_fft_tuning(cg_ctx *ctx, int *shape8, double *us5);
+/* Use the given shapes (entries 0 or 1, as cg_get_fft_tuning) and skip the tuning.  No reference
+ * counterpart. */
+int cg_set_fft_tuning(cg_ctx *ctx, const int *shape8);
+
 #ifdef __cplusplus
 }
 #endif
