@@ -161,6 +161,17 @@ struct cg_ctx {
     // tuning file when one is set
     int fft_tuned = 0;
     std::string tune_path;
+    // the mixed form's stress scale as the register FFT kernels read it: {1/scale, scale/V}
+    double* d_scale = nullptr;
+    double* h_scale = nullptr;  // pinned staging of d_scale
+    double cur_scale = 0.0;     // the scale d_scale holds (0: not yet written)
+    // HIP graphs of a pass' fixed device work: pre = FFT + Gamma stage and the sums of
+    // w dLref F^-1; post = the solve's sums, <sigma> for the next pass and their copy to the host.
+    // pre is captured again when C0 / S0, igamma or the FFT shapes change.
+    int use_graphs = 1;
+    hipGraphExec_t g_pre = nullptr, g_post = nullptr;
+    long green_version = 0, g_pre_version = -1;
+    int g_pre_igamma = -1, tune_version = 0, g_pre_tune = -1;
 };
 
 static int fail(cg_ctx* c, int code, const char* fmt, ...) {
@@ -325,6 +336,10 @@ int cg_destroy(cg_ctx* c) {
     if (c->d_sgsum) hipFree(c->d_sgsum);
     if (c->d_fluct) hipFree(c->d_fluct);
     if (c->h_red) hipHostFree(c->h_red);
+    if (c->h_scale) hipHostFree(c->h_scale);
+    if (c->d_scale) hipFree(c->d_scale);
+    if (c->g_pre) hipGraphExecDestroy(c->g_pre);
+    if (c->g_post) hipGraphExecDestroy(c->g_post);
     if (c->fft) fft_destroy(c->fft);
     for (int i = 0; i < 8; i++) hipEventDestroy(c->ev[i]);
     if (c->s) hipStreamDestroy(c->s);
@@ -485,7 +500,8 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     launch_fill(f.red, 0.0, 256, c->s);  // the status slots start clear
     if (ensure_part(c, (size_t)((V + 63) / 64 + 64) * 81)) return -3;  // the solve may run 64-thread blocks
     HIPCHK(c, hipHostMalloc(&c->h_red, 256 * sizeof(double)));
-    if (dalloc(c, &c->d_sgsum, 16) || dalloc(c, &c->d_fluct, 16)) return -3;
+    HIPCHK(c, hipHostMalloc(&c->h_scale, 2 * sizeof(double)));
+    if (dalloc(c, &c->d_sgsum, 16) || dalloc(c, &c->d_fluct, 16) || dalloc(c, &c->d_scale, 2)) return -3;
     c->fft = fft_create(f, c->err, sizeof c->err);
     if (!c->fft) return -5;
 
@@ -549,6 +565,7 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     bool ok = inv6(s066);
     to4(c->c066, c->c0);
     to4(s066, c->s0);
+    c->green_version++;
     memcpy(c->de_ds, c->s0, sizeof c->s0);
     c->sgsum_valid = 0;
     c->lold_same = 0;
@@ -646,6 +663,38 @@ static void tune_write(const std::string& path, const std::string& key, const Ff
             t.yhalf[0], t.yhalf[1], t.kt1);
     fclose(fp);
 }
+// d_scale <- {1/scale, scale/V} when the scale changed (enqueued on the context's stream; the
+// staging buffer is free again once the stream has passed the copy, which every pass waits for)
+static double pass_scale(const cg_ctx* c) { return c->mixed ? std::max(c->svmavg, 1.0) : 1.0; }
+static int set_scale(cg_ctx* c, double scale) {
+    if (scale == c->cur_scale) return 0;
+    c->h_scale[0] = 1.0 / scale;
+    c->h_scale[1] = scale / (double)c->V;
+    HIPCHK(c, hipMemcpyAsync(c->d_scale, c->h_scale, 2 * sizeof(double), hipMemcpyHostToDevice, c->s));
+    c->cur_scale = scale;
+    return 0;
+}
+// Captures what body() enqueues on the context's stream into an executable graph.
+template <class Body>
+static int capture(cg_ctx* c, hipGraphExec_t* ex, Body body) {
+    if (*ex) {
+        hipGraphExecDestroy(*ex);
+        *ex = nullptr;
+    }
+    HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    body();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->s, &g);
+    if (e != hipSuccess || !g) return fail(c, -2, "graph capture failed: %s", hipGetErrorString(e));
+    const hipError_t e2 = hipGraphInstantiate(ex, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+        *ex = nullptr;
+        return fail(c, -2, "graph instantiation failed: %s", hipGetErrorString(e2));
+    }
+    return 0;
+}
+
 static int ensure_fft_tuned(cg_ctx* c, const cg_process* p) {
     if (c->fft_tuned) return 0;
     c->fft_tuned = 1;
@@ -661,9 +710,10 @@ static int ensure_fft_tuned(cg_ctx* c, const cg_process* p) {
         launch_sigma_sum(c->f, c->s, &nb);
         launch_reduce_final(c->f.part, nb, 9, c->d_sgsum, c->s);
     }
-    const double scale = c->mixed ? std::max(c->svmavg, 1.0) : 1.0;
-    if (fft_tune(c->f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, scale, c->s) != 0)
+    if (int e = set_scale(c, pass_scale(c))) return e;
+    if (fft_tune(c->f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, c->d_scale, c->s) != 0)
         return fail(c, -2, "FFT shape tuning failed");
+    c->tune_version++;
     if (!c->tune_path.empty()) tune_write(c->tune_path, key, fft_get_tune(c->fft));
     return 0;
 }
@@ -672,7 +722,7 @@ static int ensure_fft_tuned(cg_ctx* c, const cg_process* p) {
 // launch (cg_batch_iterate): pass_begin enqueues the work before it on the context's stream
 // and fills the solve's arguments; pass_reduce enqueues the reductions of the solve's sums
 // and their copy to the host; pass_finish (after the stream has finished) is the host part.
-static void pass_begin(cg_ctx* c, cg_process* p, cg_iter_state* it, SolveArgs& sa) {
+static int pass_begin(cg_ctx* c, cg_process* p, cg_iter_state* it, SolveArgs& sa) {
     Fields& f = c->f;
     if (it->iter == 0) {  // LS-EVPFFT.f90:222-231
         if (c->ngas) launch_gas_compl(f, p->tdot, c->complgas, c->modal, c->s);
@@ -689,11 +739,26 @@ static void pass_begin(cg_ctx* c, cg_process* p, cg_iter_state* it, SolveArgs& s
     }
     tstop(c, 2);
     // stress norm used to scale the FP32 spectral stage of the mixed form
-    double scale = 1.0;
-    if (c->mixed) scale = std::max(c->svmavg, 1.0);
+    const double scale = pass_scale(c);
+    if (int e = set_scale(c, scale)) return e;
     tstart(c, 0);
-    nb = fft_green(f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, scale, c->s);
-    launch_reduce_final_cm(f.part, nb, 9, c->d_fluct, c->s);
+    auto stage = [&]() {
+        const int n = fft_green(f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, scale, c->d_scale, c->s);
+        launch_reduce_final_cm(f.part, n, 9, c->d_fluct, c->s);
+    };
+    // the register kernels take the scale from d_scale, so their stage can be replayed as a graph
+    if (c->use_graphs && fft_tunable(f, c->fft)) {
+        if (!c->g_pre || c->g_pre_version != c->green_version || c->g_pre_igamma != p->igamma ||
+            c->g_pre_tune != c->tune_version) {
+            if (int e = capture(c, &c->g_pre, stage)) return e;
+            c->g_pre_version = c->green_version;
+            c->g_pre_igamma = p->igamma;
+            c->g_pre_tune = c->tune_version;
+        }
+        HIPCHK(c, hipGraphLaunch(c->g_pre, c->s));
+    } else {
+        stage();
+    }
     tstop(c, 0);
     // correction + mixing + constitutive solve (one kernel)
     PassConst pc;
@@ -718,14 +783,25 @@ static void pass_begin(cg_ctx* c, cg_process* p, cg_iter_state* it, SolveArgs& s
     for (int i = 0; i < 9; i++) sa.dvelgradavg[i] = c->dvelgradavg[i];
     inv3(c->defgradinvavgc, sa.finv_inv);
     sa.fluct = c->d_fluct;
+    return 0;
 }
 
-static void pass_reduce(cg_ctx* c) {
+static int pass_reduce(cg_ctx* c) {
     Fields& f = c->f;
-    launch_reduce_final_cm(f.part, solve_blocks(c->V), R_NSOLVE, f.red, c->s);
-    // next pass' <sigma> = sum_all w sigma
-    launch_copy(c->d_sgsum, f.red + R_SGALL, 9, c->s);
-    hipMemcpyAsync(c->h_red, f.red, R_NCOPY * sizeof(double), hipMemcpyDeviceToHost, c->s);
+    auto sums = [&]() {
+        launch_reduce_final_cm(f.part, solve_blocks(c->V), R_NSOLVE, f.red, c->s);
+        // next pass' <sigma> = sum_all w sigma
+        launch_copy(c->d_sgsum, f.red + R_SGALL, 9, c->s);
+        hipMemcpyAsync(c->h_red, f.red, R_NCOPY * sizeof(double), hipMemcpyDeviceToHost, c->s);
+    };
+    if (!c->use_graphs) {
+        sums();
+        return 0;
+    }
+    if (!c->g_post)
+        if (int e = capture(c, &c->g_post, sums)) return e;
+    HIPCHK(c, hipGraphLaunch(c->g_post, c->s));
+    return 0;
 }
 
 static int pass_finish(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
@@ -830,14 +906,14 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     if (!c || !c->initialised || !p || !it) return -1;
     if (int e = ensure_fft_tuned(c, p)) return e;
     SolveArgs sa;
-    pass_begin(c, p, it, sa);
+    if (int e = pass_begin(c, p, it, sa)) return e;
     PhaseTables pt{c->d_ph, c->d_tab};
     int nb;
     tstart(c, 1);
     launch_correct_solve_full(c->f, pt, sa.pc, sa.dvelgradavg, sa.finv_inv, sa.fluct, c->s, &nb, c->int_pow,
                               c->sym_tau, c->small_ns);
     tstop(c, 1);
-    pass_reduce(c);
+    if (int e = pass_reduce(c)) return e;
     HIPCHK(c, hipStreamSynchronize(c->s));
     HIPCHK(c, hipGetLastError());
     tcollect(c, 0); tcollect(c, 1); tcollect(c, 2);
@@ -973,7 +1049,7 @@ int cg_batch_launch(cg_batch* b, cg_process* const* p, cg_iter_state* st, cg_ite
         if (!active[i]) continue;
         cg_ctx* c = b->ctx[i];
         SolveArgs sa;
-        pass_begin(c, p[i], &st[i], sa);
+        if (int e = pass_begin(c, p[i], &st[i], sa)) return noctx_fail(e, "cg_batch_launch: RVE %d: %s", i, c->err);
         b->h_work[i].f = c->f;
         b->h_work[i].pt = PhaseTables{c->d_ph, c->d_tab};
         b->h_work[i].a = sa;
@@ -993,7 +1069,7 @@ int cg_batch_launch(cg_batch* b, cg_process* const* p, cg_iter_state* st, cg_ite
     for (int i = 0; i < n; i++) {
         if (!active[i]) continue;
         hipStreamWaitEvent(b->ctx[i]->s, b->ev_solve, 0);
-        pass_reduce(b->ctx[i]);
+        if (int e = pass_reduce(b->ctx[i])) return noctx_fail(e, "cg_batch_launch: RVE %d: %s", i, b->ctx[i]->err);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return noctx_fail(-2, "cg_batch_launch: %s", hipGetErrorString(e));
@@ -1213,6 +1289,7 @@ int cg_calc_c0(cg_ctx* c, const cg_process* p) {
     ok = inv6(a66) && ok;
     to4(c->c066, c->c0);
     to4(a66, c->s0);
+    c->green_version++;
     if (!ok) return fail(c, 1, "Singular matrix in lu_inverse (calc_c0)");
     return 0;
 }
@@ -1410,6 +1487,13 @@ int cg_set_fft_tuning(cg_ctx* c, const int* shape8) {
     t.kt1 = shape8[7];
     fft_set_tune(c->fft, t);
     c->fft_tuned = 1;  // given, not timed
+    c->tune_version++;
+    return 0;
+}
+
+int cg_set_graphs(cg_ctx* c, int on) {
+    if (!c) return -1;
+    c->use_graphs = on ? 1 : 0;
     return 0;
 }
 

@@ -158,13 +158,16 @@ FftCtx* fft_create(const Fields& f, char* err, int errlen);
 void fft_destroy(FftCtx* c);
 // PK1 -> Gamma convolution -> dLref; writes the 9 block partial sums of
 // w dLref F^-1 to f.part, column-major (part[i * nblocks + b]), and returns the number of blocks.
+// scale: the mixed form's stress scale (1 in FP64), used by the LDS kernels; the register
+// kernels read {1/scale, scale/V} from the device array dscale, so a captured stage stays valid
+// when the scale changes.
 int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
-              int mixed, double scale, hipStream_t s);
+              int mixed, double scale, const double* dscale, hipStream_t s);
 // Times the candidate shapes of each register pass on stream s (the passes overwrite the
 // spectrum, ΔL^ref and the block partials, which the next fft_green recomputes) and keeps the
 // fastest; 0 on success or when the lengths use the LDS kernels.
 int fft_tune(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
-             int mixed, double scale, hipStream_t s);
+             int mixed, const double* dscale, hipStream_t s);
 FftTune fft_get_tune(const FftCtx* c);
 void fft_set_tune(FftCtx* c, const FftTune& t);
 bool fft_tunable(const Fields& f, const FftCtx* c);

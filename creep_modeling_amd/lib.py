@@ -21,7 +21,8 @@ EXPORTS = (
     "cg_create", "cg_destroy", "cg_last_error", "cg_set_phase", "cg_set_voxels", "cg_init",
     "cg_begin_increment", "cg_iterate", "cg_rollback", "cg_kinematic_substep", "cg_end_increment",
     "cg_calc_c0", "cg_field_ncomp", "cg_get_field", "cg_set_field", "cg_get_macro", "cg_get_stats",
-    "cg_set_timing", "cg_set_tuning_file", "cg_get_fft_tuning", "cg_set_fft_tuning", "cg_kinematic_substep_noalloc",
+    "cg_set_timing", "cg_set_tuning_file", "cg_get_fft_tuning", "cg_set_fft_tuning", "cg_set_graphs",
+    "cg_kinematic_substep_noalloc",
     "cg_set_thermo", "cg_batch_create", "cg_batch_destroy", "cg_batch_iterate", "cg_batch_launch", "cg_batch_wait",
 )
 
@@ -125,6 +126,7 @@ def load():
     L.cg_set_tuning_file.argtypes = [vp, C.c_char_p]
     L.cg_get_fft_tuning.argtypes = [vp, _I, _D]
     L.cg_set_fft_tuning.argtypes = [vp, _I]
+    L.cg_set_graphs.argtypes = [vp, C.c_int]
     L.cg_kinematic_substep_noalloc.argtypes = [C.c_long, vp, C.c_long, vp, C.c_double, vp, vp]
     L.cg_set_thermo.argtypes = [vp, _D]
     L.cg_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
@@ -239,6 +241,10 @@ class Context:
         if rc < 0:
             raise HipError("cg_get_fft_tuning rc=%d" % rc)
         return bool(rc), sh.tolist(), us.tolist()
+
+    def set_graphs(self, on):
+        """Replay each pass' fixed device work as HIP graphs (default) or launch it eagerly."""
+        self._chk(self.L.cg_set_graphs(self.h, int(bool(on))), "cg_set_graphs")
 
     def set_fft_tuning(self, shapes):
         """Fix the FFT shapes (8 entries of 0/1) and skip the tuning."""
