@@ -161,6 +161,9 @@ struct cg_ctx {
     // tuning file when one is set
     int fft_tuned = 0;
     std::string tune_path;
+    // initial displacement field (cg_set_displacement): 0 none, 1 cell values, 2 node values
+    int idispini = 0;
+    std::vector<double> dispini;
     // the mixed form's stress scale as the register FFT kernels read it: {1/scale, scale/V}
     double* d_scale = nullptr;
     double* h_scale = nullptr;  // pinned staging of d_scale
@@ -320,7 +323,7 @@ static void free_fields(cg_ctx* c) {
     Fields& f = c->f;
     void* ptrs[] = {f.phase, f.gas, f.w, f.sg, f.sgt, f.L, f.Lold, f.dLref, f.F, f.Fi, f.J, f.Fe, f.Fp, f.ag,
                     f.sg66, f.c66m, f.cgm, f.cgas, f.sop, f.sch, f.schA, f.crss, f.ttau, f.taun, f.gdot, f.gacc, f.edotp, f.ept, f.dgs,
-                    f.epvm, f.Finc, f.Feinc, f.eth, f.s_L, f.s_Lold, f.s_sg, f.s_edotp, f.s_gdot, f.s_ttau, f.spec,
+                    f.epvm, f.Finc, f.Feinc, f.eth, f.Fini, f.s_L, f.s_Lold, f.s_sg, f.s_edotp, f.s_gdot, f.s_ttau, f.spec,
                     f.specf, f.xnode, f.part, f.red};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -533,7 +536,24 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     PhaseTables pt{c->d_ph, c->d_tab};
     int nb;
     launch_init_xnode(f, c->n3node, c->s);
-    launch_init_voxels(f, pt, d_eul, d_gr, d_hp, p1->udot, p1->tdot, c->s, &nb);
+    // init_defgrad (IO_functions.f90:815-936), between data_grain and the weighted initial C0:
+    // F, F0, J, F^-1 and w_c from the initial displacement field; delt is (4, 4, 4), the value
+    // load_input gives it whatever the file says (IO_functions.f90:62-66)
+    double dsum[30];
+    if (c->idispini > 0) {
+        double* d_disp = nullptr;
+        if (!f.Fini && dalloc(c, &f.Fini, 9 * f.ld)) return -3;
+        if (dalloc(c, &d_disp, c->dispini.size())) return -3;
+        HIPCHK(c, hipMemcpy(d_disp, c->dispini.data(), c->dispini.size() * sizeof(double), hipMemcpyHostToDevice));
+        launch_init_defgrad(f, d_disp, c->idispini, 4.0, c->s, &nb);
+        launch_xnode_disp(f, d_disp, c->idispini, c->n3node, c->s);
+        const int rr = reduce_to_host(c, nb, 30, dsum);
+        hipFree(d_disp);
+        if (rr) return -2;
+        if (int e = check_singular(c, "init_defgrad: lu_inverse of F")) return e;
+        launch_div_w(f, dsum[29], c->s);  // w_c = wgt J / <J>
+    }
+    launch_init_voxels(f, pt, d_eul, d_gr, d_hp, p1->udot, p1->tdot, c->idispini > 0, c->s, &nb);
     c->taun_valid = 0;
     double sums[36];
     if (reduce_to_host(c, nb, 36, sums)) return -2;
@@ -547,6 +567,18 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     memcpy(c->defgradavg, I3, sizeof I3); memcpy(c->defgradinvavgc, I3, sizeof I3);
     memcpy(c->defgradinvavgcs, I3, sizeof I3); memcpy(c->defgradinvavgcg, I3, sizeof I3);
+    if (c->idispini > 0) {  // init_defgrad's averages (IO_functions.f90:887-935)
+        const double dfa = dsum[29];
+        c->wphcsol = dsum[27] / dfa;
+        c->wphcgas = dsum[28] / dfa;
+        memset(c->defgradavg, 0, sizeof c->defgradavg);
+        c->defgradavg[0] = c->defgradavg[4] = c->defgradavg[8] = 4.0;
+        for (int t = 0; t < 9; t++) {
+            c->defgradinvavgc[t] = dsum[t] / dfa;
+            c->defgradinvavgcs[t] = dsum[9 + t] / dfa / c->wphcsol;
+            c->defgradinvavgcg[t] = c->wphcgas > 0.0 ? dsum[18 + t] / dfa / c->wphcgas : I3[t];
+        }
+    }
     memset(c->dvelgradavg, 0, sizeof c->dvelgradavg);
     memset(c->velgradavg, 0, sizeof c->velgradavg);
     memset(c->sgavg, 0, sizeof c->sgavg);
@@ -1315,7 +1347,7 @@ bool field_desc(cg_ctx* c, const char* name, FieldDesc& d) {
         {"cgas", f.cgas, 36, 36, 0}, {"sch", f.sch, 5 * ns, 5 * ns, 0}, {"crss", f.crss, 2 * ns, 2 * ns, 0},
         {"trialtau", f.ttau, 2 * ns, 2 * ns, 0}, {"gamdot", f.gdot, ns, ns, 0}, {"gacumgr", f.gacc, 1, 1, 0},
         {"edotp", f.edotp, 6, 9, 1}, {"ept", f.ept, 6, 9, 1}, {"disgradsym", f.dgs, 6, 9, 1}, {"epvm", f.epvm, 1, 1, 0},
-        {"defgradinc", f.Finc, 9, 9, 0}, {"defgradeinc", f.Feinc, 9, 9, 0},
+        {"defgradinc", f.Finc, 9, 9, 0}, {"defgradeinc", f.Feinc, 9, 9, 0}, {"defgradini", f.Fini, 9, 9, 0},
     };
     for (auto& e : T)
         if (!strcmp(e.n, name)) {
@@ -1488,6 +1520,16 @@ int cg_set_fft_tuning(cg_ctx* c, const int* shape8) {
     fft_set_tune(c->fft, t);
     c->fft_tuned = 1;  // given, not timed
     c->tune_version++;
+    return 0;
+}
+
+int cg_set_displacement(cg_ctx* c, int idispini, const double* disp) {
+    if (!c || !disp) return -1;
+    if (c->initialised) return fail(c, -1, "cg_set_displacement: call before cg_init");
+    if (idispini != 1 && idispini != 2) return fail(c, -1, "cg_set_displacement: idispini must be 1 or 2");
+    const size_t n = idispini == 1 ? (size_t)c->V : (size_t)(c->n1 + 1) * (c->n2 + 1) * (c->n3 == 1 ? 1 : c->n3 + 1);
+    c->dispini.assign(disp, disp + 3 * n);
+    c->idispini = idispini;
     return 0;
 }
 

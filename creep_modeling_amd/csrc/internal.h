@@ -48,6 +48,7 @@ struct Fields {
     double *edotp, *ept, *dgs, *epvm;   // 6,6,6,1
     double *Finc, *Feinc;               // 9,9
     double *eth;          // 6 (b-basis): eigenstrain rate of the voxel's grain (ithermo = 1), or null
+    double *Fini;         // 9: initial deformation gradient of an initial displacement field, or null (F0 = I)
     double *s_L, *s_Lold, *s_sg, *s_edotp, *s_gdot, *s_ttau;  // snapshots
     double2* spec;        // FP64 half spectrum, [point][9] (point = (k*n2+j)*h1+kx)
     float2* specf;        // mixed form: FP32 half spectrum
@@ -136,7 +137,10 @@ void launch_kinematic_raw(const double* L, long ld, const double* J, long V, dou
 long kinematic_scratch_doubles(long V);
 void launch_snapshot(const Fields& f, int restore, hipStream_t s);
 void launch_init_voxels(const Fields& f, const PhaseTables& pt, const double* euler, const int* grain,
-                        const double* hp_voxel, const double* udot, double tdot, hipStream_t s, int* nb);
+                        const double* hp_voxel, const double* udot, double tdot, int keep_defgrad, hipStream_t s,
+                        int* nb);
+void launch_init_defgrad(const Fields& f, const double* disp, int idispini, double delt, hipStream_t s, int* nb);
+void launch_xnode_disp(const Fields& f, const double* disp, int idispini, int n3node, hipStream_t s);
 void launch_copy(double* dst, const double* src, size_t n, hipStream_t s);
 void launch_fill(double* dst, double val, size_t n, hipStream_t s);
 void launch_init_xnode(const Fields& f, int n3node, hipStream_t s);

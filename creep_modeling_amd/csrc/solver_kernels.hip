@@ -1449,7 +1449,7 @@ void launch_div_w(const Fields& f, double detfavg, hipStream_t s) {
 }
 
 // update_elstic_defgrad (:2285-2324) + update_el_stiff (:2327-2402)
-//   Fe = F Fp^-1 (F_ini = I), Fe_inc = Fe Fe_old^-1,
+//   Fe = F (Fp F_ini)^-1 (F_ini = I without an initial displacement field), Fe_inc = Fe Fe_old^-1,
 //   cg66 = T(A) C66cr T(A)^T / det Fe, sg66 = cg66^-1, A = Fe ag
 __global__ __launch_bounds__(BS) void k_elastic_stiff(Fields f, PhaseTables pt) {
     const long V = f.V;
@@ -1459,7 +1459,14 @@ __global__ __launch_bounds__(BS) void k_elastic_stiff(Fields f, PhaseTables pt) 
     double F[9], Fp[9], Fpi[9], Feo[9], Feoi[9], Fe[9], Fei[9];
     for (int c = 0; c < 9; c++) { F[c] = f.F[c * f.ld + v]; Fp[c] = f.Fp[c * f.ld + v]; Feo[c] = f.Fe[c * f.ld + v]; }
     bool ok = inv3(Feo, Feoi) != 0.0;
-    ok = (inv3(Fp, Fpi) != 0.0) && ok;
+    if (f.Fini) {  // (Fp F0)^-1 with the initial deformation gradient of init_defgrad
+        double FpF0[9], F0[9];
+        for (int c = 0; c < 9; c++) F0[c] = f.Fini[c * f.ld + v];
+        mm3(Fp, F0, FpF0);
+        ok = (inv3(FpF0, Fpi) != 0.0) && ok;
+    } else {
+        ok = (inv3(Fp, Fpi) != 0.0) && ok;
+    }
     mm3(F, Fpi, Fe);
     mm3(Fe, Feoi, Fei);
     for (int c = 0; c < 9; c++) { f.Fe[c * f.ld + v] = Fe[c]; f.Feinc[c * f.ld + v] = Fei[c]; }
@@ -1952,7 +1959,7 @@ void launch_snapshot(const Fields& f, int restore, hipStream_t s) {
 // strengths (:100-199), initial_guess (various_functions.f90:8-53) and the
 // 36 sums of w cg66 over solid voxels for the initial C0 (LS-EVPFFT.f90:156-166)
 // ---------------------------------------------------------------------------
-struct InitArgs { double udot[9]; double tdot; };
+struct InitArgs { double udot[9]; double tdot; int keep_defgrad; };
 __global__ __launch_bounds__(BS) void k_init_voxels(Fields f, PhaseTables pt, const double* __restrict__ euler,
                                                     const int* __restrict__ grain, const double* __restrict__ hp,
                                                     InitArgs a) {
@@ -1962,11 +1969,15 @@ __global__ __launch_bounds__(BS) void k_init_voxels(Fields f, PhaseTables pt, co
     for (int i = 0; i < 36; i++) acc[i] = 0.0;
     if (v < V) {
         const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-        const double wg = 1.0 / (double)V;
-        f.w[v] = wg;
-        f.J[v] = 1.0;
+        // with an initial displacement field, F, F^-1, J and w_c are init_defgrad's (k_init_defgrad)
+        const double wg = a.keep_defgrad ? f.w[v] : 1.0 / (double)V;
+        if (!a.keep_defgrad) {
+            f.w[v] = wg;
+            f.J[v] = 1.0;
+            for (int c = 0; c < 9; c++) { f.F[c * f.ld + v] = I3[c]; f.Fi[c * f.ld + v] = I3[c]; }
+        }
         for (int c = 0; c < 9; c++) {
-            f.F[c * f.ld + v] = I3[c]; f.Fi[c * f.ld + v] = I3[c]; f.Fe[c * f.ld + v] = I3[c]; f.Fp[c * f.ld + v] = I3[c];
+            f.Fe[c * f.ld + v] = I3[c]; f.Fp[c * f.ld + v] = I3[c];
             f.Finc[c * f.ld + v] = I3[c]; f.Feinc[c * f.ld + v] = I3[c];
             f.L[c * f.ld + v] = a.udot[c];
             f.Lold[c * f.ld + v] = 0.0;
@@ -2032,10 +2043,12 @@ __global__ __launch_bounds__(BS) void k_init_voxels(Fields f, PhaseTables pt, co
     block_reduce_store<36, BS>(acc, f.part);
 }
 void launch_init_voxels(const Fields& f, const PhaseTables& pt, const double* euler, const int* grain,
-                        const double* hp_voxel, const double* udot, double tdot, hipStream_t s, int* nb) {
+                        const double* hp_voxel, const double* udot, double tdot, int keep_defgrad, hipStream_t s,
+                        int* nb) {
     InitArgs a;
     for (int i = 0; i < 9; i++) a.udot[i] = udot[i];
     a.tdot = tdot;
+    a.keep_defgrad = keep_defgrad;
     *nb = nblk(f.V);
     hipLaunchKernelGGL(k_init_voxels, dim3(*nb), dim3(BS), 0, s, f, pt, euler, grain, hp_voxel, a);
 }
@@ -2058,6 +2071,108 @@ namespace cg {
 // nodal coordinates (init_micro types.f90:268-287; update_grid_velgrad_node
 // various_functions.f90:2026-2054 with cell2node_tensor1 :2652-2691)
 // ---------------------------------------------------------------------------
+// init_defgrad (IO_functions.f90:815-936): F = grad(u0) + delt I, F0 = F, J, F^-1 and w = wgt J,
+// with 30 block sums: w F^-1 (all, solid, gas voxels), w over solid and gas voxels, w.
+// The reference's gradient_tensor1 (various_functions.f90:56-112) multiplies the displacement's
+// spectrum by i sin(xi_m) (cell values, idispini 1) or by tan(xi_m/2) (i/4) prod_a (1 + e^{i xi_a})
+// (node values, idispini 2); with e^{i xi_a} the shift by one voxel along a, these are exactly the
+// periodic stencils (u(x + e_m) - u(x - e_m))/2 and, since (1 + e^{i xi}) tan(xi/2) = -i (e^{i xi} - 1),
+// (1/4) sum over the four cell edges along m of u(edge end) - u(edge start), which this kernel
+// applies in real space (the all-Nyquist term the reference zeroes is zero in the stencil).
+__global__ __launch_bounds__(BS) void k_init_defgrad(Fields f, const double* __restrict__ disp, int idispini,
+                                                     double delt) {
+    const long V = f.V;
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    double acc[30];
+    for (int t = 0; t < 30; t++) acc[t] = 0.0;
+    if (v < V) {
+        const int n1 = f.n1, n2 = f.n2, n3 = f.n3;
+        const int i = (int)(v % n1), j = (int)((v / n1) % n2), k = (int)(v / ((long)n1 * n2));
+        auto wr = [](int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); };
+        // cell (idispini 1) or node (2) value l at lattice point (a, b, c), periodic
+        const long rs = idispini == 1 ? n1 : n1 + 1, ps = idispini == 1 ? (long)n1 * n2 : (long)(n1 + 1) * (n2 + 1);
+        auto U = [&](int a, int b, int c, int l) { return disp[3 * (wr(a, n1) + rs * wr(b, n2) + ps * wr(c, n3)) + l]; };
+        double F[9];
+        for (int l = 0; l < 3; l++) {
+            if (idispini == 1) {
+                F[3 * l + 0] = 0.5 * (U(i + 1, j, k, l) - U(i - 1, j, k, l));
+                F[3 * l + 1] = 0.5 * (U(i, j + 1, k, l) - U(i, j - 1, k, l));
+                F[3 * l + 2] = 0.5 * (U(i, j, k + 1, l) - U(i, j, k - 1, l));
+            } else {
+                double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+                for (int p = 0; p < 2; p++)
+                    for (int q = 0; q < 2; q++) {
+                        g0 += U(i + 1, j + p, k + q, l) - U(i, j + p, k + q, l);
+                        g1 += U(i + p, j + 1, k + q, l) - U(i + p, j, k + q, l);
+                        g2 += U(i + p, j + q, k + 1, l) - U(i + p, j + q, k, l);
+                    }
+                F[3 * l + 0] = 0.25 * g0;
+                F[3 * l + 1] = 0.25 * g1;
+                F[3 * l + 2] = 0.25 * g2;
+            }
+        }
+        F[0] += delt;
+        F[4] += delt;
+        F[8] += delt;
+        // determinant33 (tensor_functions.f90:442-456) term order
+        const double J = F[0] * F[4] * F[8] + F[1] * F[5] * F[6] + F[2] * F[3] * F[7] - F[6] * F[4] * F[2] -
+                         F[7] * F[5] * F[0] - F[8] * F[3] * F[1];
+        double Fi[9];
+        if (inv3(F, Fi) == 0.0) flag_singular(f, v);  // lu_inverse (IO_functions.f90:907)
+        const double wj = J / (double)V;
+        const bool gas = f.gas[v] != 0;
+        for (int c = 0; c < 9; c++) {
+            f.F[c * f.ld + v] = F[c];
+            f.Fini[c * f.ld + v] = F[c];
+            f.Fi[c * f.ld + v] = Fi[c];
+            acc[c] = Fi[c] * wj;
+            acc[(gas ? 18 : 9) + c] = Fi[c] * wj;
+        }
+        f.J[v] = J;
+        f.w[v] = wj;
+        acc[gas ? 28 : 27] = wj;
+        acc[29] = wj;
+    }
+    block_reduce_store<30, BS>(acc, f.part);
+}
+void launch_init_defgrad(const Fields& f, const double* disp, int idispini, double delt, hipStream_t s, int* nb) {
+    *nb = nblk(f.V);
+    hipLaunchKernelGGL(k_init_defgrad, dim3(*nb), dim3(BS), 0, s, f, disp, idispini, delt);
+}
+// xnode += the initial nodal displacement: the file's node values (idispini 2) or the mean of the
+// eight cells around each node (idispini 1; cell2node_tensor1, various_functions.f90:2652-2691,
+// summed in its order)
+__global__ __launch_bounds__(BS) void k_xnode_disp(Fields f, const double* __restrict__ disp, int idispini,
+                                                   int n3node) {
+    const int n1 = f.n1, n2 = f.n2, n3 = f.n3;
+    const long nn = (long)(n1 + 1) * (n2 + 1) * n3node;
+    const long q = (long)blockIdx.x * BS + threadIdx.x;
+    if (q >= nn) return;
+    if (idispini == 2) {
+        for (int l = 0; l < 3; l++) f.xnode[3 * q + l] += disp[3 * q + l];
+        return;
+    }
+    const int a = (int)(q % (n1 + 1)), b = (int)((q / (n1 + 1)) % (n2 + 1)), c = (int)(q / ((long)(n1 + 1) * (n2 + 1)));
+    const int i1 = a == 0 ? n1 - 1 : a - 1, i2 = a == n1 ? 0 : a;
+    const int j1 = b == 0 ? n2 - 1 : b - 1, j2 = b == n2 ? 0 : b;
+    const int k1 = c == 0 ? n3 - 1 : c - 1, k2 = c == n3 ? 0 : c;
+    auto C = [&](int x, int y, int z, int l) { return disp[3 * (x + (long)n1 * (y + (long)n2 * z)) + l]; };
+    for (int l = 0; l < 3; l++) {
+        double s = C(i2, j2, k2, l);
+        s += C(i1, j2, k2, l);
+        s += C(i2, j1, k2, l);
+        s += C(i2, j2, k1, l);
+        s += C(i1, j1, k2, l);
+        s += C(i1, j2, k1, l);
+        s += C(i2, j1, k1, l);
+        s += C(i1, j1, k1, l);
+        f.xnode[3 * q + l] += s * 0.125;
+    }
+}
+void launch_xnode_disp(const Fields& f, const double* disp, int idispini, int n3node, hipStream_t s) {
+    const long nn = (long)(f.n1 + 1) * (f.n2 + 1) * n3node;
+    hipLaunchKernelGGL(k_xnode_disp, dim3((unsigned)((nn + BS - 1) / BS)), dim3(BS), 0, s, f, disp, idispini, n3node);
+}
 __global__ __launch_bounds__(BS) void k_init_xnode(Fields f, int n3node) {
     const long nn = (long)(f.n1 + 1) * (f.n2 + 1) * n3node;
     const long q = (long)blockIdx.x * BS + threadIdx.x;

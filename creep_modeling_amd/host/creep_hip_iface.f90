@@ -73,6 +73,12 @@ module creep_hip_iface
             real(c_double) :: euler(*)
             integer(c_int) :: grain(*), phase(*)
         end function
+        integer(c_int) function cg_set_displacement(ctx, idispini, disp) bind(C, name='cg_set_displacement')
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: idispini
+            real(c_double) :: disp(*)
+        end function
         integer(c_int) function cg_init(ctx, p1) bind(C, name='cg_init')
             import :: c_int, c_ptr, cg_process
             type(c_ptr), value :: ctx

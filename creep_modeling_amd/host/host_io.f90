@@ -30,6 +30,9 @@ module host_io
         real(8), allocatable :: euler(:, :)     ! (3, V) degrees
         integer(c_int), allocatable :: grain(:), ph(:)
         type(cg_process), allocatable :: proc(:)
+        ! initial displacement field (init_defgrad, IO_functions.f90:850-880): 3 values per cell
+        ! (idispini 1) or per node (2), x fastest, as the file lists them
+        real(c_double), allocatable :: dispini(:)
     end type input_t
 
     ! eigenstrain rates of ithermo = 1 (IO_functions.f90:316-341): one 3x3 tensor per grain from
@@ -97,7 +100,7 @@ contains
         read (11, *) inp%idispini
         if (inp%idispini > 0) then
             read (11, '(A)') filedispl
-            call fatal('idispini > 0 (initial displacement file) is not supported')
+            call read_displacement(trim(adjustl(filedispl)), inp)
         end if
         read (11, *) inp%nproc
         if (inp%nproc < 1) call fatal('cannot have less than one processes')
@@ -367,6 +370,29 @@ contains
     end subroutine read_process
 
     ! the eigenstrain file: ngrains, then eth11 eth22 eth33 eth23 eth31 eth12 per grain
+    ! the initial displacement file: one line "i j k u1 u2 u3" per cell (idispini 1) or node
+    ! (idispini 2; one z layer of nodes for a 2D grid), x fastest (IO_functions.f90:850-880)
+    subroutine read_displacement(fname, inp)
+        character(len=*), intent(in) :: fname
+        type(input_t), intent(inout) :: inp
+        integer :: nrow, r, ip1, ip2, ip3, ios, n3node
+        if (inp%idispini /= 1 .and. inp%idispini /= 2) call fatal('IDISPINI must be 0, 1 or 2')
+        n3node = merge(1, inp%npts3 + 1, inp%npts3 == 1)
+        if (inp%idispini == 1) then
+            nrow = inp%npts1 * inp%npts2 * inp%npts3
+        else
+            nrow = (inp%npts1 + 1) * (inp%npts2 + 1) * n3node
+        end if
+        allocate (inp%dispini(3 * nrow))
+        open (unit=13, file=fname, status='old', iostat=ios)
+        if (ios /= 0) call fatal('cannot open the initial displacement file '//fname)
+        do r = 1, nrow
+            read (13, *, iostat=ios) ip1, ip2, ip3, inp%dispini(3 * r - 2), inp%dispini(3 * r - 1), inp%dispini(3 * r)
+            if (ios /= 0) call fatal('initial displacement file '//fname//': too few lines')
+        end do
+        close (13)
+    end subroutine read_displacement
+
     subroutine read_thermo()
         character(len=256) :: filethermo
         integer :: ngr, ig

@@ -124,6 +124,10 @@ program lsevpfft_host
     end do
     rc = cg_set_voxels(ctx, inp%euler, inp%grain, inp%ph)
     if (rc /= 0) call fatal('cg_set_voxels: '//trim(last_error(ctx)))
+    if (inp%idispini > 0) then  ! init_defgrad (IO_functions.f90:152-154)
+        rc = cg_set_displacement(ctx, inp%idispini, inp%dispini)
+        if (rc /= 0) call fatal('cg_set_displacement: '//trim(last_error(ctx)))
+    end if
     rc = cg_init(ctx, inp%proc(1))
     if (rc /= 0) call fatal_rc('cg_init: '//trim(last_error(ctx)), rc)
     ! the FFT shape tuning of the first pass is cached where the reference keeps its FFTW wisdom

@@ -250,7 +250,7 @@ def write_cupl2(path, mat: Material):
 
 
 def write_fft_in(path, n, microstructure_path, cupl2_path, cuel_path, processes,
-                 complgas=0.01):
+                 complgas=0.01, idispini=0, displacement_path=None):
     if isinstance(n, int):
         n = (n, n, n)
     lines = [
@@ -268,7 +268,8 @@ def write_fft_in(path, n, microstructure_path, cupl2_path, cuel_path, processes,
         "0 200                  ires,imicrosave",
         "%r                   complgas" % float(complgas),
         "* initial displacement field",
-        "0                      IDISPINI",
+        "%d                      IDISPINI" % idispini,
+    ] + ([str(displacement_path)] if idispini > 0 else []) + [
         "%d                      number of processes (nproc)" % len(processes),
     ]
     for p in processes:
@@ -323,15 +324,38 @@ def write_thermo(path, eth):
     Path(path).write_text("\n".join(lines) + "\n")
 
 
-def write_case(directory, ms: Microstructure, mat: Material, processes, complgas=0.01):
-    """Write a complete input set; returns the ``fft.in`` path."""
+def write_displacement(path, n, disp, idispini):
+    """Initial displacement file of idispini = 1 (one line "i j k u1 u2 u3" per cell, i fastest)
+    or 2 (one line per node: (n1+1)(n2+1)(n3+1) nodes, n3 + 1 -> 1 for a 2D grid), as
+    init_defgrad reads it (IO_functions.f90:850-880).  disp: (lines, 3)."""
+    n1, n2, n3 = n
+    if idispini == 1:
+        dims = (n1, n2, n3)
+    else:
+        dims = (n1 + 1, n2 + 1, 1 if n3 == 1 else n3 + 1)
+    disp = np.asarray(disp, float).reshape(-1, 3)
+    if disp.shape[0] != dims[0] * dims[1] * dims[2]:
+        raise ValueError("displacement has %d rows, expected %d" % (disp.shape[0], dims[0] * dims[1] * dims[2]))
+    k, j, i = np.meshgrid(np.arange(dims[2]), np.arange(dims[1]), np.arange(dims[0]), indexing="ij")
+    with open(path, "w") as fh:
+        for (a, b, c), u in zip(zip(i.ravel() + 1, j.ravel() + 1, k.ravel() + 1), disp):
+            fh.write("%d %d %d %r %r %r\n" % (a, b, c, float(u[0]), float(u[1]), float(u[2])))
+
+
+def write_case(directory, ms: Microstructure, mat: Material, processes, complgas=0.01, displacement=None):
+    """Write a complete input set; returns the ``fft.in`` path.  displacement: (idispini, values)
+    for an initial displacement file (see write_displacement)."""
     d = Path(directory).resolve()
     d.mkdir(parents=True, exist_ok=True)
     write_microstructure(d / "micro.dat", ms)
     write_cupl2(d / "cupl2.sx", mat)
     write_cuel(d / "cuel.sx", mat)
+    idispini, dpath = 0, None
+    if displacement is not None:
+        idispini, dpath = int(displacement[0]), d / "dispini.txt"
+        write_displacement(dpath, tuple(ms.n), displacement[1], idispini)
     write_fft_in(d / "fft.in", ms.n, d / "micro.dat", d / "cupl2.sx", d / "cuel.sx", processes,
-                 complgas)
+                 complgas, idispini, dpath)
     return d / "fft.in"
 
 

@@ -18,7 +18,7 @@ _I = C.POINTER(C.c_int)
 
 #: symbols exported by the library (one per declaration in include/creep_hip.h)
 EXPORTS = (
-    "cg_create", "cg_destroy", "cg_last_error", "cg_set_phase", "cg_set_voxels", "cg_init",
+    "cg_create", "cg_destroy", "cg_last_error", "cg_set_phase", "cg_set_voxels", "cg_set_displacement", "cg_init",
     "cg_begin_increment", "cg_iterate", "cg_rollback", "cg_kinematic_substep", "cg_end_increment",
     "cg_calc_c0", "cg_field_ncomp", "cg_get_field", "cg_set_field", "cg_get_macro", "cg_get_stats",
     "cg_set_timing", "cg_set_tuning_file", "cg_get_fft_tuning", "cg_set_fft_tuning", "cg_set_graphs",
@@ -127,6 +127,7 @@ def load():
     L.cg_get_fft_tuning.argtypes = [vp, _I, _D]
     L.cg_set_fft_tuning.argtypes = [vp, _I]
     L.cg_set_graphs.argtypes = [vp, C.c_int]
+    L.cg_set_displacement.argtypes = [vp, C.c_int, _D]
     L.cg_kinematic_substep_noalloc.argtypes = [C.c_long, vp, C.c_long, vp, C.c_double, vp, vp]
     L.cg_set_thermo.argtypes = [vp, _D]
     L.cg_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
@@ -190,6 +191,9 @@ class Context:
         gr = np.ascontiguousarray(problem.grain, np.int32)
         phv = np.ascontiguousarray(problem.phase, np.int32)
         self._chk(L.cg_set_voxels(h, dp(eul), ip(gr), ip(phv)))
+        if getattr(problem, "idispini", 0) > 0:  # initial displacement field (init_defgrad)
+            d = np.ascontiguousarray(problem.dispini, np.float64).reshape(-1)
+            self._chk(L.cg_set_displacement(h, int(problem.idispini), dp(d)), "cg_set_displacement")
         self._p1 = process_struct(problem.processes[0])
         self._chk(L.cg_init(h, C.byref(self._p1)))
         eth = getattr(problem, "eth", None)

@@ -121,6 +121,7 @@ class Problem:
     idispini: int
     processes: list
     eth: np.ndarray = None  # (V, 3, 3) per-voxel eigenstrain rate (some process has ithermo = 1)
+    dispini: np.ndarray = None  # idispini 1: (V, 3) cell displacements; 2: (nodes, 3), i fastest
 
     @property
     def nvox(self):
@@ -409,13 +410,32 @@ def read_fft_in(path):
     complgas = r.values(1)[0]
     r.skip()
     idispini = r.values(1, int)[0]
+    dispini = None
     if idispini > 0:
-        raise InputError("idispini>0 (initial displacement file) is not supported")
+        if idispini not in (1, 2):
+            raise InputError("idispini must be 0, 1 or 2")
+        dispini = read_displacement(resolve(r.line()), n, idispini)
     nproc = r.values(1, int)[0]
     if nproc < 1:
         raise InputError("cannot have less than one processes")
     procs = [_read_process(r, resolve) for _ in range(nproc)]
-    return Problem(n, phases, grain, phase, euler, complgas, idispini, procs, voxel_eth(procs, grain))
+    return Problem(n, phases, grain, phase, euler, complgas, idispini, procs, voxel_eth(procs, grain), dispini)
+
+
+def read_displacement(path, n, idispini):
+    """The initial displacement file (init_defgrad, IO_functions.f90:850-880): one line
+    "i j k u1 u2 u3" per cell (idispini 1, n1 n2 n3 lines) or per node (idispini 2,
+    (n1+1)(n2+1)(n3+1) lines, one z layer for a 2D grid), i fastest; the indices are not used."""
+    n1, n2, n3 = n
+    rows = n1 * n2 * n3 if idispini == 1 else (n1 + 1) * (n2 + 1) * (1 if n3 == 1 else n3 + 1)
+    try:
+        txt = Path(path).read_text().replace("D", "E").replace("d", "e")
+    except OSError as e:
+        raise InputError("cannot read the initial displacement file %s: %s" % (path, e))
+    vals = np.array(txt.split(), dtype=float)
+    if vals.size < 6 * rows:
+        raise InputError("initial displacement file %s: %d values, expected %d lines of 6" % (path, vals.size, rows))
+    return vals[:6 * rows].reshape(rows, 6)[:, 3:].copy()
 
 
 def problem_from_case(ms, mat, processes, complgas=0.01):

@@ -102,6 +102,13 @@ int cg_set_phase(cg_ctx *ctx, int iph, int igas, int nsyst, int nmodes, const in
 /* Microstructure (data_grain, IO_functions.f90:759-809): euler (V,3) Bunge degrees,
  * grain and phase ids (1-based). */
 int cg_set_voxels(cg_ctx *ctx, const double *euler, const int *grain, const int *phase);
+/* Initial displacement field (IDISPINI and FILEDISPL of fft.in; init_defgrad,
+ * IO_functions.f90:147-154, 815-936), before cg_init: idispini 1, disp[3 V] cell values, or
+ * idispini 2, disp[3 (n1+1)(n2+1)(n3+1 or 1 for n3 = 1)] node values, x fastest, as the file
+ * lists them.  cg_init then starts from F = grad(u0) + 4 I (the reference's delt), keeps F0 = F for
+ * the elastic deformation gradient F (Fp F0)^-1, weights the voxels by J and moves the nodal
+ * coordinates by the nodal displacement. */
+int cg_set_displacement(cg_ctx *ctx, int idispini, const double *disp);
 /* Rotations, elastic stiffness, Hall-Petch strengths (IO_functions.f90:100-199,
  * 347-363), initial_guess (various_functions.f90:8-53) and the initial C0
  * (LS-EVPFFT.f90:156-173) for the first process p1. */

@@ -123,6 +123,8 @@ def lib():
         L.orc_stage.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
         L.orc_set_macro.argtypes = [C.c_void_p, C.c_char_p, _DBL]
         L.orc_get_lavg.argtypes = [C.c_void_p, _DBL]
+        L.orc_set_displacement.argtypes = [C.c_void_p, C.c_int, _DBL]
+        L.orc_gradient_tensor1.argtypes = [_DBL, _DBL, C.c_int, C.c_int, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -179,6 +181,10 @@ class Oracle:
         gr = np.ascontiguousarray(problem.grain, np.int32)
         phv = np.ascontiguousarray(problem.phase, np.int32)
         L.orc_set_voxels(self.h, dp(eul), ip(gr), ip(phv))
+        if getattr(problem, "idispini", 0) > 0:  # initial displacement field (init_defgrad)
+            d = np.ascontiguousarray(problem.dispini, np.float64).reshape(-1)
+            if L.orc_set_displacement(self.h, int(problem.idispini), dp(d)) != 0:
+                raise RuntimeError("orc_set_displacement: %s" % L.orc_error(self.h).decode())
         for p in problem.processes:
             s = process_struct(p)
             L.orc_add_process(self.h, C.byref(s))

@@ -446,6 +446,9 @@ struct orc_ctx {
     double *sch, *crss, *trialtau, *gamdot, *gacumgr;
     double *edotp, *ept, *disgradsym, *epvm, *defgradinc, *defgradeinc;
     double *eth; /* eigenstrain rate of the voxel's grain (ithermo = 1), 3x3 */
+    int idispini;      /* initial displacement field (IO_functions.f90:147-154): 0 none, 1 cells, 2 nodes */
+    double *dispini;   /* as read: [cell][3] (idispini 1) or [node][3] (2), i fastest */
+    double *defgradini; /* F0 of init_defgrad, 3x3 per voxel (null: identity) */
     double *s_velgrad, *s_velgradold, *s_sg, *s_edotp, *s_gamdot, *s_trialtau;
     double *Goper; /* V*81 */
     double *xnode; /* 3*(n1+1)*(n2+1)*(n3node) */
@@ -492,6 +495,8 @@ orc_ctx *orc_create(int n1, int n2, int n3, int nph, double complgas, int modal,
 }
 
 static void freeall(orc_ctx *c) {
+    free(c->dispini);
+    free(c->defgradini);
     void *ptrs[] = {c->grain, c->phase, c->gas, c->euler, c->wgtc, c->sg, c->sgt, c->velgrad,
                     c->velgradold, c->dvelgradref, c->defgrad, c->defgradinv, c->detF, c->defgrade,
                     c->defgradp, c->ag, c->cg66, c->sg66, c->c066mod, c->cgm, c->cgas, c->sch,
@@ -548,6 +553,16 @@ int orc_set_voxels(orc_ctx *c, const double *euler, const int *grain, const int 
     return 0;
 }
 
+/* the initial displacement file's values (read_input, IO_functions.f90:147-154): before orc_init */
+int orc_set_displacement(orc_ctx *c, int idispini, const double *disp) {
+    if (idispini != 1 && idispini != 2) { snprintf(c->err, sizeof c->err, "idispini must be 1 or 2"); return -1; }
+    long n = idispini == 1 ? c->V : (long)(c->n1 + 1) * (c->n2 + 1) * (c->n3 == 1 ? 1 : c->n3 + 1);
+    free(c->dispini);
+    c->dispini = dup(disp, 3 * n);
+    c->idispini = idispini;
+    return c->dispini ? 0 : -2;
+}
+
 int orc_add_process(orc_ctx *c, const orc_process *p) {
     if (c->nproc == c->proc_cap) {
         c->proc_cap *= 2;
@@ -599,6 +614,134 @@ static int alloc_state(orc_ctx *c) {
             for (int t_ = 0; t_ < (NV); t_++) red_[t_] += part_[(size_t)kz_ * (NV) + t_];   \
         free(part_);                                                                        \
     } while (0)
+
+/* gradient_tensor1 (various_functions.f90:56-112): u [v][3] -> grad [v][9], grad(l,m) = d_m u_l,
+ * spectrally: the forward c2c transform of the three components, times i sin(xi_m) (ideriv 1) or
+ * tan(xi_m/2) (i/4) prod_a (1 + e^{i xi_a}) (ideriv 2, zero at the all-Nyquist point of a 3D
+ * grid), inverse, real part / V */
+void orc_gradient_tensor1(const double *u, double *grad, int ideriv, int n1, int n2, int n3) {
+    long V = (long)n1 * n2 * n3;
+    cplx *f = malloc(sizeof(cplx) * 3 * V), *g = malloc(sizeof(cplx) * 9 * V);
+    for (long v = 0; v < V; v++)
+        for (int l = 0; l < 3; l++) f[l * V + v] = u[3 * v + l];
+    orc_fft3d((double *)f, n1, n2, n3, 3, -1);
+    for (int k = 1; k <= n3; k++)
+        for (int j = 1; j <= n2; j++)
+            for (int i = 1; i <= n1; i++) {
+                long v = (i - 1) + (long)n1 * ((j - 1) + (long)n2 * (k - 1));
+                double xk[3];
+                orc_spatial_freq(i, j, k, n1, n2, n3, xk);
+                cplx km[3];
+                if (ideriv == 2) {
+                    cplx base = I * 0.25 * (1.0 + cexp(I * xk[0])) * (1.0 + cexp(I * xk[1])) * (1.0 + cexp(I * xk[2]));
+                    for (int l = 0; l < 3; l++) km[l] = tan(xk[l] / 2.0) * base;
+                } else {
+                    for (int l = 0; l < 3; l++) km[l] = I * sin(xk[l]);
+                }
+                int allnyq = (n1 % 2 == 0 && i == n1 / 2 + 1) && (n2 % 2 == 0 && j == n2 / 2 + 1) &&
+                             (n3 % 2 == 0 && k == n3 / 2 + 1 && n3 > 1) && ideriv == 2;
+                for (int l = 0; l < 3; l++)
+                    for (int m = 0; m < 3; m++) g[(3 * l + m) * V + v] = allnyq ? 0.0 : f[l * V + v] * km[m];
+            }
+    orc_fft3d((double *)g, n1, n2, n3, 9, +1);
+    for (long v = 0; v < V; v++)
+        for (int t = 0; t < 9; t++) grad[9 * v + t] = creal(g[t * V + v]) / (double)V;
+    free(f);
+    free(g);
+}
+
+/* cell2node_tensor1 (various_functions.f90:2652-2691): node (i, j, k) = mean of the 8 cells
+ * around it, periodic.  (The reference wraps the x index against npts2, which is the same as
+ * against npts1 on the grids where its loop stays in bounds, n1 = n2.) */
+static void cell2node3(const double *cell, double *node, int n1, int n2, int n3) {
+    for (int k = 1; k <= n3 + 1; k++) {
+        int k1 = k - 1 == 0 ? n3 : k - 1, k2 = k == n3 + 1 ? 1 : k;
+        for (int j = 1; j <= n2 + 1; j++) {
+            int j1 = j - 1 == 0 ? n2 : j - 1, j2 = j == n2 + 1 ? 1 : j;
+            for (int i = 1; i <= n1 + 1; i++) {
+                int i1 = i - 1 == 0 ? n1 : i - 1, i2 = i == n1 + 1 ? 1 : i;
+#define CELL(a, b, cc) (cell + 3 * ((a - 1) + (long)n1 * ((b - 1) + (long)n2 * (cc - 1))))
+                const double *p[8] = {CELL(i2, j2, k2), CELL(i1, j2, k2), CELL(i2, j1, k2), CELL(i2, j2, k1),
+                                      CELL(i1, j1, k2), CELL(i1, j2, k1), CELL(i2, j1, k1), CELL(i1, j1, k1)};
+#undef CELL
+                double *o = node + 3 * ((i - 1) + (long)(n1 + 1) * ((j - 1) + (long)(n2 + 1) * (k - 1)));
+                for (int l = 0; l < 3; l++) {
+                    double s = p[0][l];
+                    for (int q = 1; q < 8; q++) s += p[q][l];
+                    o[l] = s * 0.125;
+                }
+            }
+        }
+    }
+}
+
+/* init_defgrad (IO_functions.f90:815-936): F = grad(u0) + diag(delt) with delt = (4, 4, 4) (the
+ * value load_input gives it, :62-66), F0 = F, the J-weighted averages of F^-1 and the phase
+ * weights, w_c = wgt J / <J>; the nodal coordinates move by the nodal displacement */
+static void init_defgrad(orc_ctx *c) {
+    long V = c->V;
+    int n1 = c->n1, n2 = c->n2, n3 = c->n3, n3node = n3 == 1 ? 1 : n3 + 1;
+    long nnode = (long)(n1 + 1) * (n2 + 1) * (n3 + 1);
+    double *u = malloc(sizeof(double) * 3 * V), *node = calloc((size_t)3 * nnode, sizeof(double));
+    double *grad = malloc(sizeof(double) * 9 * V);
+    int ideriv;
+    if (c->idispini == 1) {
+        memcpy(u, c->dispini, sizeof(double) * 3 * V);
+        cell2node3(u, node, n1, n2, n3);
+        ideriv = 1;
+    } else {
+        long nn = (long)(n1 + 1) * (n2 + 1) * n3node;
+        memcpy(node, c->dispini, sizeof(double) * 3 * nn);
+        for (int k = 0; k < n3; k++)
+            for (int j = 0; j < n2; j++)
+                for (int i = 0; i < n1; i++)
+                    for (int l = 0; l < 3; l++)
+                        u[3 * (i + (long)n1 * (j + (long)n2 * k)) + l] =
+                            node[3 * (i + (long)(n1 + 1) * (j + (long)(n2 + 1) * k)) + l];
+        ideriv = 2;
+    }
+    orc_gradient_tensor1(u, grad, ideriv, n1, n2, n3);
+    for (int k = 0; k < n3node; k++)
+        for (int j = 0; j < n2 + 1; j++)
+            for (int i = 0; i < n1 + 1; i++)
+                for (int l = 0; l < 3; l++) {
+                    long q = 3L * (i + (long)(n1 + 1) * (j + (long)(n2 + 1) * k)) + l;
+                    c->xnode[q] += node[q];
+                }
+    memset(c->defgradavg, 0, sizeof c->defgradavg);
+    c->defgradavg[0] = c->defgradavg[4] = c->defgradavg[8] = 4.0;
+    double inv_c[9] = {0}, inv_s[9] = {0}, inv_g[9] = {0}, detFavg = 0.0;
+    c->wphcsol = c->wphcgas = 0.0;
+    if (!c->defgradini) c->defgradini = malloc(sizeof(double) * 9 * V);
+    for (long v = 0; v < V; v++) {
+        double *F = c->defgrad + 9 * v, Fi[9];
+        for (int t = 0; t < 9; t++) F[t] = grad[9 * v + t] + c->defgradavg[t];
+        memcpy(c->defgradini + 9 * v, F, sizeof Fi);
+        c->detF[v] = det33(F);
+        memcpy(Fi, F, sizeof Fi);
+        lu_inverse(Fi, 3);
+        memcpy(c->defgradinv + 9 * v, Fi, sizeof Fi);
+        const double wj = c->wgt * c->detF[v];
+        for (int t = 0; t < 9; t++) {
+            inv_c[t] += Fi[t] * wj;
+            if (c->gas[v]) inv_g[t] += Fi[t] * wj; else inv_s[t] += Fi[t] * wj;
+        }
+        if (c->gas[v]) c->wphcgas += wj; else c->wphcsol += wj;
+        c->wgtc[v] = wj;
+        detFavg += wj;
+    }
+    c->wphcsol /= detFavg;
+    c->wphcgas /= detFavg;
+    for (int t = 0; t < 9; t++) {
+        c->defgradinvavgc[t] = inv_c[t] / detFavg;
+        c->defgradinvavgcs[t] = inv_s[t] / detFavg / c->wphcsol;
+        c->defgradinvavgcg[t] = c->wphcgas > 0.0 ? inv_g[t] / detFavg / c->wphcgas : ID3[t];
+    }
+    for (long v = 0; v < V; v++) c->wgtc[v] /= detFavg;
+    free(u);
+    free(node);
+    free(grad);
+}
 
 /* ------------------------------------------------------------------------ */
 /* load_input tail + data_grain (IO_functions.f90:100-199, 759-809, 347-363)  */
@@ -661,6 +804,7 @@ int orc_init(orc_ctx *c) {
         lu_inverse(c66, 6);
         memcpy(c->sg66 + 36 * v, c66, sizeof c66);
     }
+    if (c->idispini > 0) init_defgrad(c); /* read_input, IO_functions.f90:152-154 */
     /* Hall-Petch startup shift (IO_functions.f90:107-199) */
     const double hp_k = 632.0 / 2.5, voxel_vol = 64.0;
     const double grain_dmin = 3.0 * pow(voxel_vol, 1.0 / 3.0);
@@ -699,8 +843,10 @@ int orc_init(orc_ctx *c) {
     free(nvox);
     c->wphsol = c->wphgas = 0;
     for (long v = 0; v < V; v++) { if (c->gas[v]) c->wphgas += c->wgt; else c->wphsol += c->wgt; }
-    c->wphcgas = c->wphgas;
-    c->wphcsol = c->wphsol;
+    if (c->idispini == 0) { /* IO_functions.f90:358-361 (else init_defgrad's J-weighted ones) */
+        c->wphcgas = c->wphgas;
+        c->wphcsol = c->wphsol;
+    }
 
     /* initial_guess (various_functions.f90:8-53) */
     orc_process *p1 = &c->proc[0];
@@ -1500,7 +1646,8 @@ static void update_elstic_defgrad(orc_ctx *c) {
         double inv_old[9], fpi[9];
         memcpy(inv_old, c->defgrade + 9 * v, sizeof inv_old);
         lu_inverse(inv_old, 3);
-        memcpy(fpi, c->defgradp + 9 * v, sizeof fpi); /* Fp * Fini with Fini = I */
+        if (c->defgradini) mm33(c->defgradp + 9 * v, c->defgradini + 9 * v, fpi); /* Fp Fini */
+        else memcpy(fpi, c->defgradp + 9 * v, sizeof fpi);                        /* Fini = I */
         lu_inverse(fpi, 3);
         mm33(c->defgrad + 9 * v, fpi, c->defgrade + 9 * v);
         mm33(c->defgrade + 9 * v, inv_old, c->defgradeinc + 9 * v);
@@ -2096,7 +2243,7 @@ static double *field_ptr(orc_ctx *c, const char *name, int *ncomp) {
         {"trialtau", c->trialtau, 2 * c->ns}, {"gamdot", c->gamdot, c->ns}, {"gacumgr", c->gacumgr, 1},
         {"edotp", c->edotp, 9}, {"ept", c->ept, 9}, {"disgradsym", c->disgradsym, 9}, {"epvm", c->epvm, 1},
         {"eth", c->eth, 9},
-        {"defgradinc", c->defgradinc, 9}, {"defgradeinc", c->defgradeinc, 9},
+        {"defgradinc", c->defgradinc, 9}, {"defgradeinc", c->defgradeinc, 9}, {"defgradini", c->defgradini, 9},
     };
     for (size_t i = 0; i < sizeof T / sizeof T[0]; i++)
         if (!strcmp(T[i].n, name)) { *ncomp = T[i].k; return T[i].p; }

@@ -79,6 +79,10 @@ int orc_set_phase(orc_ctx *c, int iph, int igas, int nsyst, int nmodes, const in
 /* Voxels in x-fastest order: euler (V,3) degrees, grain/phase 1-based. */
 int orc_set_voxels(orc_ctx *c, const double *euler, const int *grain, const int *phase);
 int orc_add_process(orc_ctx *c, const orc_process *p);
+/* initial displacement field (idispini 1: 3 values per cell, 2: per node), before orc_init */
+int orc_set_displacement(orc_ctx *c, int idispini, const double *disp);
+/* gradient_tensor1 (various_functions.f90:56-112): u [v][3] -> grad [v][9], grad(l,m) = d_m u_l */
+void orc_gradient_tensor1(const double *u, double *grad, int ideriv, int n1, int n2, int n3);
 /* Initial state: load_input tail (Hall-Petch, wphsol), initial_guess and the
  * initial C0 (LS-EVPFFT.f90:149-173). */
 int orc_init(orc_ctx *c);
