@@ -628,7 +628,7 @@ int cg_begin_increment(cg_ctx* c, const cg_process* p, int do_schmid) {
     memcpy(c->scauchy_snap, p->scauchy, sizeof c->scauchy_snap);
     launch_snapshot(c->f, 0, c->s);
     PhaseTables pt{c->d_ph, c->d_tab};
-    if (do_schmid) launch_update_schmid(c->f, pt, c->s);
+    if (do_schmid) launch_update_schmid(c->f, pt, c->modal, c->s);
     if (c->modal) {  // form_G: Goperr0 = (1/V) sum Gamma(xi)
         int nb;
         launch_goperr0(c->f, c->c0, c->s0, p->igamma, c->s, &nb);
@@ -1210,7 +1210,7 @@ int cg_end_increment(cg_ctx* c, const cg_process* p, int scan_diagnostics, cg_in
         mm3(dm, inv, dud);
         launch_tensor_config_b(f, dud, c->s);
     }
-    if (p->iuphard == 1) launch_hardening(f, pt, tdot, c->s);
+    if (p->iuphard == 1) launch_hardening(f, pt, tdot, c->small_ns, c->sym_tau, c->s);
     // macro averages (LS-EVPFFT.f90:460-523)
     double vs[9];
     sym3(c->velgradavg, vs);
@@ -1379,6 +1379,10 @@ int cg_get_field(cg_ctx* c, const char* name, double* out, size_t n) {
     if (!field_desc(c, name, d) || !d.p) return fail(c, -1, "unknown or absent field %s", name);
     const long V = c->V;
     if (n < (size_t)d.ncomp * V) return fail(c, -1, "output buffer too small");
+    if (d.p == c->f.sch && !c->modal) {  // not kept by this context's solve: expand A, A^-1
+        PhaseTables pt{c->d_ph, c->d_tab};
+        launch_sch_from_A(c->f, pt, c->s);
+    }
     if (d.p == c->f.Lold && c->lold_same) {  // velgradold == velgrad since the last pass
         launch_copy(c->f.Lold, c->f.L, 9 * c->f.ld, c->s);
         c->lold_same = 0;

@@ -111,7 +111,10 @@ void launch_correct_solve_full(const Fields& f, const PhaseTables& pt, const Pas
 void launch_correct_solve_batch(const SolveWork* work, const int* blk_rve, int nblocks, int modal, int int_pow,
                                 int sym_tau, int small_ns, hipStream_t s);
 int solve_blocks(long V);  // blocks of one RVE in the constitutive launch
-void launch_update_schmid(const Fields& f, const PhaseTables& pt, hipStream_t s);
+// write_sch: also store the Schmid tensors (only the modified-AL solve reads them; the others
+// rebuild them from A and A^-1)
+void launch_update_schmid(const Fields& f, const PhaseTables& pt, int write_sch, hipStream_t s);
+void launch_sch_from_A(const Fields& f, const PhaseTables& pt, hipStream_t s);
 void launch_c066mod(const Fields& f, const double* c066, const double* goperr0, int modal, hipStream_t s);
 void launch_gas_compl(const Fields& f, double tdot, double complgas, int modal, hipStream_t s);
 void launch_goperr0(const Fields& f, const double* c0, const double* s0, int igamma, hipStream_t s, int* nb);
@@ -122,7 +125,7 @@ void launch_div_w(const Fields& f, double detfavg, hipStream_t s);
 void launch_elastic_stiff(const Fields& f, const PhaseTables& pt, hipStream_t s);
 void launch_tensor_config_a(const Fields& f, hipStream_t s, int* nb);
 void launch_tensor_config_b(const Fields& f, const double* dud, hipStream_t s);
-void launch_hardening(const Fields& f, const PhaseTables& pt, double tdot, hipStream_t s);
+void launch_hardening(const Fields& f, const PhaseTables& pt, double tdot, int small_ns, int sym_tau, hipStream_t s);
 void launch_solve_ops(const Fields& f, double tdot, hipStream_t s);
 void launch_taun(const Fields& f, const PhaseTables& pt, int sym, hipStream_t s);
 void launch_averages(const Fields& f, const PhaseTables& pt, hipStream_t s, int* nb);

@@ -240,6 +240,42 @@ __device__ __forceinline__ void slip_eval(const Fields& f, const double* __restr
         dd[14] += s4 * t4;
     }
 }
+// slip_eval with the Schmid tensors rebuilt from the A, A^-1 that update_schmid stored (the
+// same schmid5), for kernels of contexts that do not keep sch
+template <bool INTPOW>
+__device__ __forceinline__ void slip_eval_A(const Fields& f, const double* __restrict__ tab, const PhaseDev& P,
+                                            const Vox& x, const double* sg, double* ep, double* dd) {
+    double A[9], Ai[9];
+    for (int k = 0; k < 9; k++) {
+        A[k] = x.ld(f.schA, k);
+        Ai[k] = x.ld(f.schA, 9 + k);
+    }
+    for (int i = 0; i < 5; i++) ep[i] = 0.0;
+    for (int i = 0; i < 15; i++) dd[i] = 0.0;
+    const double* nrs = tab + P.off_nrs;
+    const double* g0 = tab + P.off_gamd0;
+#pragma unroll 1
+    for (int is = 0; is < P.nsyst; is++) {
+        double sv[5];
+        schmid5(A, Ai, tab + P.off_dbca + 3 * is, tab + P.off_dnca + 3 * is, sv);
+        const double s0 = sv[0], s1 = sv[1], s2 = sv[2], s3 = sv[3], s4 = sv[4];
+        double rss = s0 * sg[0] + s1 * sg[1] + s2 * sg[2] + s3 * sg[3] + s4 * sg[4];
+        const int isg = rss < 0.0 ? 1 : 0;
+        const double tau = x.ld(f.ttau, 2 * is + isg);
+        rss = rss / tau;
+        const double n = nrs[is];
+        const double dum = g0[is] * powm1<INTPOW>(fabs(rss), n);
+        const double r1 = n * dum / tau;
+        const double r2 = rss * dum;
+        ep[0] += s0 * r2; ep[1] += s1 * r2; ep[2] += s2 * r2; ep[3] += s3 * r2; ep[4] += s4 * r2;
+        const double t0 = s0 * r1, t1 = s1 * r1, t2 = s2 * r1, t3 = s3 * r1, t4 = s4 * r1;
+        dd[0] += s0 * t0; dd[1] += s0 * t1; dd[2] += s0 * t2; dd[3] += s0 * t3; dd[4] += s0 * t4;
+        dd[5] += s1 * t1; dd[6] += s1 * t2; dd[7] += s1 * t3; dd[8] += s1 * t4;
+        dd[9] += s2 * t2; dd[10] += s2 * t3; dd[11] += s2 * t4;
+        dd[12] += s3 * t3; dd[13] += s3 * t4;
+        dd[14] += s4 * t4;
+    }
+}
 // slip_eval for the symmetric Newton solve: the Schmid tensors are rebuilt from A, A^-1
 // (this thread's column of an LDS array [18][SBS]) and the wave-uniform crystal b, n, so
 // an evaluation reads 2 n_s trial strengths from memory instead of 5 n_s Schmid
@@ -418,8 +454,10 @@ __device__ __forceinline__ double harden_voxel(const Fields& f, TB tab, const PH
 // system, unused rates weighted by zero), so the rates, gacc and the first chunk's crss
 // and old strengths are in flight together, and each later chunk of HC systems is read
 // while the previous one computes.  The sums run over the same terms in the same order
-// as harden_voxel (the added terms are exact zeros).
-template <bool SYMTAU, class PH, class TB>
+// as harden_voxel (the added terms are exact zeros).  FINAL: the increment's last harden
+// (update_hardening): crss and both strength columns take the new values, gacc the new
+// accumulated shear, and the old strengths are not read.
+template <bool SYMTAU, bool FINAL = false, class PH, class TB>
 __device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH& P, const Vox& x, double tdot) {
     const int ns = __builtin_amdgcn_readfirstlane(P.nsyst);
     const int oh = __builtin_amdgcn_readfirstlane(P.off_hard);
@@ -431,10 +469,10 @@ __device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH
         for (int k = 0; k < HC; k++) {
             const int i = clampi(c0 + k);
             c2[2 * k] = x.ld(f.crss, 2 * i);
-            o2[2 * k] = x.ld(f.ttau, 2 * i);
+            o2[2 * k] = FINAL ? 0.0 : x.ld(f.ttau, 2 * i);
             // with equal forward and backward strengths the forward column stands for both
             c2[2 * k + 1] = SYMTAU ? c2[2 * k] : x.ld(f.crss, 2 * i + 1);
-            o2[2 * k + 1] = SYMTAU ? o2[2 * k] : x.ld(f.ttau, 2 * i + 1);
+            o2[2 * k + 1] = SYMTAU || FINAL ? o2[2 * k] : x.ld(f.ttau, 2 * i + 1);
         }
     };
     load_chunk(0, cr, ot);
@@ -474,11 +512,15 @@ __device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH
             const double t0 = cr[2 * k] + dtau * sc;
             const double t1 = cr[2 * k + 1] + dtau * sc;
             const double o0 = ot[2 * k], o1 = ot[2 * k + 1];
-            e2 += (o0 - t0) * (o0 - t0) + (o1 - t1) * (o1 - t1);
+            if (!FINAL) e2 += (o0 - t0) * (o0 - t0) + (o1 - t1) * (o1 - t1);
             tn2 += t0 * t0;
             tn2 += t1 * t1;
             x.st(f.ttau, 2 * is, t0);
-            if (!SYMTAU) x.st(f.ttau, 2 * is + 1, t1);
+            if (!SYMTAU || FINAL) x.st(f.ttau, 2 * is + 1, t1);
+            if (FINAL) {
+                x.st(f.crss, 2 * is, t0);
+                x.st(f.crss, 2 * is + 1, t1);
+            }
         }
 #pragma unroll
         for (int k = 0; k < 2 * HC; k++) {
@@ -487,6 +529,7 @@ __device__ __forceinline__ double harden_small(const Fields& f, TB tab, const PH
         }
     }
     x.st(f.taun, 0, sqrt(tn2));
+    if (FINAL) x.st(f.gacc, 0, gamma1);
     return e2;
 }
 
@@ -1140,7 +1183,7 @@ void launch_correct_solve_batch(const SolveWork* work, const int* blk_rve, int n
 // ---------------------------------------------------------------------------
 // update_schmid (various_functions.f90:472-539): sch = b5(sym(A (b x n) A^-1)), A = Fe ag
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BS) void k_update_schmid(Fields f, PhaseTables pt) {
+__global__ __launch_bounds__(BS) void k_update_schmid(Fields f, PhaseTables pt, int write_sch) {
     const long V = f.V;
     const long v = (long)blockIdx.x * BS + threadIdx.x;
     if (v >= V || f.gas[v]) return;
@@ -1155,14 +1198,35 @@ __global__ __launch_bounds__(BS) void k_update_schmid(Fields f, PhaseTables pt) 
         f.schA[c * f.ld + v] = A[c];
         f.schA[(9 + c) * f.ld + v] = Ai[c];
     }
+    if (!write_sch) return;
     for (int is = 0; is < P.nsyst; is++) {
         double s5[5];
         schmid5(A, Ai, db + 3 * is, dn + 3 * is, s5);
         for (int c = 0; c < 5; c++) f.sch[(5 * is + c) * f.ld + v] = s5[c];
     }
 }
-void launch_update_schmid(const Fields& f, const PhaseTables& pt, hipStream_t s) {
-    hipLaunchKernelGGL(k_update_schmid, dim3(nblk(f.V)), dim3(BS), 0, s, f, pt);
+void launch_update_schmid(const Fields& f, const PhaseTables& pt, int write_sch, hipStream_t s) {
+    hipLaunchKernelGGL(k_update_schmid, dim3(nblk(f.V)), dim3(BS), 0, s, f, pt, write_sch);
+}
+// sch from the A, A^-1 that update_schmid stored (what it would have written), for a host
+// read of sch in a context whose solve does not keep it
+__global__ __launch_bounds__(BS) void k_sch_from_A(Fields f, PhaseTables pt) {
+    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    if (v >= f.V || f.gas[v]) return;
+    const PhaseDev& P = pt.ph[f.phase[v]];
+    double A[9], Ai[9];
+    for (int c = 0; c < 9; c++) {
+        A[c] = f.schA[c * f.ld + v];
+        Ai[c] = f.schA[(9 + c) * f.ld + v];
+    }
+    for (int is = 0; is < P.nsyst; is++) {
+        double s5[5];
+        schmid5(A, Ai, pt.data + P.off_dbca + 3 * is, pt.data + P.off_dnca + 3 * is, s5);
+        for (int c = 0; c < 5; c++) f.sch[(5 * is + c) * f.ld + v] = s5[c];
+    }
+}
+void launch_sch_from_A(const Fields& f, const PhaseTables& pt, hipStream_t s) {
+    hipLaunchKernelGGL(k_sch_from_A, dim3(nblk(f.V)), dim3(BS), 0, s, f, pt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1544,6 +1608,9 @@ void launch_tensor_config_b(const Fields& f, const double* dud, hipStream_t s) {
 }
 
 // update_hardening (:1493-1523)
+// SMALLNS (one mode of at most 12 systems in every phase): the branch-free harden of the solve
+// in its FINAL form; else the general harden_voxel.  Same sums in the same order.
+template <bool SMALLNS, bool SYMTAU>
 __global__ __launch_bounds__(BS) void k_hardening(Fields f, PhaseTables pt, double tdot) {
     const long V = f.V;
     const long v = (long)blockIdx.x * BS + threadIdx.x;
@@ -1554,9 +1621,13 @@ __global__ __launch_bounds__(BS) void k_hardening(Fields f, PhaseTables pt, doub
     for (;;) {
         const int p0 = __builtin_amdgcn_readfirstlane(ph);
         if (ph == p0) {
-            double g1;
-            harden_voxel<true>(f, (cdbl*)pt.data, ((cphase*)pt.ph)[p0], Vox(v, f.ld), tdot, &g1, true);
-            f.gacc[v] = g1;
+            if (SMALLNS) {
+                harden_small<SYMTAU, true>(f, (cdbl*)pt.data, ((cphase*)pt.ph)[p0], Vox(v, f.ld), tdot);
+            } else {
+                double g1;
+                harden_voxel<true>(f, (cdbl*)pt.data, ((cphase*)pt.ph)[p0], Vox(v, f.ld), tdot, &g1, true);
+                f.gacc[v] = g1;
+            }
             break;
         }
     }
@@ -1618,8 +1689,11 @@ void launch_solve_ops(const Fields& f, double tdot, hipStream_t s) {
     hipLaunchKernelGGL(k_solve_ops, dim3(nblk(f.V)), dim3(BS), 0, s, f, tdot);
 }
 
-void launch_hardening(const Fields& f, const PhaseTables& pt, double tdot, hipStream_t s) {
-    hipLaunchKernelGGL(k_hardening, dim3(nblk(f.V)), dim3(BS), 0, s, f, pt, tdot);
+void launch_hardening(const Fields& f, const PhaseTables& pt, double tdot, int small_ns, int sym_tau, hipStream_t s) {
+    const dim3 g(nblk(f.V)), b(BS);
+    if (small_ns && sym_tau) hipLaunchKernelGGL((k_hardening<true, true>), g, b, 0, s, f, pt, tdot);
+    else if (small_ns) hipLaunchKernelGGL((k_hardening<true, false>), g, b, 0, s, f, pt, tdot);
+    else hipLaunchKernelGGL((k_hardening<false, false>), g, b, 0, s, f, pt, tdot);
 }
 
 // macro averages (LS-EVPFFT.f90:460-523): 50 sums over solid voxels
@@ -1669,17 +1743,19 @@ void launch_averages(const Fields& f, const PhaseTables& pt, hipStream_t s, int*
 //   sum w T (36) and sum wgt N T N^T J (36),  N_{ca} = b_c : (b_a Fi^T)
 // ---------------------------------------------------------------------------
 template <bool INTPOW>
-__global__ __launch_bounds__(BS) void k_calc_c0(Fields f, PhaseTables pt, double tdot, double tdotref) {
+__global__ __launch_bounds__(SBS) void k_calc_c0(Fields f, PhaseTables pt, double tdot, double tdotref) {
+    __shared__ double sm[24 * SBS];  // block sums, 24 of the 72 values at a time
     const long V = f.V;
-    const long v = (long)blockIdx.x * BS + threadIdx.x;
+    const long v = (long)blockIdx.x * SBS + threadIdx.x;
     double acc[72];
     for (int i = 0; i < 72; i++) acc[i] = 0.0;
     if (v < V && !f.gas[v]) {
         const PhaseDev& P = pt.ph[f.phase[v]];
         double sg[6], ep[5], dd[15], T[36];
         for (int c = 0; c < 6; c++) sg[c] = f.sg[c * f.ld + v];
-        // plastic_strain_rate on a local copy (gdot is not modified: STORE=false)
-        slip_eval<INTPOW, false>(f, pt.data, P, Vox(v, f.ld), sg, ep, dd);
+        // plastic_strain_rate on a local copy (gdot is not modified), the Schmid tensors
+        // rebuilt from the A, A^-1 of the increment's update_schmid
+        slip_eval_A<INTPOW>(f, pt.data, P, Vox(v, f.ld), sg, ep, dd);
         for (int i = 0; i < 6; i++)
             for (int j = 0; j < 6; j++) {
                 double e = f.sg66[(long)(6 * i + j) * f.ld + v] / tdot * tdot / tdotref;
@@ -1697,13 +1773,16 @@ __global__ __launch_bounds__(BS) void k_calc_c0(Fields f, PhaseTables pt, double
             acc[36 + i] = dP[i] * J * wg;
         }
     }
-    block_reduce_store<72, BS>(acc, f.part);
+    for (int k = 0; k < 3; k++) {
+        block_reduce_lds<24, SBS>(acc + 24 * k, f.part, 72, 24 * k, sm);
+        __syncthreads();  // the last readers of sm are done before the next chunk writes it
+    }
 }
 void launch_calc_c0(const Fields& f, const PhaseTables& pt, double tdot, double tdotref, hipStream_t s, int* nb,
                     int int_pow) {
-    *nb = nblk(f.V);
-    if (int_pow) hipLaunchKernelGGL(k_calc_c0<true>, dim3(*nb), dim3(BS), 0, s, f, pt, tdot, tdotref);
-    else hipLaunchKernelGGL(k_calc_c0<false>, dim3(*nb), dim3(BS), 0, s, f, pt, tdot, tdotref);
+    *nb = (int)((f.V + SBS - 1) / SBS);
+    if (int_pow) hipLaunchKernelGGL(k_calc_c0<true>, dim3(*nb), dim3(SBS), 0, s, f, pt, tdot, tdotref);
+    else hipLaunchKernelGGL(k_calc_c0<false>, dim3(*nb), dim3(SBS), 0, s, f, pt, tdot, tdotref);
 }
 
 // ---------------------------------------------------------------------------
