@@ -68,13 +68,13 @@ def flops_constitutive(stats, ns):
     return nr * 932 + ev * per_eval + (ev - nr) * 198 + outer * (2 * ns * ns + 10 * ns + 20)
 
 
-def run_gpu(problem, precision, steps, warmup):
+def run_gpu(problem, precision, steps, warmup, graphs=False):
     import torch
 
     from creep_modeling_amd import driver
 
     torch.cuda.init()
-    s = driver.Solver(problem, precision=precision, scan_cond=True)
+    s = driver.Solver(problem, precision=precision, scan_cond=True, graphs=graphs)
     lib = s.L
     for i in range(warmup):
         r = s.step()
@@ -277,6 +277,7 @@ def main():
     ap.add_argument("--s33", type=float, default=None, help="creep stress sigma33 in MPa (config default 300)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--graphs", action="store_true", help="replay each pass' fixed device work as HIP graphs")
     ap.add_argument("--cpu-passes", type=int, default=8,
                     help="fixed-point passes of the CPU oracle's bounded sample (first increment)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -315,7 +316,7 @@ def main():
     V = problem.nvox
     ns = problem.phases[0].nsyst
 
-    rows, st, T = run_gpu(problem, 0, args.steps, args.warmup)
+    rows, st, T = run_gpu(problem, 0, args.steps, args.warmup, args.graphs)
     passes = sum(r["passes"] for r in rows)
     value = passes * V / T
     ms_per_step = T / args.steps * 1e3
@@ -373,7 +374,7 @@ def main():
                          "pow_per_voxel_iter": st.evals * ns / max(st.voxel_iters, 1)},
     }
     if not args.no_mixed:
-        rows_m, st_m, T_m = run_gpu(problem, 1, args.steps, args.warmup)
+        rows_m, st_m, T_m = run_gpu(problem, 1, args.steps, args.warmup, args.graphs)
         passes_m = sum(r["passes"] for r in rows_m)
         vm = passes_m * V / T_m
         t_fft_m = st_m.t_ms[0] / max(st_m.passes, 1)

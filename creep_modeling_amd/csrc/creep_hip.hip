@@ -170,8 +170,10 @@ struct cg_ctx {
     double cur_scale = 0.0;     // the scale d_scale holds (0: not yet written)
     // HIP graphs of a pass' fixed device work: pre = FFT + Gamma stage and the sums of
     // w dLref F^-1; post = the solve's sums, <sigma> for the next pass and their copy to the host.
-    // pre is captured again when C0 / S0, igamma or the FFT shapes change.
-    int use_graphs = 1;
+    // pre is captured again when C0 / S0, igamma or the FFT shapes change.  Off by default: the
+    // stream never drains between a pass' kernels, and a graph launch delays the first kernel of
+    // the stage (1% slower at 128^3, no gain on the 32^3-96^3 sweep).
+    int use_graphs = 0;
     hipGraphExec_t g_pre = nullptr, g_post = nullptr;
     long green_version = 0, g_pre_version = -1;
     int g_pre_igamma = -1, tune_version = 0, g_pre_tune = -1;

@@ -56,15 +56,15 @@ class Solver:
     """One RVE: fixed-point increments on the GPU, one accepted increment per :meth:`step`."""
 
     def __init__(self, problem, precision=0, modal=False, do_mix=True, scan_cond=True, max_pass_factor=100,
-                 tune_file=None, fft_shapes=None, graphs=True):
+                 tune_file=None, fft_shapes=None, graphs=False):
         self.problem = problem
         self.ctx = _lib.Context(problem, precision=precision, modal=modal, do_mix=do_mix)
         if tune_file:  # FFT shape cache (--fftw_save)
             self.ctx.set_tuning_file(tune_file)
         if fft_shapes is not None:  # fixed FFT shapes, no tuning
             self.ctx.set_fft_tuning(fft_shapes)
-        if not graphs:  # eager launches instead of the per-pass HIP graphs
-            self.ctx.set_graphs(False)
+        if graphs:  # replay each pass' fixed device work as HIP graphs
+            self.ctx.set_graphs(True)
         self.L = self.ctx.L
         self.h = self.ctx.h
         self.procs = [_lib.process_struct(p) for p in problem.processes]

@@ -247,7 +247,7 @@ class Context:
         return bool(rc), sh.tolist(), us.tolist()
 
     def set_graphs(self, on):
-        """Replay each pass' fixed device work as HIP graphs (default) or launch it eagerly."""
+        """Replay each pass' fixed device work as HIP graphs, or launch it eagerly (default)."""
         self._chk(self.L.cg_set_graphs(self.h, int(bool(on))), "cg_set_graphs")
 
     def set_fft_tuning(self, shapes):

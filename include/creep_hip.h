@@ -216,8 +216,8 @@ int cg_get_fft_tuning(cg_ctx *ctx, int *shape8, double *us5);
  * counterpart. */
 int cg_set_fft_tuning(cg_ctx *ctx, const int *shape8);
 
-/* Replay each pass' fixed device work as HIP graphs (1, the default) or launch it kernel by
- * kernel (0): the FFT + Gamma stage with its reduction, and the reductions and host copy after
+/* Replay each pass' fixed device work as HIP graphs (1) or launch it kernel by kernel (0, the
+ * default): the FFT + Gamma stage with its reduction, and the reductions and host copy after
  * the solve; captured at their first use and again when C0, igamma or the FFT shapes change.
  * The results are the same either way.  No reference counterpart. */
 int cg_set_graphs(cg_ctx *ctx, int on);
