@@ -68,7 +68,14 @@ def flops_constitutive(stats, ns):
     return nr * 932 + ev * per_eval + (ev - nr) * 198 + outer * (2 * ns * ns + 10 * ns + 20)
 
 
+STAGE_STEPS = 3  # increments after the timed region that time every stage
+
+
 def run_gpu(problem, precision, steps, warmup, graphs=False):
+    """warm-up increments, then `steps` timed increments (wall clock; the constitutive kernel's own
+    launch events, cg_set_timing mode 2, which adds nothing between kernels), then STAGE_STEPS
+    increments with every stage timed by stream-marker events (mode 1) for the stage breakdown.
+    Returns (timed rows, timed stats, wall seconds, stage stats)."""
     import torch
 
     from creep_modeling_amd import driver
@@ -79,7 +86,7 @@ def run_gpu(problem, precision, steps, warmup, graphs=False):
     for i in range(warmup):
         r = s.step()
         print("warm-up increment %d: %d passes" % (i + 1, r["passes"]), file=sys.stderr, flush=True)
-    lib.cg_set_timing(s.h, 1)
+    lib.cg_set_timing(s.h, 2)
     s.ctx.stats(reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -90,8 +97,14 @@ def run_gpu(problem, precision, steps, warmup, graphs=False):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     st = s.ctx.stats()
+    lib.cg_set_timing(s.h, 1)
+    s.ctx.stats(reset=True)
+    for i in range(STAGE_STEPS):
+        r = s.step()
+        print("stage-timing increment %d: %d passes" % (i + 1, r["passes"]), file=sys.stderr, flush=True)
+    st_stage = s.ctx.stats()
     s.close()
-    return rows, st, t1 - t0
+    return rows, st, t1 - t0, st_stage
 
 
 def host_cpu():
@@ -310,17 +323,17 @@ def main():
     if args.grains is None:
         args.grains = 2048 if args.config == 3 else 512
     ms, mat, procs = inputs.config_case(args.config, n=args.n, ngrains=args.grains,
-                                        nsteps=args.warmup + args.steps + 1, s33=args.s33)
+                                        nsteps=args.warmup + args.steps + STAGE_STEPS + 1, s33=args.s33)
     problem = parse.problem_from_case(ms, mat, procs)
     pr0 = procs[0]
     V = problem.nvox
     ns = problem.phases[0].nsyst
 
-    rows, st, T = run_gpu(problem, 0, args.steps, args.warmup, args.graphs)
+    rows, st, T, sst = run_gpu(problem, 0, args.steps, args.warmup, args.graphs)
     passes = sum(r["passes"] for r in rows)
     value = passes * V / T
     ms_per_step = T / args.steps * 1e3
-    t_fft = st.t_ms[0] / max(st.passes, 1)
+    t_fft = sst.t_ms[0] / max(sst.passes, 1)
     t_solve = st.t_ms[1] / max(st.passes, 1)
     fft_bytes = fft_alg_bytes(args.n) * V
     fft_gbs = fft_bytes / (t_fft * 1e-3) / 1e9
@@ -363,8 +376,12 @@ def main():
                    "passes": passes, "voxels": V, "rve_increments_per_s": args.steps / T},
         "roofline": roof,
         "stages_ms_per_pass": {"fft_green": t_fft, "correct_solve": t_solve,
-                               "reduce_copy": st.t_ms[2] / max(st.passes, 1),
-                               "per_increment_updates_ms": st.t_ms[3] / max(args.steps, 1)},
+                               "sums_copy": sst.t_ms[2] / max(sst.passes, 1),
+                               "per_increment_updates_ms": st.t_ms[3] / max(args.steps, 1),
+                               "timing": "correct_solve and per-increment updates: the timed increments "
+                                         "(the solve's launch events); fft_green and sums_copy: stream-marker "
+                                         "events over the %d increments after them (%d passes)"
+                                         % (STAGE_STEPS, sst.passes)},
         "fft_green_roofline": {"bound": "hbm", "achieved": fft_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": fft_gbs / HBM_PEAK_GBS, "alg_bytes_per_voxel": fft_alg_bytes(args.n),
                                "traffic": measured("fft_green", "bytes_per_launch", args)},
@@ -374,10 +391,10 @@ def main():
                          "pow_per_voxel_iter": st.evals * ns / max(st.voxel_iters, 1)},
     }
     if not args.no_mixed:
-        rows_m, st_m, T_m = run_gpu(problem, 1, args.steps, args.warmup, args.graphs)
+        rows_m, st_m, T_m, sst_m = run_gpu(problem, 1, args.steps, args.warmup, args.graphs)
         passes_m = sum(r["passes"] for r in rows_m)
         vm = passes_m * V / T_m
-        t_fft_m = st_m.t_ms[0] / max(st_m.passes, 1)
+        t_fft_m = sst_m.t_ms[0] / max(sst_m.passes, 1)
         out["mixed"] = {"value": vm, "ratio_vs_fp64": vm / value, "passes": passes_m,
                         "fft_green_ms_per_pass": t_fft_m,
                         "fft_green_gbs_alg": fft_alg_bytes(args.n, 8) * V / (t_fft_m * 1e-3) / 1e9,

@@ -135,6 +135,11 @@ struct cg_ctx {
     FftCtx* fft = nullptr;
     hipStream_t s = nullptr;
     double* h_red = nullptr;   // pinned
+    // a pass' sums as the reduction kernel writes them (host-mapped, fine-grained) and its
+    // device address
+    double* h_pass = nullptr;
+    double* h_pass_dev = nullptr;
+
     double* d_sgsum = nullptr;
     double* d_fluct = nullptr;
     size_t part_cap = 0;
@@ -155,7 +160,7 @@ struct cg_ctx {
     double dsim_snap[9], scauchy_snap[9];
     // timing
     int timing = 0;
-    hipEvent_t ev[8];
+    hipEvent_t ev[12];
     cg_stats st;
     // FFT shape tuning: done at the first pass (or given by cg_set_fft_tuning), cached in the
     // tuning file when one is set
@@ -165,19 +170,36 @@ struct cg_ctx {
     int idispini = 0;
     std::vector<double> dispini;
     // the mixed form's stress scale as the register FFT kernels read it: {1/scale, scale/V}
-    double* d_scale = nullptr;
-    double* h_scale = nullptr;  // pinned staging of d_scale
-    double cur_scale = 0.0;     // the scale d_scale holds (0: not yet written)
+    // the previous pass' solid stress sums (9), from which the register FFT kernels take the
+    // mixed form's stress scale (zero before the first pass: scale 1)
+    double* d_sgsol = nullptr;
     // HIP graphs of a pass' fixed device work: pre = FFT + Gamma stage and the sums of
     // w dLref F^-1; post = the solve's sums, <sigma> for the next pass and their copy to the host.
     // pre is captured again when C0 / S0, igamma or the FFT shapes change.  Off by default: the
     // stream never drains between a pass' kernels, and a graph launch delays the first kernel of
     // the stage (1% slower at 128^3, no gain on the 32^3-96^3 sweep).
     int use_graphs = 0;
+    // Overlap of the host's part of a pass with the next pass' transforms: after cg_iterate has
+    // enqueued a pass' sums (written straight to host memory), it enqueues the x, y, z + Gamma
+    // and y passes of the next pass (they read only sigma, <sigma>, F^-1, J, C0 and, in the
+    // mixed form, the scale that the device derives from the same sums) and waits for the sums
+    // alone.  The next cg_iterate runs only the x inverse then, if nothing changed the state in
+    // between (any other entry point that writes state drops the early passes, and the next
+    // pass runs all five).  The x inverse, the only pass that writes a readable field
+    // (dvelgradref), always runs in its own pass.
+    int overlap = 1;
+    int early = 0;                  // the next pass' first four passes are enqueued
+    int early_igamma = -1, early_tune = -1;
+    long early_green = -1;
+    int early_slot = 0;             // timing events of the early passes: ev[8 + 2 slot], ev[9 + 2 slot]
+    int early_timed[2] = {0, 0};
+    hipEvent_t ev_sums = nullptr;   // the pass' sums have reached the host
     hipGraphExec_t g_pre = nullptr, g_post = nullptr;
     long green_version = 0, g_pre_version = -1;
     int g_pre_igamma = -1, tune_version = 0, g_pre_tune = -1;
 };
+
+static void drop_early(cg_ctx* c);
 
 static int fail(cg_ctx* c, int code, const char* fmt, ...) {
     if (c) {
@@ -310,7 +332,8 @@ int cg_create(int n1, int n2, int n3, int nph, double complgas, int modal, int d
     memset(&c->f, 0, sizeof c->f);
     memset(&c->st, 0, sizeof c->st);
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return -2; }
-    for (int i = 0; i < 8; i++) hipEventCreate(&c->ev[i]);
+    for (int i = 0; i < 12; i++) hipEventCreate(&c->ev[i]);
+    hipEventCreateWithFlags(&c->ev_sums, hipEventDisableTiming);
     *out = c;
     return 0;
 }
@@ -341,12 +364,13 @@ int cg_destroy(cg_ctx* c) {
     if (c->d_sgsum) hipFree(c->d_sgsum);
     if (c->d_fluct) hipFree(c->d_fluct);
     if (c->h_red) hipHostFree(c->h_red);
-    if (c->h_scale) hipHostFree(c->h_scale);
-    if (c->d_scale) hipFree(c->d_scale);
+    if (c->h_pass) hipHostFree(c->h_pass);
+    if (c->d_sgsol) hipFree(c->d_sgsol);
     if (c->g_pre) hipGraphExecDestroy(c->g_pre);
     if (c->g_post) hipGraphExecDestroy(c->g_post);
     if (c->fft) fft_destroy(c->fft);
-    for (int i = 0; i < 8; i++) hipEventDestroy(c->ev[i]);
+    for (int i = 0; i < 12; i++) hipEventDestroy(c->ev[i]);
+    if (c->ev_sums) hipEventDestroy(c->ev_sums);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
     return 0;
@@ -357,6 +381,7 @@ const char* cg_last_error(cg_ctx* c) { return c ? c->err : g_noctx_err; }
 int cg_set_phase(cg_ctx* c, int iph, int igas, int nsyst, int nmodes, const int* nsm, const double* dnca,
                  const double* dbca, const double* tau, const double* thet, const double* nrs,
                  const double* gamd0, const double* hpfac, const double* hard, const double* C) {
+    drop_early(c);
     if (!c) return -1;
     if (iph < 1 || iph > c->nph) return fail(c, -1, "phase index %d out of range", iph);
     HostPhase& P = c->ph[iph - 1];
@@ -383,6 +408,7 @@ int cg_set_phase(cg_ctx* c, int iph, int igas, int nsyst, int nmodes, const int*
 }
 
 int cg_set_voxels(cg_ctx* c, const double* euler, const int* grain, const int* phase) {
+    drop_early(c);
     if (!c) return -1;
     c->euler.assign(euler, euler + 3 * c->V);
     c->grain.assign(grain, grain + c->V);
@@ -394,6 +420,7 @@ int cg_set_voxels(cg_ctx* c, const double* euler, const int* grain, const int* p
 }
 
 int cg_init(cg_ctx* c, const cg_process* p1) {
+    drop_early(c);
     if (!c || !p1) return -1;
     if (!c->voxels_set) return fail(c, -1, "cg_set_voxels not called");
     for (int i = 0; i < c->nph; i++)
@@ -505,8 +532,10 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     launch_fill(f.red, 0.0, 256, c->s);  // the status slots start clear
     if (ensure_part(c, (size_t)((V + 63) / 64 + 64) * 81)) return -3;  // the solve may run 64-thread blocks
     HIPCHK(c, hipHostMalloc(&c->h_red, 256 * sizeof(double)));
-    HIPCHK(c, hipHostMalloc(&c->h_scale, 2 * sizeof(double)));
-    if (dalloc(c, &c->d_sgsum, 16) || dalloc(c, &c->d_fluct, 16) || dalloc(c, &c->d_scale, 2)) return -3;
+    HIPCHK(c, hipHostMalloc(&c->h_pass, R_NCOPY * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(c, hipHostGetDevicePointer((void**)&c->h_pass_dev, c->h_pass, 0));
+    if (dalloc(c, &c->d_sgsum, 16) || dalloc(c, &c->d_fluct, 16) || dalloc(c, &c->d_sgsol, 16)) return -3;
+    HIPCHK(c, hipMemset(c->d_sgsol, 0, 16 * sizeof(double)));
     c->fft = fft_create(f, c->err, sizeof c->err);
     if (!c->fft) return -5;
 
@@ -609,21 +638,43 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     return 0;
 }
 
+// Timing (cg_set_timing): 1 = every stage (marker events around the FFT + Gamma stage and the
+// sums, and around the per-increment work), 2 = the constitutive solve and the per-increment work
+// only.  The solve's events are attached to its launch in both modes; slot k uses ev[2k], ev[2k+1].
+static bool timed(const cg_ctx* c, int k) { return c->timing == 1 || (c->timing == 2 && k == 3); }
 static void tstart(cg_ctx* c, int k) {
-    if (c->timing) hipEventRecord(c->ev[2 * k], c->s);
+    if (timed(c, k)) hipEventRecord(c->ev[2 * k], c->s);
 }
 static void tstop(cg_ctx* c, int k) {
-    if (c->timing) hipEventRecord(c->ev[2 * k + 1], c->s);
+    if (timed(c, k)) hipEventRecord(c->ev[2 * k + 1], c->s);
 }
 static void tcollect(cg_ctx* c, int k) {
-    if (!c->timing) return;
+    if (!c->timing || (k != 1 && !timed(c, k))) return;
     float ms = 0.0f;
     hipEventSynchronize(c->ev[2 * k + 1]);
     hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]);
     c->st.t_ms[k] += ms;
 }
+// the early passes' time (slot 0 or 1) counts as FFT + Gamma stage time
+static void tcollect_early(cg_ctx* c, int slot) {
+    if (!c->early_timed[slot]) return;
+    c->early_timed[slot] = 0;
+    float ms = 0.0f;
+    hipEventSynchronize(c->ev[9 + 2 * slot]);
+    hipEventElapsedTime(&ms, c->ev[8 + 2 * slot], c->ev[9 + 2 * slot]);
+    c->st.t_ms[0] += ms;
+}
+// Drops the early passes of the next pass (the state they read is about to change, or the
+// host runs something else next).  Their writes (the spectrum) are overwritten by the next
+// pass' own transforms, in stream order; their time is not counted as a pass' stage time.
+static void drop_early(cg_ctx* c) {
+    if (!c) return;
+    c->early = 0;
+    c->early_timed[0] = c->early_timed[1] = 0;
+}
 
 int cg_begin_increment(cg_ctx* c, const cg_process* p, int do_schmid) {
+    drop_early(c);
     if (!c || !c->initialised || !p) return -1;
     tstart(c, 3);
     memcpy(c->dsim_snap, p->dsim, sizeof c->dsim_snap);
@@ -697,17 +748,9 @@ static void tune_write(const std::string& path, const std::string& key, const Ff
             t.yhalf[0], t.yhalf[1], t.kt1);
     fclose(fp);
 }
-// d_scale <- {1/scale, scale/V} when the scale changed (enqueued on the context's stream; the
-// staging buffer is free again once the stream has passed the copy, which every pass waits for)
+// the mixed form's stress scale for the LDS FFT kernels (the register kernels derive the same
+// max(svm, 1) from d_sgsol on the device)
 static double pass_scale(const cg_ctx* c) { return c->mixed ? std::max(c->svmavg, 1.0) : 1.0; }
-static int set_scale(cg_ctx* c, double scale) {
-    if (scale == c->cur_scale) return 0;
-    c->h_scale[0] = 1.0 / scale;
-    c->h_scale[1] = scale / (double)c->V;
-    HIPCHK(c, hipMemcpyAsync(c->d_scale, c->h_scale, 2 * sizeof(double), hipMemcpyHostToDevice, c->s));
-    c->cur_scale = scale;
-    return 0;
-}
 // Captures what body() enqueues on the context's stream into an executable graph.
 template <class Body>
 static int capture(cg_ctx* c, hipGraphExec_t* ex, Body body) {
@@ -744,8 +787,7 @@ static int ensure_fft_tuned(cg_ctx* c, const cg_process* p) {
         launch_sigma_sum(c->f, c->s, &nb);
         launch_reduce_final(c->f.part, nb, 9, c->d_sgsum, c->s);
     }
-    if (int e = set_scale(c, pass_scale(c))) return e;
-    if (fft_tune(c->f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, c->d_scale, c->s) != 0)
+    if (fft_tune(c->f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, c->d_sgsol, c->s) != 0)
         return fail(c, -2, "FFT shape tuning failed");
     c->tune_version++;
     if (!c->tune_path.empty()) tune_write(c->tune_path, key, fft_get_tune(c->fft));
@@ -758,6 +800,13 @@ static int ensure_fft_tuned(cg_ctx* c, const cg_process* p) {
 // and their copy to the host; pass_finish (after the stream has finished) is the host part.
 static int pass_begin(cg_ctx* c, cg_process* p, cg_iter_state* it, SolveArgs& sa) {
     Fields& f = c->f;
+    // the early passes enqueued by the previous cg_iterate hold for this pass if the Green
+    // operator, its kind and the FFT shapes are those they ran with, and this is not the first
+    // pass of an increment
+    const bool early = c->early && it->iter > 0 && c->early_igamma == p->igamma &&
+                       c->early_green == c->green_version && c->early_tune == c->tune_version;
+    if (c->early && !early) drop_early(c);
+    c->early = 0;
     if (it->iter == 0) {  // LS-EVPFFT.f90:222-231
         if (c->ngas) launch_gas_compl(f, p->tdot, c->complgas, c->modal, c->s);
         memset(c->dvelgradavg, 0, sizeof c->dvelgradavg);
@@ -766,22 +815,24 @@ static int pass_begin(cg_ctx* c, cg_process* p, cg_iter_state* it, SolveArgs& sa
     }
     it->iter++;
     int nb;
-    tstart(c, 2);
+    tstart(c, 0);
     if (!c->sgsum_valid) {
         launch_sigma_sum(f, c->s, &nb);
         launch_reduce_final(f.part, nb, 9, c->d_sgsum, c->s);
     }
-    tstop(c, 2);
-    // stress norm used to scale the FP32 spectral stage of the mixed form
+    // stress norm used to scale the FP32 spectral stage of the mixed form (written by the
+    // device after each pass once device_scale is set: the same max(svm, 1) of the pass' sums)
     const double scale = pass_scale(c);
-    if (int e = set_scale(c, scale)) return e;
-    tstart(c, 0);
     auto stage = [&]() {
-        const int n = fft_green(f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, scale, c->d_scale, c->s);
+        int n;
+        if (early)
+            n = fft_green_passes(f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, c->d_sgsol, 4, 5, c->s);
+        else
+            n = fft_green(f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, scale, c->d_sgsol, c->s);
         launch_reduce_final_cm(f.part, n, 9, c->d_fluct, c->s);
     };
-    // the register kernels take the scale from d_scale, so their stage can be replayed as a graph
-    if (c->use_graphs && fft_tunable(f, c->fft)) {
+    // the register kernels take the scale from d_sgsol, so their stage can be replayed as a graph
+    if (c->use_graphs && fft_tunable(f, c->fft) && !early) {
         if (!c->g_pre || c->g_pre_version != c->green_version || c->g_pre_igamma != p->igamma ||
             c->g_pre_tune != c->tune_version) {
             if (int e = capture(c, &c->g_pre, stage)) return e;
@@ -822,29 +873,32 @@ static int pass_begin(cg_ctx* c, cg_process* p, cg_iter_state* it, SolveArgs& sa
 
 static int pass_reduce(cg_ctx* c) {
     Fields& f = c->f;
-    auto sums = [&]() {
-        launch_reduce_final_cm(f.part, solve_blocks(c->V), R_NSOLVE, f.red, c->s);
-        // next pass' <sigma> = sum_all w sigma
-        launch_copy(c->d_sgsum, f.red + R_SGALL, 9, c->s);
-        hipMemcpyAsync(c->h_red, f.red, R_NCOPY * sizeof(double), hipMemcpyDeviceToHost, c->s);
+    // the solve's sums, written also to the host's copy, the next pass' <sigma> (= sum_all w sigma)
+    // and the solid sums its mixed-form scale derives from; with the overlap, ev_sums is recorded
+    // at the end of the last kernel
+    auto sums = [&](hipEvent_t done) {
+        launch_pass_sums(f.part, solve_blocks(c->V), R_NSOLVE, R_NCOPY, f.red, c->h_pass_dev, c->d_sgsum,
+                         c->d_sgsol, c->s, done);
     };
     if (!c->use_graphs) {
-        sums();
+        tstart(c, 2);
+        sums(c->overlap && fft_tunable(f, c->fft) ? c->ev_sums : nullptr);
+        tstop(c, 2);
         return 0;
     }
     if (!c->g_post)
-        if (int e = capture(c, &c->g_post, sums)) return e;
+        if (int e = capture(c, &c->g_post, [&]() { sums(nullptr); })) return e;
     HIPCHK(c, hipGraphLaunch(c->g_post, c->s));
     return 0;
 }
 
 static int pass_finish(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     const long V = c->V;
-    if (int e = singular_from(c, c->h_red, "update_schmid / calc_gas_compl / solve_res")) return e;
+    if (int e = singular_from(c, c->h_pass, "update_schmid / calc_gas_compl / solve_res")) return e;
     const bool kin_ctrl = (p->iscau[2] == 0);
     c->sgsum_valid = 1;
     c->lold_same = 1;
-    const double* r = c->h_red;
+    const double* r = c->h_pass;
     c->st.passes++;
     c->st.voxel_iters += V;
     c->st.nr_steps += (int64_t)r[R_NR];
@@ -940,18 +994,39 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
     if (!c || !c->initialised || !p || !it) return -1;
     if (int e = ensure_fft_tuned(c, p)) return e;
     SolveArgs sa;
+    const int used_slot = c->early ? c->early_slot : -1;
     if (int e = pass_begin(c, p, it, sa)) return e;
     PhaseTables pt{c->d_ph, c->d_tab};
     int nb;
-    tstart(c, 1);
+    // the solve's own start and end events (no marker packets between the kernels)
     launch_correct_solve_full(c->f, pt, sa.pc, sa.dvelgradavg, sa.finv_inv, sa.fluct, c->s, &nb, c->int_pow,
-                              c->sym_tau, c->small_ns);
-    tstop(c, 1);
+                              c->sym_tau, c->small_ns, c->timing ? c->ev[2] : nullptr, c->timing ? c->ev[3] : nullptr);
     if (int e = pass_reduce(c)) return e;
-    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (c->overlap && !c->use_graphs && fft_tunable(c->f, c->fft)) {
+        // the next pass' transforms run while the host does its part of this one (ev_sums is
+        // recorded at the end of the sums' kernel)
+        const int slot = used_slot == 0 ? 1 : 0;
+        if (c->timing == 1) {
+            hipEventRecord(c->ev[8 + 2 * slot], c->s);
+            c->early_timed[slot] = 1;
+        }
+        fft_green_passes(c->f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, c->d_sgsol, 0, 4, c->s);
+        if (c->timing == 1) hipEventRecord(c->ev[9 + 2 * slot], c->s);
+        c->early = 1;
+        c->early_slot = slot;
+        c->early_igamma = p->igamma;
+        c->early_green = c->green_version;
+        c->early_tune = c->tune_version;
+        HIPCHK(c, hipEventSynchronize(c->ev_sums));
+    } else {
+        HIPCHK(c, hipStreamSynchronize(c->s));
+    }
     HIPCHK(c, hipGetLastError());
     tcollect(c, 0); tcollect(c, 1); tcollect(c, 2);
-    return pass_finish(c, p, it, out);
+    if (used_slot >= 0) tcollect_early(c, used_slot);
+    const int rc = pass_finish(c, p, it, out);
+    if (rc) drop_early(c);
+    return rc;
 }
 
 }  // extern "C"
@@ -1133,6 +1208,7 @@ int cg_batch_iterate(cg_batch* b, cg_process* const* p, cg_iter_state* st, cg_it
 }
 
 int cg_rollback(cg_ctx* c, cg_process* p) {
+    drop_early(c);
     if (!c || !c->initialised || !p) return -1;
     memcpy(p->dsim, c->dsim_snap, sizeof c->dsim_snap);
     memcpy(p->scauchy, c->scauchy_snap, sizeof c->scauchy_snap);
@@ -1160,6 +1236,7 @@ int cg_kinematic_substep(cg_ctx* c, double tdot, double* out4, int* nsub) {
 }
 
 int cg_end_increment(cg_ctx* c, const cg_process* p, int scan_diagnostics, cg_incr_out* out) {
+    drop_early(c);
     if (!c || !c->initialised || !p) return -1;
     Fields& f = c->f;
     const long V = c->V;
@@ -1299,6 +1376,7 @@ int cg_end_increment(cg_ctx* c, const cg_process* p, int scan_diagnostics, cg_in
 }
 
 int cg_calc_c0(cg_ctx* c, const cg_process* p) {
+    drop_early(c);
     if (!c || !c->initialised || !p) return -1;
     PhaseTables pt{c->d_ph, c->d_tab};
     int nb;
@@ -1411,6 +1489,7 @@ int cg_get_field(cg_ctx* c, const char* name, double* out, size_t n) {
 }
 
 int cg_set_field(cg_ctx* c, const char* name, const double* in, size_t n) {
+    drop_early(c);
     if (!c || !c->initialised || !name || !in) return -1;
     c->sop_valid = 0;
     c->taun_valid = 0;
@@ -1455,6 +1534,7 @@ int cg_set_field(cg_ctx* c, const char* name, const double* in, size_t n) {
 }
 
 int cg_set_thermo(cg_ctx* c, const double* eth) {
+    drop_early(c);
     if (!c || !c->initialised || !eth) return -1;
     Fields& f = c->f;
     const long V = c->V;
@@ -1489,12 +1569,14 @@ int cg_get_stats(cg_ctx* c, cg_stats* st, int reset) {
 }
 
 int cg_set_timing(cg_ctx* c, int on) {
+    drop_early(c);
     if (!c) return -1;
-    c->timing = on ? 1 : 0;
+    c->timing = on == 2 ? 2 : on ? 1 : 0;
     return 0;
 }
 
 int cg_set_tuning_file(cg_ctx* c, const char* path) {
+    drop_early(c);
     if (!c) return -1;
     c->tune_path = path ? path : "";
     return 0;
@@ -1515,6 +1597,7 @@ int cg_get_fft_tuning(cg_ctx* c, int* shape8, double* us5) {
 }
 
 int cg_set_fft_tuning(cg_ctx* c, const int* shape8) {
+    drop_early(c);
     if (!c || !c->fft || !shape8) return -1;
     FftTune t;
     for (int k = 0; k < 8; k++)
@@ -1530,6 +1613,7 @@ int cg_set_fft_tuning(cg_ctx* c, const int* shape8) {
 }
 
 int cg_set_displacement(cg_ctx* c, int idispini, const double* disp) {
+    drop_early(c);
     if (!c || !disp) return -1;
     if (c->initialised) return fail(c, -1, "cg_set_displacement: call before cg_init");
     if (idispini != 1 && idispini != 2) return fail(c, -1, "cg_set_displacement: idispini must be 1 or 2");
@@ -1540,8 +1624,16 @@ int cg_set_displacement(cg_ctx* c, int idispini, const double* disp) {
 }
 
 int cg_set_graphs(cg_ctx* c, int on) {
+    drop_early(c);
     if (!c) return -1;
     c->use_graphs = on ? 1 : 0;
+    return 0;
+}
+
+int cg_set_overlap(cg_ctx* c, int on) {
+    drop_early(c);
+    if (!c) return -1;
+    c->overlap = on ? 1 : 0;
     return 0;
 }
 

@@ -548,17 +548,18 @@ static int run_green(const Fields& f, FftCtx* c, const double* sgsum, const Gree
 }
 
 bool fft_reg_supported(const Fields& f);
+// passes st0 .. st1-1 of the register form; returns the last pass' block count
 template <typename R>
 int run_green_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
-                  const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, const double* dscale,
-                  const FftTune& t, hipStream_t s);
+                  const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, const double* sgsol,
+                  const FftTune& t, hipStream_t s, int st0, int st1);
 template <typename R>
 int tune_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
-             const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, const double* dscale, FftTune& t,
+             const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, const double* sgsol, FftTune& t,
              hipStream_t s);
 
 int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
-              int mixed, double scale, const double* dscale, hipStream_t s) {
+              int mixed, double scale, const double* sgsol, hipStream_t s) {
     GreenDev g;
     for (int i = 0; i < 81; i++) { g.c0[i] = c0[i]; g.s0[i] = s0[i]; }
     g.igamma = igamma;
@@ -566,16 +567,16 @@ int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double*
         const Plan3& pl = c->plan;
         if (mixed)
             return run_green_reg<float>(f, c->tw32 + pl.ax[0].tw_off, c->tw32 + pl.ax[1].tw_off,
-                                        c->tw32 + pl.ax[2].tw_off, sgsum_dev, g, dscale, c->tune, s);
+                                        c->tw32 + pl.ax[2].tw_off, sgsum_dev, g, sgsol, c->tune, s, 0, 5);
         return run_green_reg<double>(f, c->tw64 + pl.ax[0].tw_off, c->tw64 + pl.ax[1].tw_off,
-                                     c->tw64 + pl.ax[2].tw_off, sgsum_dev, g, dscale, c->tune, s);
+                                     c->tw64 + pl.ax[2].tw_off, sgsum_dev, g, sgsol, c->tune, s, 0, 5);
     }
     if (mixed) return run_green<float>(f, c, sgsum_dev, g, scale, s);
     return run_green<double>(f, c, sgsum_dev, g, 1.0, s);
 }
 
 int fft_tune(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
-             int mixed, const double* dscale, hipStream_t s) {
+             int mixed, const double* sgsol, hipStream_t s) {
     if (!(c->use_reg && fft_reg_supported(f))) return 0;  // the LDS kernels have no shapes to pick
     GreenDev g;
     for (int i = 0; i < 81; i++) { g.c0[i] = c0[i]; g.s0[i] = s0[i]; }
@@ -583,9 +584,22 @@ int fft_tune(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* 
     const Plan3& pl = c->plan;
     if (mixed)
         return tune_reg<float>(f, c->tw32 + pl.ax[0].tw_off, c->tw32 + pl.ax[1].tw_off, c->tw32 + pl.ax[2].tw_off,
-                               sgsum_dev, g, dscale, c->tune, s);
+                               sgsum_dev, g, sgsol, c->tune, s);
     return tune_reg<double>(f, c->tw64 + pl.ax[0].tw_off, c->tw64 + pl.ax[1].tw_off, c->tw64 + pl.ax[2].tw_off,
-                            sgsum_dev, g, dscale, c->tune, s);
+                            sgsum_dev, g, sgsol, c->tune, s);
+}
+int fft_green_passes(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0,
+                     int igamma, int mixed, const double* sgsol, int st0, int st1, hipStream_t s) {
+    if (!(c->use_reg && fft_reg_supported(f))) return -1;
+    GreenDev g;
+    for (int i = 0; i < 81; i++) { g.c0[i] = c0[i]; g.s0[i] = s0[i]; }
+    g.igamma = igamma;
+    const Plan3& pl = c->plan;
+    if (mixed)
+        return run_green_reg<float>(f, c->tw32 + pl.ax[0].tw_off, c->tw32 + pl.ax[1].tw_off,
+                                    c->tw32 + pl.ax[2].tw_off, sgsum_dev, g, sgsol, c->tune, s, st0, st1);
+    return run_green_reg<double>(f, c->tw64 + pl.ax[0].tw_off, c->tw64 + pl.ax[1].tw_off,
+                                 c->tw64 + pl.ax[2].tw_off, sgsum_dev, g, sgsol, c->tune, s, st0, st1);
 }
 FftTune fft_get_tune(const FftCtx* c) { return c->tune; }
 void fft_set_tune(FftCtx* c, const FftTune& t) { c->tune = t; }
@@ -827,9 +841,10 @@ template <typename R> __device__ __forceinline__ typename Cx<R>::T* spec_of(cons
 // separated with X[N-k] afterwards.
 template <int N, typename R>
 __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
-                                                   const double* __restrict__ sgsum, const double* __restrict__ dscale,
+                                                   const double* __restrict__ sgsum, const double* __restrict__ sgsol,
                                                    int xcd) {
-    const double inv_scale = dscale[0];  // 1 / the mixed form's stress scale (1 in FP64)
+    // 1 / the mixed form's stress scale (1 in FP64)
+    const double inv_scale = sizeof(R) == 8 ? 1.0 : 1.0 / mixed_scale(sgsol);
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
     using X = XDims<N, R>;
@@ -998,8 +1013,8 @@ __global__ __launch_bounds__((ZDims<N, KT1>::ZB)) void k_z_green_reg(Fields f, c
 // c2c inverse does); block partial sums of w dLref F^-1.
 template <int N, typename R>
 __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
-                                                   const double* __restrict__ dscale, int xcd) {
-    const double out_scale = dscale[1];  // stress scale / V
+                                                   const double* __restrict__ sgsol, int xcd) {
+    const double out_scale = (sizeof(R) == 8 ? 1.0 : mixed_scale(sgsol)) / (double)f.V;  // stress scale / V
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
     using X = XDims<N, R>;
@@ -1105,18 +1120,18 @@ static bool reg_size(int n) { return n == 16 || n == 32 || n == 48 || n == 64 ||
 bool fft_reg_supported(const Fields& f) { return reg_size(f.n1) && reg_size(f.n2) && reg_size(f.n3); }
 
 template <int N, typename R> static void launch_x_fwd_reg(const Fields& f, const typename Cx<R>::T* tw, const double* sgsum,
-                                                          const double* dscale, int xcd, hipStream_t s) {
+                                                          const double* sgsol, int xcd, hipStream_t s) {
     const long nrows = (long)f.n2 * f.n3;
     using X = XDims<N, R>;
     const int nb = (int)((nrows + X::RW - 1) / X::RW);
-    hipLaunchKernelGGL((k_x_fwd_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, f, tw, sgsum, dscale, xcd);
+    hipLaunchKernelGGL((k_x_fwd_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, f, tw, sgsum, sgsol, xcd);
 }
-template <int N, typename R> static int launch_x_inv_reg(const Fields& f, const typename Cx<R>::T* tw, const double* dscale,
+template <int N, typename R> static int launch_x_inv_reg(const Fields& f, const typename Cx<R>::T* tw, const double* sgsol,
                                                          int xcd, hipStream_t s) {
     const long nrows = (long)f.n2 * f.n3;
     using X = XDims<N, R>;
     const int nb = (int)((nrows + X::RW - 1) / X::RW);
-    hipLaunchKernelGGL((k_x_inv_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, f, tw, dscale, xcd);
+    hipLaunchKernelGGL((k_x_inv_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, f, tw, sgsol, xcd);
     return nb;
 }
 template <int N, bool INV, typename R, int YB>
@@ -1167,14 +1182,14 @@ static void launch_z_reg(const Fields& f, const typename Cx<R>::T* tw, const Gre
 // form with the block shape and placement of t; returns the x inverse's block count.
 template <typename R>
 static int reg_stage(int stage, const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
-                     const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, const double* dscale,
+                     const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, const double* sgsol,
                      const FftTune& t, hipStream_t s) {
     int nb = -1;
-#define CG_XF(N) launch_x_fwd_reg<N, R>(f, tw1, sgsum, dscale, t.xcd[0], s)
+#define CG_XF(N) launch_x_fwd_reg<N, R>(f, tw1, sgsum, sgsol, t.xcd[0], s)
 #define CG_YF(N) launch_y_reg<N, false, R>(f, tw2, t.yhalf[0], t.xcd[1], s)
 #define CG_Z(N) launch_z_reg<N, R>(f, tw3, g, t.kt1, t.xcd[2], s)
 #define CG_YI(N) launch_y_reg<N, true, R>(f, tw2, t.yhalf[1], t.xcd[3], s)
-#define CG_XI(N) nb = launch_x_inv_reg<N, R>(f, tw1, dscale, t.xcd[4], s)
+#define CG_XI(N) nb = launch_x_inv_reg<N, R>(f, tw1, sgsol, t.xcd[4], s)
     switch (stage) {
         case 0: CG_SWITCH_N(f.n1, CG_XF) break;
         case 1: CG_SWITCH_N(f.n2, CG_YF) break;
@@ -1192,10 +1207,10 @@ static int reg_stage(int stage, const Fields& f, const typename Cx<R>::T* tw1, c
 
 template <typename R>
 int run_green_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
-                  const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, const double* dscale,
-                  const FftTune& t, hipStream_t s) {
+                  const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, const double* sgsol,
+                  const FftTune& t, hipStream_t s, int st0, int st1) {
     int nb = -1;
-    for (int st = 0; st < 5; st++) nb = reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, dscale, t, s);
+    for (int st = st0; st < st1; st++) nb = reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, sgsol, t, s);
     return nb;
 }
 
@@ -1205,7 +1220,7 @@ int run_green_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename 
 // candidate computes the same values in the same order, so the choice changes time only.
 template <typename R>
 int tune_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
-                    const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, const double* dscale, FftTune& t,
+                    const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, const double* sgsol, FftTune& t,
                     hipStream_t s) {
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return -1;
@@ -1225,9 +1240,9 @@ int tune_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>
                 if (st == 1) c.yhalf[0] = sh;
                 if (st == 3) c.yhalf[1] = sh;
                 if (st == 2) c.kt1 = sh;
-                for (int w = 0; w < 2; w++) reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, dscale, c, s);
+                for (int w = 0; w < 2; w++) reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, sgsol, c, s);
                 hipEventRecord(e0, s);
-                for (int w = 0; w < 5; w++) reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, dscale, c, s);
+                for (int w = 0; w < 5; w++) reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, sgsol, c, s);
                 hipEventRecord(e1, s);
                 float ms = 0.0f;
                 if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
@@ -1247,9 +1262,9 @@ int tune_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>
     return rc;
 }
 template int run_green_reg<double>(const Fields&, const double2*, const double2*, const double2*, const double*,
-                                   const GreenDev&, const double*, const FftTune&, hipStream_t);
+                                   const GreenDev&, const double*, const FftTune&, hipStream_t, int, int);
 template int run_green_reg<float>(const Fields&, const float2*, const float2*, const float2*, const double*,
-                                  const GreenDev&, const double*, const FftTune&, hipStream_t);
+                                  const GreenDev&, const double*, const FftTune&, hipStream_t, int, int);
 template int tune_reg<double>(const Fields&, const double2*, const double2*, const double2*, const double*,
                               const GreenDev&, const double*, FftTune&, hipStream_t);
 template int tune_reg<float>(const Fields&, const float2*, const float2*, const float2*, const double*,

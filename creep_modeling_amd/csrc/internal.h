@@ -106,7 +106,13 @@ void launch_reduce_final_cm(double* part, int nblocks, int nvals, double* out, h
 void launch_sigma_sum(const Fields& f, hipStream_t s, int* nblocks_out);
 void launch_correct_solve_full(const Fields& f, const PhaseTables& pt, const PassConst& pc,
                                const double* dvelgradavg, const double* finv_inv, const double* fluct_dev,
-                               hipStream_t s, int* nb, int int_pow, int sym_tau, int small_ns);
+                               hipStream_t s, int* nb, int int_pow, int sym_tau, int small_ns,
+                               hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// a pass' R_NSOLVE sums: red, the host's copy of red[0, ncopy) (host-mapped memory), <sigma> for
+// the next pass and the solid stress sums of the mixed-form scale (k_pass_final); done (or
+// null): recorded at the end of the last kernel
+void launch_pass_sums(double* part, int nblocks, int nvals, int ncopy, double* red, double* host, double* sgsum,
+                      double* sgsol, hipStream_t s, hipEvent_t done);
 // the correction + constitutive kernel over several RVEs: block b belongs to RVE blk_rve[b]
 void launch_correct_solve_batch(const SolveWork* work, const int* blk_rve, int nblocks, int modal, int int_pow,
                                 int sym_tau, int small_ns, hipStream_t s);
@@ -166,15 +172,19 @@ void fft_destroy(FftCtx* c);
 // PK1 -> Gamma convolution -> dLref; writes the 9 block partial sums of
 // w dLref F^-1 to f.part, column-major (part[i * nblocks + b]), and returns the number of blocks.
 // scale: the mixed form's stress scale (1 in FP64), used by the LDS kernels; the register
-// kernels read {1/scale, scale/V} from the device array dscale, so a captured stage stays valid
-// when the scale changes.
+// kernels take it from the previous pass' solid stress sums in the device array sgsol (9) as
+// max(von Mises, 1), so a captured stage stays valid when the scale changes.
 int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
-              int mixed, double scale, const double* dscale, hipStream_t s);
+              int mixed, double scale, const double* sgsol, hipStream_t s);
+// Passes st0 .. st1-1 of the register form (0 x forward, 1 y forward, 2 z + Gamma, 3 y inverse,
+// 4 x inverse); returns the last pass' block count.  Only for grids with fft_tunable().
+int fft_green_passes(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0,
+                     int igamma, int mixed, const double* sgsol, int st0, int st1, hipStream_t s);
 // Times the candidate shapes of each register pass on stream s (the passes overwrite the
 // spectrum, ΔL^ref and the block partials, which the next fft_green recomputes) and keeps the
 // fastest; 0 on success or when the lengths use the LDS kernels.
 int fft_tune(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
-             int mixed, const double* dscale, hipStream_t s);
+             int mixed, const double* sgsol, hipStream_t s);
 FftTune fft_get_tune(const FftCtx* c);
 void fft_set_tune(FftCtx* c, const FftTune& t);
 bool fft_tunable(const Fields& f, const FftCtx* c);

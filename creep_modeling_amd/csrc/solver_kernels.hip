@@ -6,6 +6,8 @@
 // :1062-1139, update_* :1748-2492, calc_c0 :2537-2649, kinematic_increments
 // :2766-2837), LS-EVPFFT.f90 (mixing :277-327, accumulation :431-446,
 // averages :460-523) and types.f90 (update_diagnostics :345-758).
+#include <hip/hip_ext.h>
+
 #include "internal.h"
 #include "reduce.h"
 #include "tensor.h"
@@ -142,6 +144,52 @@ void launch_reduce_final_cm(double* part, int nblocks, int nvals, double* out, h
     double* tmp = part + (size_t)nvals * nblocks;
     hipLaunchKernelGGL(k_reduce_cols, dim3(nch, nvals), dim3(BS), 0, s, part, nblocks, tmp);
     hipLaunchKernelGGL(k_reduce_final_cm, dim3(nvals), dim3(BS), 0, s, tmp, nch, out);
+}
+// The second level of a pass' sums (after k_reduce_cols, or directly over the partials when one
+// chunk covers them): block i sums column i as k_reduce_final_cm does and writes red[i], the
+// host's copy (host-mapped memory, so no copy follows), <sigma> for the next pass
+// (red[R_SGALL..]) and the solid stress sums the next pass' mixed-form scale is taken from
+// (red[R_SGSOL..]); block 0 also copies red[nvals, ncopy) (the flags the solve wrote) to the host.
+struct PassFinal {
+    const double* part;  // nvals columns of n values
+    int n, nvals, ncopy;
+    double* red;
+    double* host;   // host-mapped, fine-grained
+    double* sgsum;  // 9
+    double* sgsol;  // 9
+};
+__global__ __launch_bounds__(BS) void k_pass_final(PassFinal a) {
+    const int i = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < a.n; b += BS) s += a.part[(size_t)i * a.n + b];
+    __shared__ double sm[BS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (lane == 0) sm[wid] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = sm[0];
+        for (int w = 1; w < BS / 64; w++) t += sm[w];
+        a.red[i] = t;
+        a.host[i] = t;
+        if (i >= R_SGALL && i < R_SGALL + 9) a.sgsum[i - R_SGALL] = t;
+        if (i >= R_SGSOL && i < R_SGSOL + 9) a.sgsol[i - R_SGSOL] = t;
+    }
+    if (i == 0)
+        for (int k = a.nvals + threadIdx.x; k < a.ncopy; k += BS) a.host[k] = a.red[k];
+    __threadfence_system();
+}
+void launch_pass_sums(double* part, int nblocks, int nvals, int ncopy, double* red, double* host, double* sgsum,
+                      double* sgsol, hipStream_t s, hipEvent_t done) {
+    const int nch = (nblocks + RCH - 1) / RCH;
+    PassFinal a{part, nblocks, nvals, ncopy, red, host, sgsum, sgsol};
+    if (nch > 1) {
+        double* tmp = part + (size_t)nvals * nblocks;
+        hipLaunchKernelGGL(k_reduce_cols, dim3(nch, nvals), dim3(BS), 0, s, part, nblocks, tmp);
+        a.part = tmp;
+        a.n = nch;
+    }
+    hipExtLaunchKernelGGL(k_pass_final, dim3(nvals), dim3(BS), 0, s, nullptr, done, 0, a);
 }
 void launch_reduce_final(const double* part, int nblocks, int nvals, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_final, dim3(nvals), dim3(BS), 0, s, part, nblocks, nvals, out);
@@ -1131,20 +1179,23 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(MODAL ? 1 :
 // Public launcher with the full argument set.
 void launch_correct_solve_full(const Fields& f, const PhaseTables& pt, const PassConst& pc,
                                const double* dvelgradavg, const double* finv_inv, const double* fluct_dev,
-                               hipStream_t s, int* nb, int int_pow, int sym_tau, int small_ns) {
+                               hipStream_t s, int* nb, int int_pow, int sym_tau, int small_ns, hipEvent_t e0,
+                               hipEvent_t e1) {
     SolveArgs a;
     a.pc = pc;
     for (int i = 0; i < 9; i++) { a.dvelgradavg[i] = dvelgradavg[i]; a.finv_inv[i] = finv_inv[i]; }
     a.fluct = fluct_dev;
     *nb = (int)((f.V + SBS - 1) / SBS);
     if (pc.modal) {
-        if (int_pow) hipLaunchKernelGGL((k_correct_solve<true, true, false, false>), dim3(*nb), dim3(SBS), 0, s, f, pt, a);
-        else hipLaunchKernelGGL((k_correct_solve<true, false, false, false>), dim3(*nb), dim3(SBS), 0, s, f, pt, a);
+        if (int_pow) hipExtLaunchKernelGGL((k_correct_solve<true, true, false, false>), dim3(*nb), dim3(SBS), 0, s, e0, e1, 0, f, pt, a);
+        else hipExtLaunchKernelGGL((k_correct_solve<true, false, false, false>), dim3(*nb), dim3(SBS), 0, s, e0, e1, 0, f, pt, a);
         return;
     }
     // the symmetric-solve kernel is instantiated per (integer exponents, equal forward and
-    // backward strengths, at most 12 systems per phase)
-#define CG_SOLVE(IP, ST, SM) hipLaunchKernelGGL((k_correct_solve<false, IP, ST, SM>), dim3(*nb), dim3(SBS), 0, s, f, pt, a)
+    // backward strengths, at most 12 systems per phase); e0, e1 (or null): events recorded at the
+    // launch's start and end (no separate marker packets in the stream)
+#define CG_SOLVE(IP, ST, SM) \
+    hipExtLaunchKernelGGL((k_correct_solve<false, IP, ST, SM>), dim3(*nb), dim3(SBS), 0, s, e0, e1, 0, f, pt, a)
     switch ((int_pow ? 4 : 0) + (sym_tau ? 2 : 0) + (small_ns ? 1 : 0)) {
         case 0: CG_SOLVE(false, false, false); break;
         case 1: CG_SOLVE(false, false, true); break;

@@ -56,7 +56,7 @@ class Solver:
     """One RVE: fixed-point increments on the GPU, one accepted increment per :meth:`step`."""
 
     def __init__(self, problem, precision=0, modal=False, do_mix=True, scan_cond=True, max_pass_factor=100,
-                 tune_file=None, fft_shapes=None, graphs=False):
+                 tune_file=None, fft_shapes=None, graphs=False, overlap=True):
         self.problem = problem
         self.ctx = _lib.Context(problem, precision=precision, modal=modal, do_mix=do_mix)
         if tune_file:  # FFT shape cache (--fftw_save)
@@ -65,6 +65,8 @@ class Solver:
             self.ctx.set_fft_tuning(fft_shapes)
         if graphs:  # replay each pass' fixed device work as HIP graphs
             self.ctx.set_graphs(True)
+        if not overlap:  # no early transforms of the next pass during the host's part of a pass
+            self.ctx.set_overlap(False)
         self.L = self.ctx.L
         self.h = self.ctx.h
         self.procs = [_lib.process_struct(p) for p in problem.processes]

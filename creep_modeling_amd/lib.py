@@ -22,6 +22,7 @@ EXPORTS = (
     "cg_begin_increment", "cg_iterate", "cg_rollback", "cg_kinematic_substep", "cg_end_increment",
     "cg_calc_c0", "cg_field_ncomp", "cg_get_field", "cg_set_field", "cg_get_macro", "cg_get_stats",
     "cg_set_timing", "cg_set_tuning_file", "cg_get_fft_tuning", "cg_set_fft_tuning", "cg_set_graphs",
+    "cg_set_overlap",
     "cg_kinematic_substep_noalloc",
     "cg_set_thermo", "cg_batch_create", "cg_batch_destroy", "cg_batch_iterate", "cg_batch_launch", "cg_batch_wait",
 )
@@ -127,6 +128,7 @@ def load():
     L.cg_get_fft_tuning.argtypes = [vp, _I, _D]
     L.cg_set_fft_tuning.argtypes = [vp, _I]
     L.cg_set_graphs.argtypes = [vp, C.c_int]
+    L.cg_set_overlap.argtypes = [vp, C.c_int]
     L.cg_set_displacement.argtypes = [vp, C.c_int, _D]
     L.cg_kinematic_substep_noalloc.argtypes = [C.c_long, vp, C.c_long, vp, C.c_double, vp, vp]
     L.cg_set_thermo.argtypes = [vp, _D]
@@ -249,6 +251,10 @@ class Context:
     def set_graphs(self, on):
         """Replay each pass' fixed device work as HIP graphs, or launch it eagerly (default)."""
         self._chk(self.L.cg_set_graphs(self.h, int(bool(on))), "cg_set_graphs")
+
+    def set_overlap(self, on):
+        """Overlap each pass' host part with the next pass' transforms (default), or not."""
+        self._chk(self.L.cg_set_overlap(self.h, int(bool(on))), "cg_set_overlap")
 
     def set_fft_tuning(self, shapes):
         """Fix the FFT shapes (8 entries of 0/1) and skip the tuning."""

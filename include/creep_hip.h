@@ -196,7 +196,11 @@ typedef struct {
     int64_t passes, voxel_iters, nr_steps, evals, outer;
 } cg_stats;
 int cg_get_stats(cg_ctx *ctx, cg_stats *st, int reset);
-/* Enable (1) / disable (0) the per-stage HIP-event timing (default off). */
+/* HIP-event timing (default 0, off): 1 = every stage (the FFT + Gamma stage, the constitutive
+ * solve, the sums, the per-increment work); 2 = the constitutive solve and the per-increment work
+ * only.  The solve's events are attached to its launch; the other stages' events are stream
+ * markers, which leave a few microseconds between kernels, so mode 2 keeps them out of the
+ * passes. */
 int cg_set_timing(cg_ctx *ctx, int on);
 
 /* FFT shape tuning.  Before a context's first pass the block shapes and block placement of its
@@ -221,6 +225,14 @@ int cg_set_fft_tuning(cg_ctx *ctx, const int *shape8);
  * the solve; captured at their first use and again when C0, igamma or the FFT shapes change.
  * The results are the same either way.  No reference counterpart. */
 int cg_set_graphs(cg_ctx *ctx, int on);
+
+/* Overlap each pass' host part with the next pass' transforms (1, the default) or not (0).
+ * With overlap on, cg_iterate enqueues the x, y, z + Gamma and y passes of the next pass after
+ * the pass' sums and waits only for the sums; the next cg_iterate then runs the x inverse alone.
+ * Any entry point that writes state, a new increment, or another igamma makes the next pass run
+ * all five.  The results are bitwise the same either way.  Not with graphs (cg_set_graphs).  No
+ * reference counterpart. */
+int cg_set_overlap(cg_ctx *ctx, int on);
 
 #ifdef __cplusplus
 }
