@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -139,6 +140,7 @@ struct cg_ctx {
     // device address
     double* h_pass = nullptr;
     double* h_pass_dev = nullptr;
+
 
     double* d_sgsum = nullptr;
     double* d_fluct = nullptr;
@@ -798,12 +800,14 @@ static int ensure_fft_tuned(cg_ctx* c, const cg_process* p) {
 // launch (cg_batch_iterate): pass_begin enqueues the work before it on the context's stream
 // and fills the solve's arguments; pass_reduce enqueues the reductions of the solve's sums
 // and their copy to the host; pass_finish (after the stream has finished) is the host part.
-static int pass_begin(cg_ctx* c, cg_process* p, cg_iter_state* it, SolveArgs& sa) {
+// run_stage = false: the FFT + Gamma stage and its sums are left to the caller (the batched
+// passes of cg_batch_launch run them for several RVEs at once).
+static int pass_begin(cg_ctx* c, cg_process* p, cg_iter_state* it, SolveArgs& sa, bool run_stage = true) {
     Fields& f = c->f;
     // the early passes enqueued by the previous cg_iterate hold for this pass if the Green
     // operator, its kind and the FFT shapes are those they ran with, and this is not the first
     // pass of an increment
-    const bool early = c->early && it->iter > 0 && c->early_igamma == p->igamma &&
+    const bool early = run_stage && c->early && it->iter > 0 && c->early_igamma == p->igamma &&
                        c->early_green == c->green_version && c->early_tune == c->tune_version;
     if (c->early && !early) drop_early(c);
     c->early = 0;
@@ -832,7 +836,8 @@ static int pass_begin(cg_ctx* c, cg_process* p, cg_iter_state* it, SolveArgs& sa
         launch_reduce_final_cm(f.part, n, 9, c->d_fluct, c->s);
     };
     // the register kernels take the scale from d_sgsol, so their stage can be replayed as a graph
-    if (c->use_graphs && fft_tunable(f, c->fft) && !early) {
+    if (!run_stage) {
+    } else if (c->use_graphs && fft_tunable(f, c->fft) && !early) {
         if (!c->g_pre || c->g_pre_version != c->green_version || c->g_pre_igamma != p->igamma ||
             c->g_pre_tune != c->tune_version) {
             if (int e = capture(c, &c->g_pre, stage)) return e;
@@ -1046,6 +1051,10 @@ struct cg_batch {
     std::vector<int> table_active;    // active set d_blk_rve was built for
     std::vector<int> key;             // kernel instance of every RVE: modal, integer exponents, equal strengths, <= 12 systems
     std::vector<int> grp_key, grp_off, grp_nblk;  // launches of the current table
+    // batched FFT + Gamma stages and sums: the entries of RVEs with the same grid, precision and
+    // Green-operator kind one after another, one group per launch set
+    FftWork* h_fw = nullptr;          // pinned
+    FftWork* d_fw = nullptr;
     // the pass in flight (cg_batch_launch -> cg_batch_wait)
     std::vector<cg_process*> pend_p;
     cg_iter_state* pend_st = nullptr;
@@ -1109,7 +1118,9 @@ int cg_batch_create(cg_ctx* const* ctxs, int n, cg_batch** out) {
     bool ok = hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreateWithFlags(&b->ev_solve, hipEventDisableTiming) == hipSuccess &&
               hipHostMalloc(&b->h_work, n * sizeof(SolveWork)) == hipSuccess &&
-              hipMalloc(&b->d_work, n * sizeof(SolveWork)) == hipSuccess;
+              hipMalloc(&b->d_work, n * sizeof(SolveWork)) == hipSuccess &&
+              hipHostMalloc(&b->h_fw, n * sizeof(FftWork)) == hipSuccess &&
+              hipMalloc(&b->d_fw, n * sizeof(FftWork)) == hipSuccess;
     for (int i = 0; ok && i < n; i++) ok = hipEventCreateWithFlags(&b->ev_pre[i], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         cg_batch_destroy(b);
@@ -1128,6 +1139,8 @@ int cg_batch_destroy(cg_batch* b) {
     if (b->ev_solve) hipEventDestroy(b->ev_solve);
     if (b->h_work) hipHostFree(b->h_work);
     if (b->d_work) hipFree(b->d_work);
+    if (b->h_fw) hipHostFree(b->h_fw);
+    if (b->d_fw) hipFree(b->d_fw);
     if (b->d_blk_rve) hipFree(b->d_blk_rve);
     if (b->s) hipStreamDestroy(b->s);
     delete b;
@@ -1153,17 +1166,58 @@ int cg_batch_launch(cg_batch* b, cg_process* const* p, cg_iter_state* st, cg_ite
     b->pend_active.assign(active, active + n);
     if (!nact) return 0;
     if (int e = batch_table(b, active)) return e;
+    // The RVEs whose lengths have register FFT kernels (and no graphs) run their FFT + Gamma
+    // stages and their sums batched: one launch per pass for all RVEs of a grid, precision and
+    // Green-operator kind.  Others run theirs on their own streams.
+    std::vector<int> bat(n, 0);
+    std::vector<std::pair<std::array<int, 5>, int>> order;
+    for (int i = 0; i < n; i++) {
+        if (!active[i]) continue;
+        cg_ctx* c = b->ctx[i];
+        bat[i] = fft_tunable(c->f, c->fft) && !c->use_graphs;
+        if (bat[i]) order.push_back({{c->f.n1, c->f.n2, c->f.n3, c->mixed, p[i]->igamma ? 1 : 0}, i});
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<std::array<int, 5>, int>& a, const std::pair<std::array<int, 5>, int>& q) {
+                         return a.first < q.first;
+                     });
     // work before the solve, on every RVE's own stream
     for (int i = 0; i < n; i++) {
         if (!active[i]) continue;
         cg_ctx* c = b->ctx[i];
         SolveArgs sa;
-        if (int e = pass_begin(c, p[i], &st[i], sa)) return noctx_fail(e, "cg_batch_launch: RVE %d: %s", i, c->err);
+        if (int e = pass_begin(c, p[i], &st[i], sa, !bat[i]))
+            return noctx_fail(e, "cg_batch_launch: RVE %d: %s", i, c->err);
         b->h_work[i].f = c->f;
         b->h_work[i].pt = PhaseTables{c->d_ph, c->d_tab};
         b->h_work[i].a = sa;
         hipEventRecord(b->ev_pre[i], c->s);
         hipStreamWaitEvent(b->s, b->ev_pre[i], 0);
+    }
+    std::vector<size_t> gstart;  // first entry of each group in order / h_fw
+    for (size_t k = 0; k < order.size(); k++) {
+        if (k == 0 || order[k].first != order[k - 1].first) gstart.push_back(k);
+        cg_ctx* c = b->ctx[order[k].second];
+        FftWork& w = b->h_fw[k];
+        w.f = c->f;
+        fft_twiddles(c->fft, c->mixed, w.tw);
+        w.sgsum = c->d_sgsum;
+        w.sgsol = c->d_sgsol;
+        w.fluct = c->d_fluct;
+        w.host = c->h_pass_dev;
+        memcpy(w.g.c0, c->c0, sizeof w.g.c0);
+        memcpy(w.g.s0, c->s0, sizeof w.g.s0);
+        w.g.igamma = p[order[k].second]->igamma;
+    }
+    gstart.push_back(order.size());
+    if (!order.empty() &&
+        hipMemcpyAsync(b->d_fw, b->h_fw, order.size() * sizeof(FftWork), hipMemcpyHostToDevice, b->s) != hipSuccess)
+        return noctx_fail(-2, "cg_batch_iterate: FFT work table upload failed");
+    for (size_t g = 0; g + 1 < gstart.size(); g++) {
+        const int k0 = (int)gstart[g], cnt = (int)(gstart[g + 1] - gstart[g]);
+        const cg_ctx* c = b->ctx[order[k0].second];
+        if (fft_green_batch(b->d_fw + k0, cnt, c->f, c->mixed, order[k0].first[4], b->s) < 0)
+            return noctx_fail(-2, "cg_batch_iterate: batched FFT stage failed");
     }
     // the shared constitutive launches (one per kernel instance present)
     if (hipMemcpyAsync(b->d_work, b->h_work, n * sizeof(SolveWork), hipMemcpyHostToDevice, b->s) != hipSuccess)
@@ -1173,12 +1227,20 @@ int cg_batch_launch(cg_batch* b, cg_process* const* p, cg_iter_state* st, cg_ite
         launch_correct_solve_batch(b->d_work, b->d_blk_rve + b->grp_off[g], b->grp_nblk[g], (k >> 3) & 1, (k >> 2) & 1,
                                    (k >> 1) & 1, k & 1, b->s);
     }
+    // the batched RVEs' sums (straight to their host copies)
+    for (size_t g = 0; g + 1 < gstart.size(); g++) {
+        const int k0 = (int)gstart[g], cnt = (int)(gstart[g + 1] - gstart[g]);
+        launch_pass_sums_batch(b->d_fw + k0, cnt, solve_blocks(b->ctx[order[k0].second]->V), R_NSOLVE, R_NCOPY, b->s,
+                               nullptr);
+    }
     hipEventRecord(b->ev_solve, b->s);
-    // reductions and copies back, on every RVE's stream
+    // the other RVEs' sums on their own streams; every RVE's later work waits for the round
     for (int i = 0; i < n; i++) {
         if (!active[i]) continue;
         hipStreamWaitEvent(b->ctx[i]->s, b->ev_solve, 0);
-        if (int e = pass_reduce(b->ctx[i])) return noctx_fail(e, "cg_batch_launch: RVE %d: %s", i, b->ctx[i]->err);
+        if (!bat[i])
+            if (int e = pass_reduce(b->ctx[i]))
+                return noctx_fail(e, "cg_batch_launch: RVE %d: %s", i, b->ctx[i]->err);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return noctx_fail(-2, "cg_batch_launch: %s", hipGetErrorString(e));

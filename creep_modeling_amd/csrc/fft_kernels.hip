@@ -205,11 +205,6 @@ __global__ __launch_bounds__(FBS) void k_y(Fields f, Plan3 p, int T, int inv, in
 // ---------------------------------------------------------------------------
 // z pass with the Green operator: forward z, Gamma(xi) : P^, inverse z
 // ---------------------------------------------------------------------------
-struct GreenDev {
-    double c0[81];
-    double s0[81];
-    int igamma;
-};
 
 // full 81-component Gamma at 0-based frequency (kx, ky, kz) for igamma = 1, 2
 __device__ void green81(const GreenDev& g, int kx, int ky, int kz, int n1, int n2, int n3, double* G) {
@@ -353,18 +348,19 @@ __device__ __forceinline__ void green_apply(const GreenDev& g, int kx, int ky, i
         for (int i = 0; i < 3; i++)
             for (int k = 0; k < 3; k++) {
                 const double* c = g.c0 + 27 * i + 3 * k;  // c[9*j + l] = c0_{i j k l}
-                A[3 * i + k] = c[0] * d00 + c[10] * d11 + c[20] * d22 + (c[1] + c[9]) * d01 + (c[2] + c[18]) * d02 +
-                               (c[11] + c[19]) * d12;
+                A[3 * i + k] = fma(c[11] + c[19], d12,
+                                   fma(c[2] + c[18], d02,
+                                       fma(c[1] + c[9], d01, fma(c[20], d22, fma(c[10], d11, c[0] * d00)))));
             }
         inv3(A, Ai);
         double qr[3], qi[3];
         for (int k = 0; k < 3; k++) {
-            qr[k] = (double)in[3 * k].x * xi[0] + (double)in[3 * k + 1].x * xi[1] + (double)in[3 * k + 2].x * xi[2];
-            qi[k] = (double)in[3 * k].y * xi[0] + (double)in[3 * k + 1].y * xi[1] + (double)in[3 * k + 2].y * xi[2];
+            qr[k] = fma((double)in[3 * k + 2].x, xi[2], fma((double)in[3 * k + 1].x, xi[1], (double)in[3 * k].x * xi[0]));
+            qi[k] = fma((double)in[3 * k + 2].y, xi[2], fma((double)in[3 * k + 1].y, xi[1], (double)in[3 * k].y * xi[0]));
         }
         for (int i = 0; i < 3; i++) {
-            const double ur = -(Ai[3 * i] * qr[0] + Ai[3 * i + 1] * qr[1] + Ai[3 * i + 2] * qr[2]);
-            const double ui = -(Ai[3 * i] * qi[0] + Ai[3 * i + 1] * qi[1] + Ai[3 * i + 2] * qi[2]);
+            const double ur = -fma(Ai[3 * i + 2], qr[2], fma(Ai[3 * i + 1], qr[1], Ai[3 * i] * qr[0]));
+            const double ui = -fma(Ai[3 * i + 2], qi[2], fma(Ai[3 * i + 1], qi[1], Ai[3 * i] * qi[0]));
             for (int j = 0; j < 3; j++) {
                 P[(3 * i + j) * st].x = (R)(ur * xi[j]);
                 P[(3 * i + j) * st].y = (R)(ui * xi[j]);
@@ -601,6 +597,12 @@ int fft_green_passes(const Fields& f, FftCtx* c, const double* sgsum_dev, const 
     return run_green_reg<double>(f, c->tw64 + pl.ax[0].tw_off, c->tw64 + pl.ax[1].tw_off,
                                  c->tw64 + pl.ax[2].tw_off, sgsum_dev, g, sgsol, c->tune, s, st0, st1);
 }
+int fft_green_batch(const FftWork* w, int nrve, const Fields& f0, int mixed, int general, hipStream_t s);
+void fft_twiddles(const FftCtx* c, int mixed, const void** tw) {
+    const Plan3& pl = c->plan;
+    for (int k = 0; k < 3; k++)
+        tw[k] = mixed ? (const void*)(c->tw32 + pl.ax[k].tw_off) : (const void*)(c->tw64 + pl.ax[k].tw_off);
+}
 FftTune fft_get_tune(const FftCtx* c) { return c->tune; }
 void fft_set_tune(FftCtx* c, const FftTune& t) { c->tune = t; }
 bool fft_tunable(const Fields& f, const FftCtx* c) { return c->use_reg && fft_reg_supported(f); }
@@ -825,10 +827,35 @@ template <int N, typename R> struct XDims {
 #ifndef CG_XPAD
 #define CG_XPAD 1
 #endif
+#ifndef CG_Y_HX
+#define CG_Y_HX 1
+#endif
+#ifndef CG_XF_ATTR
+#define CG_XF_ATTR
+#endif
+#ifndef CG_XI_ATTR
+#define CG_XI_ATTR
+#endif
+#ifndef CG_Y_ATTR
+#define CG_Y_ATTR
+#endif
+#ifndef CG_Z_ATTR
+#define CG_Z_ATTR
+#endif
 __device__ __forceinline__ int fft_block(int xcd) {
     const int b = blockIdx.x, nb = gridDim.x;
     if (!xcd) return b;
     const int q = nb / 8;
+    return b < 8 * q ? (b % 8) * q + b / 8 : b;
+}
+// The batched passes run over FftWork entries: blocks [r nbs, (r+1) nbs) of a launch work for
+// entry r (one per RVE); an entry's blocks do what a context's own launch does (the same block
+// functions), with xcd placed as fft_block places a launch's blocks.
+__device__ __forceinline__ int entry_block(int nbs, int xcd, int* r) {
+    *r = __builtin_amdgcn_readfirstlane(blockIdx.x / nbs);
+    const int b = blockIdx.x - *r * nbs;
+    if (!xcd) return b;
+    const int q = nbs / 8;
     return b < 8 * q ? (b % 8) * q + b / 8 : b;
 }
 
@@ -840,9 +867,8 @@ template <typename R> __device__ __forceinline__ typename Cx<R>::T* spec_of(cons
 // Components (2p, 2p+1) of a row form one complex signal; the two half spectra are
 // separated with X[N-k] afterwards.
 template <int N, typename R>
-__global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
-                                                   const double* __restrict__ sgsum, const double* __restrict__ sgsol,
-                                                   int xcd) {
+__device__ __forceinline__ void x_fwd_blk(const Fields& f, const typename Cx<R>::T* __restrict__ tw,
+                                          const double* __restrict__ sgsum, const double* __restrict__ sgsol, int lb) {
     // 1 / the mixed form's stress scale (1 in FP64)
     const double inv_scale = sizeof(R) == 8 ? 1.0 : 1.0 / mixed_scale(sgsol);
     using T2 = typename Cx<R>::T;
@@ -857,8 +883,7 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_reg(Fields f, const
     static_assert(RW * 9 * SP * sizeof(R) <= N * NS * sizeof(T2), "stage fits the exchange");
     const long V = f.V;
     const long nrows = (long)f.n2 * f.n3;
-    const int lb = fft_block(xcd);  // the block's rows (and, in the x inverse, its partial-sum slot)
-    const long row0 = (long)lb * RW;
+    const long row0 = (long)lb * RW;  // the block's rows
     double sa[9];
     for (int c = 0; c < 9; c++) sa[c] = sgsum[c];
     // each thread's voxels: every load is issued before the products (rows past the grid
@@ -929,15 +954,31 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_reg(Fields f, const
         spec[row * w + rem] = o;
     }
 }
+template <int N, typename R>
+__global__ __launch_bounds__((XDims<N, R>::XB)) CG_XF_ATTR void k_x_fwd_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
+                                                                          const double* __restrict__ sgsum,
+                                                                          const double* __restrict__ sgsol, int xcd) {
+    x_fwd_blk<N, R>(f, tw, sgsum, sgsol, fft_block(xcd));
+}
+template <int N, typename R>
+__global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_bat(const FftWork* __restrict__ w, int nbs) {
+    int r;
+    const int lb = entry_block(nbs, 0, &r);
+    const FftWork& e = w[r];
+    const Fields f = e.f;
+    x_fwd_blk<N, R>(f, (const typename Cx<R>::T*)e.tw[0], e.sgsum, e.sgsol, lb);
+}
 
-// y pass over every (kx, c) column of one z plane
+// y pass over every (kx, c) column of one z plane.  With two level-2 rounds per thread
+// (U == 2: lengths 32, 96 and 128) the exchange runs in two rounds through half the LDS and each
+// round's outputs are stored at once (fft_dist2), so more blocks fit a CU.
 template <int N, bool INV, typename R, int YB>
-__global__ __launch_bounds__(YB) void k_y_reg(Fields f, const typename Cx<R>::T* __restrict__ tw, int ntile, int xcd) {
+__device__ __forceinline__ void y_blk(const Fields& f, const typename Cx<R>::T* __restrict__ tw, int ntile, int bid) {
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
     constexpr int T = D::T, E = D::E, U = D::U, NS = YDims<N, YB>::NS;
-    __shared__ T2 ex[N * NS];
-    const int bid = fft_block(xcd);
+    constexpr bool HX = CG_Y_HX && U == 2;
+    __shared__ T2 ex[HX ? T * T * NS : N * NS];
     const int k = bid / ntile, tile = bid - k * ntile;
     const int ncol = f.h1 * 9;
     const int s = threadIdx.x % NS, t = threadIdx.x / NS;
@@ -947,6 +988,14 @@ __global__ __launch_bounds__(YB) void k_y_reg(Fields f, const typename Cx<R>::T*
     T2 a[D::AN];
 #pragma unroll
     for (int m = 0; m < E; m++) a[m] = act ? base[(long)(t + T * m) * ncol] : T2{0, 0};
+    if constexpr (HX) {
+        fft_dist2<N, T, INV>(a, s, t, NS, ex, tw, [&](int, int q, const T2* d) {
+            if (act)
+#pragma unroll
+                for (int j = 0; j < T; j++) base[(long)(q + E * j) * ncol] = d[j];
+        });
+        return;
+    }
     fft_dist<N, T, INV>(a, s, t, NS, ex, tw);
     if (act)
 #pragma unroll
@@ -957,18 +1006,29 @@ __global__ __launch_bounds__(YB) void k_y_reg(Fields f, const typename Cx<R>::T*
                 for (int j = 0; j < T; j++) base[(long)(q + E * j) * ncol] = a[u * T + j];
         }
 }
+template <int N, bool INV, typename R, int YB>
+__global__ __launch_bounds__(YB) CG_Y_ATTR void k_y_reg(Fields f, const typename Cx<R>::T* __restrict__ tw, int ntile, int xcd) {
+    y_blk<N, INV, R, YB>(f, tw, ntile, fft_block(xcd));
+}
+template <int N, bool INV, typename R, int YB>
+__global__ __launch_bounds__(YB) CG_Y_ATTR void k_y_bat(const FftWork* __restrict__ w, int ntile, int nbs) {
+    int r;
+    const int lb = entry_block(nbs, 0, &r);
+    const FftWork& e = w[r];
+    const Fields f = e.f;
+    y_blk<N, INV, R, YB>(f, (const typename Cx<R>::T*)e.tw[1], ntile, lb);
+}
 
 // z pass with the Green operator: KT whole kx columns (9 components each) of one y row
 template <int N, typename R, bool GENERAL, bool KT1>
-__global__ __launch_bounds__((ZDims<N, KT1>::ZB)) void k_z_green_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
-                                                                    GreenDev g, int ntile, int xcd) {
+__device__ __forceinline__ void z_blk(const Fields& f, const typename Cx<R>::T* __restrict__ tw, const GreenDev& g,
+                                      int ntile, int bid) {
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
     using Z = ZDims<N, KT1>;
     constexpr int T = D::T, E = D::E, U = D::U, NS = Z::NSZ, KT = Z::KT;
     __shared__ T2 ex[N * NS];
     const int h1 = f.h1;
-    const int bid = fft_block(xcd);
     const int j = bid / ntile, tile = bid - j * ntile;
     const int kx0 = tile * KT;
     const int KTe = min(KT, h1 - kx0);
@@ -1007,13 +1067,26 @@ __global__ __launch_bounds__((ZDims<N, KT1>::ZB)) void k_z_green_reg(Fields f, c
                 for (int jj = 0; jj < T; jj++) base[(long)(q + E * jj) * ps] = a[u * T + jj];
         }
 }
+template <int N, typename R, bool GENERAL, bool KT1>
+__global__ __launch_bounds__((ZDims<N, KT1>::ZB)) CG_Z_ATTR void k_z_green_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
+                                                                          GreenDev g, int ntile, int xcd) {
+    z_blk<N, R, GENERAL, KT1>(f, tw, g, ntile, fft_block(xcd));
+}
+template <int N, typename R, bool GENERAL, bool KT1>
+__global__ __launch_bounds__((ZDims<N, KT1>::ZB)) CG_Z_ATTR void k_z_bat(const FftWork* __restrict__ w, int ntile, int nbs) {
+    int r;
+    const int lb = entry_block(nbs, 0, &r);
+    const FftWork& e = w[r];
+    const Fields f = e.f;
+    z_blk<N, R, GENERAL, KT1>(f, (const typename Cx<R>::T*)e.tw[2], e.g, ntile, lb);
+}
 
 // x inverse (c2r): dLref = Re(IFFT)/V with the Hermitian continuation of the half
 // spectrum (imaginary parts at kx = 0 and n1/2 dropped, as Re() of the reference's
 // c2c inverse does); block partial sums of w dLref F^-1.
 template <int N, typename R>
-__global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
-                                                   const double* __restrict__ sgsol, int xcd) {
+__device__ __forceinline__ void x_inv_blk(const Fields& f, const typename Cx<R>::T* __restrict__ tw,
+                                          const double* __restrict__ sgsol, int lb, int nbs) {
     const double out_scale = (sizeof(R) == 8 ? 1.0 : mixed_scale(sgsol)) / (double)f.V;  // stress scale / V
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
@@ -1026,8 +1099,7 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_reg(Fields f, const
     static_assert(RW * 9 * SP * sizeof(R) <= N * NS * sizeof(T2), "output rows fit the exchange");
     const long V = f.V;
     const long nrows = (long)f.n2 * f.n3;
-    const int lb = fft_block(xcd);  // the block's rows (and, in the x inverse, its partial-sum slot)
-    const long row0 = (long)lb * RW;
+    const long row0 = (long)lb * RW;  // the block's rows (and its partial-sum slot of nbs)
     constexpr int w = H * 9;
     const T2* spec = spec_of<R>(f);
     // the block's spectrum rows: every load is issued before the first is written to LDS
@@ -1113,7 +1185,20 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_reg(Fields f, const
         for (int c = 0; c < 9; c++) acc[c] += q9[c] * wp[q];
     }
     // partials in row order whatever the block placement, so the reduced sums do not depend on it
-    block_reduce_store_cm<9, XBS>(acc, f.part, 0, lb, gridDim.x);
+    block_reduce_store_cm<9, XBS>(acc, f.part, 0, lb, nbs);
+}
+template <int N, typename R>
+__global__ __launch_bounds__((XDims<N, R>::XB)) CG_XI_ATTR void k_x_inv_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
+                                                                          const double* __restrict__ sgsol, int xcd) {
+    x_inv_blk<N, R>(f, tw, sgsol, fft_block(xcd), gridDim.x);
+}
+template <int N, typename R>
+__global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_bat(const FftWork* __restrict__ w, int nbs) {
+    int r;
+    const int lb = entry_block(nbs, 0, &r);
+    const FftWork& e = w[r];
+    const Fields f = e.f;
+    x_inv_blk<N, R>(f, (const typename Cx<R>::T*)e.tw[0], e.sgsol, lb, nbs);
 }
 
 static bool reg_size(int n) { return n == 16 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128 || n == 256; }
@@ -1202,6 +1287,56 @@ static int reg_stage(int stage, const Fields& f, const typename Cx<R>::T* tw1, c
 #undef CG_Z
 #undef CG_YI
 #undef CG_XI
+    return nb;
+}
+
+// the batched passes over nrve FftWork entries of grids with f's dimensions (default shapes)
+template <int N, typename R> static int bat_x_fwd(const FftWork* w, int nrve, const Fields& f, hipStream_t s) {
+    using X = XDims<N, R>;
+    const int nbs = (int)(((long)f.n2 * f.n3 + X::RW - 1) / X::RW);
+    hipLaunchKernelGGL((k_x_fwd_bat<N, R>), dim3(nbs * nrve), dim3(X::XB), 0, s, w, nbs);
+    return nbs;
+}
+template <int N, typename R> static int bat_x_inv(const FftWork* w, int nrve, const Fields& f, hipStream_t s) {
+    using X = XDims<N, R>;
+    const int nbs = (int)(((long)f.n2 * f.n3 + X::RW - 1) / X::RW);
+    hipLaunchKernelGGL((k_x_inv_bat<N, R>), dim3(nbs * nrve), dim3(X::XB), 0, s, w, nbs);
+    return nbs;
+}
+template <int N, bool INV, typename R> static void bat_y(const FftWork* w, int nrve, const Fields& f, hipStream_t s) {
+    constexpr int NS = YDims<N, YBS>::NS;
+    const int ntile = (f.h1 * 9 + NS - 1) / NS, nbs = f.n3 * ntile;
+    hipLaunchKernelGGL((k_y_bat<N, INV, R, YBS>), dim3(nbs * nrve), dim3(YBS), 0, s, w, ntile, nbs);
+}
+template <int N, typename R> static void bat_z(const FftWork* w, int nrve, const Fields& f, int general, hipStream_t s) {
+    using Z = ZDims<N, false>;
+    const int ntile = (f.h1 + Z::KT - 1) / Z::KT, nbs = f.n2 * ntile;
+    if (general) hipLaunchKernelGGL((k_z_bat<N, R, true, false>), dim3(nbs * nrve), dim3(Z::ZB), 0, s, w, ntile, nbs);
+    else hipLaunchKernelGGL((k_z_bat<N, R, false, false>), dim3(nbs * nrve), dim3(Z::ZB), 0, s, w, ntile, nbs);
+}
+template <typename R> static int green_batch(const FftWork* w, int nrve, const Fields& f, int general, hipStream_t s) {
+    int nb = -1;
+#define CG_BXF(N) bat_x_fwd<N, R>(w, nrve, f, s)
+#define CG_BYF(N) bat_y<N, false, R>(w, nrve, f, s)
+#define CG_BZ(N) bat_z<N, R>(w, nrve, f, general, s)
+#define CG_BYI(N) bat_y<N, true, R>(w, nrve, f, s)
+#define CG_BXI(N) nb = bat_x_inv<N, R>(w, nrve, f, s)
+    CG_SWITCH_N(f.n1, CG_BXF)
+    CG_SWITCH_N(f.n2, CG_BYF)
+    CG_SWITCH_N(f.n3, CG_BZ)
+    CG_SWITCH_N(f.n2, CG_BYI)
+    CG_SWITCH_N(f.n1, CG_BXI)
+#undef CG_BXF
+#undef CG_BYF
+#undef CG_BZ
+#undef CG_BYI
+#undef CG_BXI
+    return nb;
+}
+int fft_green_batch(const FftWork* w, int nrve, const Fields& f0, int mixed, int general, hipStream_t s) {
+    if (!fft_reg_supported(f0) || nrve < 1) return -1;
+    const int nb = mixed ? green_batch<float>(w, nrve, f0, general, s) : green_batch<double>(w, nrve, f0, general, s);
+    if (nb > 0) launch_reduce_fluct_batch(w, nrve, nb, s);
     return nb;
 }
 

@@ -154,6 +154,45 @@ __device__ __forceinline__ void fft_dist(T2* a, int s, int t, int NS, T2* ex, co
     }
 }
 
+// fft_dist with the exchange in two rounds (U == 2: E = T + r, 0 < r <= T): round u passes the
+// level-1 outputs q in [u T, min((u + 1) T, E)) through ex, which therefore holds T*T*NS
+// elements (element (q - u T, j, s) at ex[((q - u T) * T + j) * NS + s]) instead of N*NS;
+// thread t then does the level-2 transform q = t + T u (if q < E) and hands its outputs
+// d[j] = X[q + E j] to sink(u, q, d) before the next round.  Same arithmetic as fft_dist.
+// The caller separates earlier uses of ex with a barrier; ex is free again when this returns
+// (after a barrier).
+template <int N, int T, bool INV, typename T2, typename TW, class Sink>
+__device__ __forceinline__ void fft_dist2(T2* a, int s, int t, int NS, T2* ex, const TW* tw, Sink sink) {
+    constexpr int E = N / T;
+    constexpr int U = (E + T - 1) / T;
+    static_assert(E * T == N && U == 2, "two exchange rounds");
+    T2 b[E];
+    Dft<E, INV>::run(a, b);
+#pragma unroll
+    for (int q = 1; q < E; q++) {
+        TW w = tw[(t * q) % N];
+        if (INV) w.y = -w.y;
+        b[q] = cxmul(b[q], T2{w.x, w.y});
+    }
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int nq = u == 0 ? T : E - T;
+#pragma unroll
+        for (int qq = 0; qq < T; qq++)
+            if (qq < nq) ex[(qq * T + t) * NS + s] = b[u * T + qq];
+        __syncthreads();
+        const int q = t + T * u;
+        if (t < nq) {
+            T2 c[T], d[T];
+#pragma unroll
+            for (int j = 0; j < T; j++) c[j] = ex[(t * T + j) * NS + s];
+            Dft<T, INV>::run(c, d);
+            sink(u, q, d);
+        }
+        __syncthreads();
+    }
+}
+
 // Shapes of the distributed transforms for the sizes that have kernels
 // (T threads per signal, E points per thread, U level-2 transforms per thread).
 template <int N> struct Shape;

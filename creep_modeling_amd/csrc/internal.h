@@ -158,6 +158,32 @@ void launch_xnode(const Fields& f, const double* dv, const double* lavg, double 
 
 // ---- fft_kernels.hip ----
 struct FftCtx;
+// the reference medium's stiffness and compliance (4th-order, 81 each) and the Green operator kind
+struct GreenDev {
+    double c0[81];
+    double s0[81];
+    int igamma;
+};
+// One RVE of a batched pass (cg_batch_*): what its FFT + Gamma stage and its sums read and write.
+struct FftWork {
+    Fields f;
+    const void* tw[3];   // twiddles of the x, y, z transforms (the group's precision)
+    const double* sgsum; // <sigma> (9)
+    const double* sgsol; // the previous pass' solid stress sums (9), for the mixed-form scale
+    double* fluct;       // 9 sums of w dLref F^-1
+    double* host;        // the pass' sums, host-mapped
+    GreenDev g;
+};
+// The FFT + Gamma stage of nrve RVEs with the same grid (f0's dimensions) and precision, w on the
+// device: one launch per pass over all of them (default block shapes), then their sums of
+// w dLref F^-1 (two launches).  Each RVE computes what fft_green computes for it alone.
+int fft_green_batch(const FftWork* w, int nrve, const Fields& f0, int mixed, int general, hipStream_t s);
+// (solver_kernels.hip) the two-level sums of w dLref F^-1 of such a group into each RVE's fluct;
+// nblocks: the x inverse's blocks per RVE
+void launch_reduce_fluct_batch(const FftWork* w, int nrve, int nblocks, hipStream_t s);
+// a pass' sums of a group of RVEs with nblocks solve blocks each (launch_pass_sums for each RVE)
+void launch_pass_sums_batch(const FftWork* w, int nrve, int nblocks, int nvals, int ncopy, hipStream_t s,
+                            hipEvent_t done);
 // Block shapes and placement of the register FFT passes, picked by the autotuner (stages:
 // 0 x forward, 1 y forward, 2 z + Gamma, 3 y inverse, 4 x inverse); none changes the results.
 struct FftTune {
@@ -186,6 +212,8 @@ int fft_green_passes(const Fields& f, FftCtx* c, const double* sgsum_dev, const 
 int fft_tune(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
              int mixed, const double* sgsol, hipStream_t s);
 FftTune fft_get_tune(const FftCtx* c);
+// the register passes' twiddle tables (x, y, z) of a context, for its FftWork entry
+void fft_twiddles(const FftCtx* c, int mixed, const void** tw);
 void fft_set_tune(FftCtx* c, const FftTune& t);
 bool fft_tunable(const Fields& f, const FftCtx* c);
 // spectral integration of L F for the nodal update: dv (3 x V) and 9 block partial

@@ -90,10 +90,9 @@ __global__ __launch_bounds__(BS) void k_reduce_final(const double* __restrict__ 
         out[i] = t;
     }
 }
-// the same over a column-major partial array part[i * nblocks + b]
-__global__ __launch_bounds__(BS) void k_reduce_final_cm(const double* __restrict__ part, int nblocks,
-                                                        double* __restrict__ out) {
-    const int i = blockIdx.x;
+// the same over a column-major partial array part[i * nblocks + b] (block i sums column i)
+__device__ __forceinline__ void reduce_final_cm_blk(const double* __restrict__ part, int nblocks,
+                                                    double* __restrict__ out, int i) {
     double s = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += BS) s += part[(size_t)i * nblocks + b];
     __shared__ double sm[BS / 64];
@@ -107,12 +106,15 @@ __global__ __launch_bounds__(BS) void k_reduce_final_cm(const double* __restrict
         out[i] = t;
     }
 }
+__global__ __launch_bounds__(BS) void k_reduce_final_cm(const double* __restrict__ part, int nblocks,
+                                                        double* __restrict__ out) {
+    reduce_final_cm_blk(part, nblocks, out, blockIdx.x);
+}
 // First level of the column-major reduction: block (c, i) sums the partials
-// [c*CH, (c+1)*CH) of value i into tmp[i * gridDim.x + c] (fixed order).
+// [c*CH, (c+1)*CH) of value i into tmp[i * nch + c] (fixed order).
 constexpr int RCH = 1024;
-__global__ __launch_bounds__(BS) void k_reduce_cols(const double* __restrict__ part, int nblocks,
-                                                    double* __restrict__ tmp) {
-    const int c = blockIdx.x, i = blockIdx.y;
+__device__ __forceinline__ void reduce_cols_blk(const double* __restrict__ part, int nblocks, double* __restrict__ tmp,
+                                                int c, int i, int nch) {
     const long b0 = (long)c * RCH;
     const long n = nblocks - b0 < RCH ? nblocks - b0 : RCH;
     const double* col = part + (size_t)i * nblocks + b0;
@@ -130,8 +132,29 @@ __global__ __launch_bounds__(BS) void k_reduce_cols(const double* __restrict__ p
     if (threadIdx.x == 0) {
         double t = sm[0];
         for (int w = 1; w < BS / 64; w++) t += sm[w];
-        tmp[(size_t)i * gridDim.x + c] = t;
+        tmp[(size_t)i * nch + c] = t;
     }
+}
+__global__ __launch_bounds__(BS) void k_reduce_cols(const double* __restrict__ part, int nblocks,
+                                                    double* __restrict__ tmp) {
+    reduce_cols_blk(part, nblocks, tmp, blockIdx.x, blockIdx.y, gridDim.x);
+}
+// The same two levels for a group of RVEs (FftWork entries, RVE = blockIdx.z, resp. blockIdx.y):
+// nvals columns of nblocks partials in each RVE's f.part, chunk sums after them.
+__global__ __launch_bounds__(BS) void k_reduce_cols_bat(const FftWork* __restrict__ w, int nblocks, int nvals) {
+    const FftWork& e = w[blockIdx.z];
+    reduce_cols_blk(e.f.part, nblocks, e.f.part + (size_t)nvals * nblocks, blockIdx.x, blockIdx.y, gridDim.x);
+}
+// second level into each RVE's fluct (9 sums of w dLref F^-1)
+__global__ __launch_bounds__(BS) void k_reduce_fluct_bat(const FftWork* __restrict__ w, int nblocks, int nch) {
+    const FftWork& e = w[blockIdx.y];
+    if (nch <= 1) reduce_final_cm_blk(e.f.part, nblocks, e.fluct, blockIdx.x);
+    else reduce_final_cm_blk(e.f.part + (size_t)9 * nblocks, nch, e.fluct, blockIdx.x);
+}
+void launch_reduce_fluct_batch(const FftWork* w, int nrve, int nblocks, hipStream_t s) {
+    const int nch = (nblocks + RCH - 1) / RCH;
+    if (nch > 1) hipLaunchKernelGGL(k_reduce_cols_bat, dim3(nch, 9, nrve), dim3(BS), 0, s, w, nblocks, 9);
+    hipLaunchKernelGGL(k_reduce_fluct_bat, dim3(9, nrve), dim3(BS), 0, s, w, nblocks, nch);
 }
 // part holds nvals columns of nblocks partials; the space after them (nvals * ceil(nblocks /
 // 1024) doubles) is scratch for the two-level reduction of long columns
@@ -158,8 +181,7 @@ struct PassFinal {
     double* sgsum;  // 9
     double* sgsol;  // 9
 };
-__global__ __launch_bounds__(BS) void k_pass_final(PassFinal a) {
-    const int i = blockIdx.x;
+__device__ __forceinline__ void pass_final_blk(const PassFinal& a, int i) {
     double s = 0.0;
     for (int b = threadIdx.x; b < a.n; b += BS) s += a.part[(size_t)i * a.n + b];
     __shared__ double sm[BS / 64];
@@ -178,6 +200,26 @@ __global__ __launch_bounds__(BS) void k_pass_final(PassFinal a) {
     if (i == 0)
         for (int k = a.nvals + threadIdx.x; k < a.ncopy; k += BS) a.host[k] = a.red[k];
     __threadfence_system();
+}
+__global__ __launch_bounds__(BS) void k_pass_final(PassFinal a) { pass_final_blk(a, blockIdx.x); }
+__global__ __launch_bounds__(BS) void k_pass_final_bat(const FftWork* __restrict__ w, int nblocks, int nvals,
+                                                       int ncopy, int nch) {
+    const FftWork& e = w[blockIdx.y];
+    PassFinal a{e.f.part, nblocks, nvals, ncopy, e.f.red, e.host, const_cast<double*>(e.sgsum),
+                const_cast<double*>(e.sgsol)};
+    if (nch > 1) {
+        a.part = e.f.part + (size_t)nvals * nblocks;
+        a.n = nch;
+    }
+    pass_final_blk(a, blockIdx.x);
+}
+// the pass sums of a group of RVEs with the same voxel count (as launch_pass_sums for each)
+void launch_pass_sums_batch(const FftWork* w, int nrve, int nblocks, int nvals, int ncopy, hipStream_t s,
+                            hipEvent_t done) {
+    const int nch = (nblocks + RCH - 1) / RCH;
+    if (nch > 1) hipLaunchKernelGGL(k_reduce_cols_bat, dim3(nch, nvals, nrve), dim3(BS), 0, s, w, nblocks, nvals);
+    hipExtLaunchKernelGGL(k_pass_final_bat, dim3(nvals, nrve), dim3(BS), 0, s, nullptr, done, 0, w, nblocks, nvals,
+                          ncopy, nch);
 }
 void launch_pass_sums(double* part, int nblocks, int nvals, int ncopy, double* red, double* host, double* sgsum,
                       double* sgsol, hipStream_t s, hipEvent_t done) {
