@@ -3,13 +3,15 @@
 The reference runs its calibration and microstructure sweeps as independent solver
 processes (``utils/calibrate.py:1054-1058``, ``utils/sweep_microstructures.py:24-29``,
 ``utils/run.py:183-196``).  Here every RVE is one context of the library and the RVEs advance
-together, one fixed-point pass of every RVE per round (``cg_batch_iterate``): each RVE's
-spectral stage is enqueued on its own stream and the correction + constitutive kernel, most of
-a pass, runs as one launch over a per-RVE work table, so small RVEs together fill the GPU.
-Between passes each RVE runs its own increment control (convergence, Delta-t cut-back,
-rollback); an RVE whose increment has ended takes its accepted-increment updates and starts the
-next one, and an RVE whose run is over retires from the work table while the others continue.
-A batched RVE computes exactly what it computes alone.
+together, one fixed-point pass of every RVE per round (``cg_batch_launch`` / ``cg_batch_wait``):
+each FFT pass, the sums and the correction + constitutive kernel run as one launch per kernel
+over per-RVE work tables (one set of FFT launches per grid size), so small RVEs together fill
+the GPU.  The RVEs are split into two batches launched in turn, so the host's increment
+control of one batch overlaps the other's pass on the GPU.  Between passes each RVE runs its
+own increment control (convergence, Delta-t cut-back, rollback); an RVE whose increment has
+ended takes its accepted-increment updates and starts the next one, and an RVE whose run is
+over retires from the work tables while the others continue.  A batched RVE computes exactly
+what it computes alone.
 
 ``engine="threads"`` instead drives each RVE's ordinary loop from a pool of host threads
 (one context and stream per RVE, ``workers`` in flight).
@@ -24,24 +26,40 @@ from . import driver
 from . import lib as _lib
 
 
-def run_batch(problems, steps=None, workers=4, precision=0, solvers=None, on_done=None, engine="shared"):
+def run_batch(problems, steps=None, workers=4, precision=0, solvers=None, on_done=None, engine="shared", groups=2):
     """Run `steps` accepted increments (all of them when None) of every problem.
 
     Returns the per-problem lists of row dicts, in input order.  `solvers`, if given, are
     already-created :class:`driver.Solver` objects (one per problem, closed here).
     `on_done(i, solver)`, if given, is called when RVE i has finished, before its context is
-    closed (e.g. to read its fields)."""
+    closed (e.g. to read its fields).  The shared engine splits the RVEs into `groups` batches of
+    about equal voxel counts, launched in turn: while the GPU runs one batch's pass the host
+    does the increment control of the other's."""
     if engine == "threads":
         return _run_threads(problems, steps, workers, precision, solvers, on_done)
     n = len(problems)
     if solvers is None:
         solvers = [driver.Solver(p, precision=precision) for p in problems]
     L = _lib.load()
-    handles = (C.c_void_p * n)(*[s.h.value for s in solvers])
-    b = C.c_void_p()
+    # groups of about equal voxel counts (largest RVE first into the lightest group)
+    ng = max(1, min(groups, n))
+    members = [[] for _ in range(ng)]
+    load = [0] * ng
+    for i in sorted(range(n), key=lambda i: -problems[i].nvox):
+        g = load.index(min(load))
+        members[g].append(i)
+        load[g] += problems[i].nvox
+    batches = []
     try:
-        if L.cg_batch_create(handles, n, C.byref(b)) != 0:
-            raise _lib.HipError("cg_batch_create: %s" % L.cg_last_error(None).decode())
+        for mem in members:
+            k = len(mem)
+            handles = (C.c_void_p * k)(*[solvers[i].h.value for i in mem])
+            b = C.c_void_p()
+            if L.cg_batch_create(handles, k, C.byref(b)) != 0:
+                raise _lib.HipError("cg_batch_create: %s" % L.cg_last_error(None).decode())
+            batches.append(dict(b=b, mem=mem, procs=(C.POINTER(_lib.CgProcess) * k)(), sts=(_lib.CgIterState * k)(),
+                                outs=(_lib.CgIterOut * k)(), active=(C.c_int * k)(), rc=(C.c_int * k)(),
+                                pending=False))
         results = [[] for _ in range(n)]
         live = [False] * n
 
@@ -51,37 +69,54 @@ def run_batch(problems, steps=None, workers=4, precision=0, solvers=None, on_don
             if not live[i] and on_done is not None:
                 on_done(i, s)
 
-        for i in range(n):
-            start(i)
-        procs = (C.POINTER(_lib.CgProcess) * n)()
-        sts = (_lib.CgIterState * n)()
-        outs = (_lib.CgIterOut * n)()
-        active = (C.c_int * n)()
-        rc = (C.c_int * n)()
-        while any(live):
-            for i in range(n):
-                active[i] = int(live[i])
+        def launch(bt):
+            mem, any_live = bt["mem"], False
+            for j, i in enumerate(mem):
+                bt["active"][j] = int(live[i])
                 if live[i]:
+                    any_live = True
                     s = solvers[i]
-                    procs[i] = C.pointer(s.P)
-                    C.memmove(C.byref(sts, i * C.sizeof(_lib.CgIterState)), C.byref(s.st), C.sizeof(_lib.CgIterState))
-            if L.cg_batch_iterate(b, procs, sts, outs, active, rc) != 0:
-                raise _lib.HipError("cg_batch_iterate: %s" % L.cg_last_error(None).decode())
-            for i in range(n):
-                if not live[i]:
+                    bt["procs"][j] = C.pointer(s.P)
+                    C.memmove(C.byref(bt["sts"], j * C.sizeof(_lib.CgIterState)), C.byref(s.st),
+                              C.sizeof(_lib.CgIterState))
+            if not any_live:
+                return
+            if L.cg_batch_launch(bt["b"], bt["procs"], bt["sts"], bt["outs"], bt["active"]) != 0:
+                raise _lib.HipError("cg_batch_launch: %s" % L.cg_last_error(None).decode())
+            bt["pending"] = True
+
+        def finish(bt):
+            bt["pending"] = False
+            if L.cg_batch_wait(bt["b"], bt["rc"]) != 0:
+                raise _lib.HipError("cg_batch_wait: %s" % L.cg_last_error(None).decode())
+            for j, i in enumerate(bt["mem"]):
+                if not bt["active"][j]:
                     continue
                 s = solvers[i]
-                if rc[i] != 0:
-                    raise _lib.HipError("RVE %d: cg_iterate rc=%d: %s" % (i, rc[i], L.cg_last_error(s.h).decode()))
-                C.memmove(C.byref(s.st), C.byref(sts, i * C.sizeof(_lib.CgIterState)), C.sizeof(_lib.CgIterState))
-                C.memmove(C.byref(s.out), C.byref(outs, i * C.sizeof(_lib.CgIterOut)), C.sizeof(_lib.CgIterOut))
+                if bt["rc"][j] != 0:
+                    raise _lib.HipError("RVE %d: cg_iterate rc=%d: %s" % (i, bt["rc"][j], L.cg_last_error(s.h).decode()))
+                C.memmove(C.byref(s.st), C.byref(bt["sts"], j * C.sizeof(_lib.CgIterState)),
+                          C.sizeof(_lib.CgIterState))
+                C.memmove(C.byref(s.out), C.byref(bt["outs"], j * C.sizeof(_lib.CgIterOut)), C.sizeof(_lib.CgIterOut))
                 if s.pass_done():
                     results[i].append(s.finish())
                     start(i)
+
+        for i in range(n):
+            start(i)
+        for bt in batches:
+            launch(bt)
+        while any(bt["pending"] for bt in batches):
+            for bt in batches:
+                if bt["pending"]:
+                    finish(bt)
+                    launch(bt)
         return results
     finally:
-        if b.value:
-            L.cg_batch_destroy(b)
+        for bt in batches:
+            if bt["pending"]:
+                L.cg_batch_wait(bt["b"], bt["rc"])
+            L.cg_batch_destroy(bt["b"])
         for s in solvers:
             s.close()
 
