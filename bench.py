@@ -379,9 +379,9 @@ def main():
                                "sums_copy": sst.t_ms[2] / max(sst.passes, 1),
                                "per_increment_updates_ms": st.t_ms[3] / max(args.steps, 1),
                                "timing": "correct_solve and per-increment updates: the timed increments "
-                                         "(the solve's launch events); fft_green and sums_copy: stream-marker "
-                                         "events over the %d increments after them (%d passes)"
-                                         % (STAGE_STEPS, sst.passes)},
+                                         "(events attached to the solve's launches); fft_green: events attached "
+                                         "to the stage's first and last kernels, sums_copy: stream markers, over "
+                                         "the %d increments after them (%d passes)" % (STAGE_STEPS, sst.passes)},
         "fft_green_roofline": {"bound": "hbm", "achieved": fft_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": fft_gbs / HBM_PEAK_GBS, "alg_bytes_per_voxel": fft_alg_bytes(args.n),
                                "traffic": measured("fft_green", "bytes_per_launch", args)},

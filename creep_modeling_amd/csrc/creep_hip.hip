@@ -819,7 +819,10 @@ static int pass_begin(cg_ctx* c, cg_process* p, cg_iter_state* it, SolveArgs& sa
     }
     it->iter++;
     int nb;
-    tstart(c, 0);
+    // stage timers: events attached to the stage's first and last kernels where the register
+    // passes run outside a graph, else stream markers around the stage
+    const bool attach = c->timing == 1 && fft_tunable(f, c->fft) && !c->use_graphs;
+    if (!attach) tstart(c, 0);
     if (!c->sgsum_valid) {
         launch_sigma_sum(f, c->s, &nb);
         launch_reduce_final(f.part, nb, 9, c->d_sgsum, c->s);
@@ -829,10 +832,12 @@ static int pass_begin(cg_ctx* c, cg_process* p, cg_iter_state* it, SolveArgs& sa
     const double scale = pass_scale(c);
     auto stage = [&]() {
         int n;
+        hipEvent_t e0 = attach ? c->ev[0] : nullptr, e1 = attach ? c->ev[1] : nullptr;
         if (early)
-            n = fft_green_passes(f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, c->d_sgsol, 4, 5, c->s);
+            n = fft_green_passes(f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, c->d_sgsol, 4, 5, c->s, e0,
+                                 e1);
         else
-            n = fft_green(f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, scale, c->d_sgsol, c->s);
+            n = fft_green(f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, scale, c->d_sgsol, c->s, e0, e1);
         launch_reduce_final_cm(f.part, n, 9, c->d_fluct, c->s);
     };
     // the register kernels take the scale from d_sgsol, so their stage can be replayed as a graph
@@ -849,7 +854,7 @@ static int pass_begin(cg_ctx* c, cg_process* p, cg_iter_state* it, SolveArgs& sa
     } else {
         stage();
     }
-    tstop(c, 0);
+    if (!attach) tstop(c, 0);
     // correction + mixing + constitutive solve (one kernel)
     PassConst pc;
     pc.tdot = p->tdot; pc.complgas = c->complgas; pc.error = p->error; pc.erroral = p->erroral;
@@ -1011,12 +1016,10 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
         // the next pass' transforms run while the host does its part of this one (ev_sums is
         // recorded at the end of the sums' kernel)
         const int slot = used_slot == 0 ? 1 : 0;
-        if (c->timing == 1) {
-            hipEventRecord(c->ev[8 + 2 * slot], c->s);
-            c->early_timed[slot] = 1;
-        }
-        fft_green_passes(c->f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, c->d_sgsol, 0, 4, c->s);
-        if (c->timing == 1) hipEventRecord(c->ev[9 + 2 * slot], c->s);
+        const bool t1 = c->timing == 1;
+        c->early_timed[slot] = t1 ? 1 : 0;
+        fft_green_passes(c->f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, c->d_sgsol, 0, 4, c->s,
+                         t1 ? c->ev[8 + 2 * slot] : nullptr, t1 ? c->ev[9 + 2 * slot] : nullptr);
         c->early = 1;
         c->early_slot = slot;
         c->early_igamma = p->igamma;

@@ -14,6 +14,9 @@
 // The half spectrum is component-interleaved: spec[((k*n2 + j)*h1 + kx)*9 + c].
 // Template parameter R selects the arithmetic of the spectral stage (double:
 // FP64 form; float: mixed form with the input scaled by a stress norm).
+#include <hip/hip_ext.h>
+#include <string.h>
+
 #include <vector>
 
 #include "internal.h"
@@ -470,7 +473,22 @@ struct FftCtx {
     int rows, ty, tz;
     int use_reg;  // register-FFT kernels for the per-pass stage (all axis lengths supported)
     FftTune tune;  // block shapes and placement of the register passes
+    // the Green operator constants the register z pass reads (device copy, its pinned staging)
+    GreenDev* d_g = nullptr;
+    GreenDev* h_g = nullptr;
+    int g_valid = 0;
 };
+// The device copy of g, uploaded on stream s when it differs from the last one (g changes with C0,
+// once per increment; the stream has passed the previous upload by then, as every pass waits
+// for its sums).
+static const GreenDev* green_dev(FftCtx* c, const GreenDev& g, hipStream_t s) {
+    if (!c->g_valid || memcmp(c->h_g, &g, sizeof g) != 0) {
+        *c->h_g = g;
+        hipMemcpyAsync(c->d_g, c->h_g, sizeof g, hipMemcpyHostToDevice, s);
+        c->g_valid = 1;
+    }
+    return c->d_g;
+}
 
 FftCtx* fft_create(const Fields& f, char* err, int errlen) {
     FftCtx* c = new FftCtx();
@@ -507,6 +525,11 @@ FftCtx* fft_create(const Fields& f, char* err, int errlen) {
     hipMemcpy(c->tw32, h32.data(), sizeof(float2) * off, hipMemcpyHostToDevice);
     c->plan.tw64 = c->tw64;
     c->plan.tw32 = c->tw32;
+    if (hipMalloc(&c->d_g, sizeof(GreenDev)) != hipSuccess || hipHostMalloc(&c->h_g, sizeof(GreenDev)) != hipSuccess) {
+        snprintf(err, errlen, "fft_create: allocation of the Green operator constants failed");
+        fft_destroy(c);
+        return nullptr;
+    }
     // rows per x block and kx tile widths (two complex buffers must fit the LDS)
     c->rows = 2;
     while ((long)(f.n2) * f.n3 % c->rows) c->rows /= 2;
@@ -519,6 +542,8 @@ void fft_destroy(FftCtx* c) {
     if (!c) return;
     hipFree(c->tw64);
     hipFree(c->tw32);
+    if (c->d_g) hipFree(c->d_g);
+    if (c->h_g) hipHostFree(c->h_g);
     delete c;
 }
 int fft_rows(const FftCtx* c) { return c->rows; }
@@ -547,15 +572,16 @@ bool fft_reg_supported(const Fields& f);
 // passes st0 .. st1-1 of the register form; returns the last pass' block count
 template <typename R>
 int run_green_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
-                  const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, const double* sgsol,
-                  const FftTune& t, hipStream_t s, int st0, int st1);
+                  const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev* g, int general,
+                  const double* sgsol, const FftTune& t, hipStream_t s, int st0, int st1, hipEvent_t ev0,
+                  hipEvent_t ev1);
 template <typename R>
 int tune_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
-             const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, const double* sgsol, FftTune& t,
-             hipStream_t s);
+             const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev* g, int general, const double* sgsol,
+             FftTune& t, hipStream_t s);
 
 int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
-              int mixed, double scale, const double* sgsol, hipStream_t s) {
+              int mixed, double scale, const double* sgsol, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     GreenDev g;
     for (int i = 0; i < 81; i++) { g.c0[i] = c0[i]; g.s0[i] = s0[i]; }
     g.igamma = igamma;
@@ -563,9 +589,11 @@ int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double*
         const Plan3& pl = c->plan;
         if (mixed)
             return run_green_reg<float>(f, c->tw32 + pl.ax[0].tw_off, c->tw32 + pl.ax[1].tw_off,
-                                        c->tw32 + pl.ax[2].tw_off, sgsum_dev, g, sgsol, c->tune, s, 0, 5);
+                                        c->tw32 + pl.ax[2].tw_off, sgsum_dev, green_dev(c, g, s), igamma != 0, sgsol,
+                                        c->tune, s, 0, 5, ev0, ev1);
         return run_green_reg<double>(f, c->tw64 + pl.ax[0].tw_off, c->tw64 + pl.ax[1].tw_off,
-                                     c->tw64 + pl.ax[2].tw_off, sgsum_dev, g, sgsol, c->tune, s, 0, 5);
+                                     c->tw64 + pl.ax[2].tw_off, sgsum_dev, green_dev(c, g, s), igamma != 0, sgsol,
+                                     c->tune, s, 0, 5, ev0, ev1);
     }
     if (mixed) return run_green<float>(f, c, sgsum_dev, g, scale, s);
     return run_green<double>(f, c, sgsum_dev, g, 1.0, s);
@@ -580,12 +608,13 @@ int fft_tune(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* 
     const Plan3& pl = c->plan;
     if (mixed)
         return tune_reg<float>(f, c->tw32 + pl.ax[0].tw_off, c->tw32 + pl.ax[1].tw_off, c->tw32 + pl.ax[2].tw_off,
-                               sgsum_dev, g, sgsol, c->tune, s);
+                               sgsum_dev, green_dev(c, g, s), igamma != 0, sgsol, c->tune, s);
     return tune_reg<double>(f, c->tw64 + pl.ax[0].tw_off, c->tw64 + pl.ax[1].tw_off, c->tw64 + pl.ax[2].tw_off,
-                            sgsum_dev, g, sgsol, c->tune, s);
+                            sgsum_dev, green_dev(c, g, s), igamma != 0, sgsol, c->tune, s);
 }
 int fft_green_passes(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0,
-                     int igamma, int mixed, const double* sgsol, int st0, int st1, hipStream_t s) {
+                     int igamma, int mixed, const double* sgsol, int st0, int st1, hipStream_t s, hipEvent_t ev0,
+                     hipEvent_t ev1) {
     if (!(c->use_reg && fft_reg_supported(f))) return -1;
     GreenDev g;
     for (int i = 0; i < 81; i++) { g.c0[i] = c0[i]; g.s0[i] = s0[i]; }
@@ -593,9 +622,11 @@ int fft_green_passes(const Fields& f, FftCtx* c, const double* sgsum_dev, const 
     const Plan3& pl = c->plan;
     if (mixed)
         return run_green_reg<float>(f, c->tw32 + pl.ax[0].tw_off, c->tw32 + pl.ax[1].tw_off,
-                                    c->tw32 + pl.ax[2].tw_off, sgsum_dev, g, sgsol, c->tune, s, st0, st1);
+                                    c->tw32 + pl.ax[2].tw_off, sgsum_dev, green_dev(c, g, s), igamma != 0, sgsol,
+                                    c->tune, s, st0, st1, ev0, ev1);
     return run_green_reg<double>(f, c->tw64 + pl.ax[0].tw_off, c->tw64 + pl.ax[1].tw_off,
-                                 c->tw64 + pl.ax[2].tw_off, sgsum_dev, g, sgsol, c->tune, s, st0, st1);
+                                 c->tw64 + pl.ax[2].tw_off, sgsum_dev, green_dev(c, g, s), igamma != 0, sgsol,
+                                 c->tune, s, st0, st1, ev0, ev1);
 }
 int fft_green_batch(const FftWork* w, int nrve, const Fields& f0, int mixed, int general, hipStream_t s);
 void fft_twiddles(const FftCtx* c, int mixed, const void** tw) {
@@ -1069,8 +1100,9 @@ __device__ __forceinline__ void z_blk(const Fields& f, const typename Cx<R>::T* 
 }
 template <int N, typename R, bool GENERAL, bool KT1>
 __global__ __launch_bounds__((ZDims<N, KT1>::ZB)) CG_Z_ATTR void k_z_green_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
-                                                                          GreenDev g, int ntile, int xcd) {
-    z_blk<N, R, GENERAL, KT1>(f, tw, g, ntile, fft_block(xcd));
+                                                                          const GreenDev* __restrict__ g, int ntile,
+                                                                          int xcd) {
+    z_blk<N, R, GENERAL, KT1>(f, tw, *g, ntile, fft_block(xcd));
 }
 template <int N, typename R, bool GENERAL, bool KT1>
 __global__ __launch_bounds__((ZDims<N, KT1>::ZB)) CG_Z_ATTR void k_z_bat(const FftWork* __restrict__ w, int ntile, int nbs) {
@@ -1204,51 +1236,58 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_bat(const FftWork* 
 static bool reg_size(int n) { return n == 16 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128 || n == 256; }
 bool fft_reg_supported(const Fields& f) { return reg_size(f.n1) && reg_size(f.n2) && reg_size(f.n3); }
 
+// e0, e1 (or null): events recorded at the launch's start and end (the stage's timers)
 template <int N, typename R> static void launch_x_fwd_reg(const Fields& f, const typename Cx<R>::T* tw, const double* sgsum,
-                                                          const double* sgsol, int xcd, hipStream_t s) {
+                                                          const double* sgsol, int xcd, hipStream_t s, hipEvent_t e0,
+                                                          hipEvent_t e1) {
     const long nrows = (long)f.n2 * f.n3;
     using X = XDims<N, R>;
     const int nb = (int)((nrows + X::RW - 1) / X::RW);
-    hipLaunchKernelGGL((k_x_fwd_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, f, tw, sgsum, sgsol, xcd);
+    hipExtLaunchKernelGGL((k_x_fwd_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, e0, e1, 0, f, tw, sgsum, sgsol, xcd);
 }
 template <int N, typename R> static int launch_x_inv_reg(const Fields& f, const typename Cx<R>::T* tw, const double* sgsol,
-                                                         int xcd, hipStream_t s) {
+                                                         int xcd, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     const long nrows = (long)f.n2 * f.n3;
     using X = XDims<N, R>;
     const int nb = (int)((nrows + X::RW - 1) / X::RW);
-    hipLaunchKernelGGL((k_x_inv_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, f, tw, sgsol, xcd);
+    hipExtLaunchKernelGGL((k_x_inv_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, e0, e1, 0, f, tw, sgsol, xcd);
     return nb;
 }
 template <int N, bool INV, typename R, int YB>
-static void launch_y_reg_b(const Fields& f, const typename Cx<R>::T* tw, int xcd, hipStream_t s) {
+static void launch_y_reg_b(const Fields& f, const typename Cx<R>::T* tw, int xcd, hipStream_t s, hipEvent_t e0,
+                           hipEvent_t e1) {
     constexpr int NS = YDims<N, YB>::NS;
     const int ntile = (f.h1 * 9 + NS - 1) / NS;
-    hipLaunchKernelGGL((k_y_reg<N, INV, R, YB>), dim3(f.n3 * ntile), dim3(YB), 0, s, f, tw, ntile, xcd);
+    hipExtLaunchKernelGGL((k_y_reg<N, INV, R, YB>), dim3(f.n3 * ntile), dim3(YB), 0, s, e0, e1, 0, f, tw, ntile, xcd);
 }
 template <int N, bool INV, typename R>
-static void launch_y_reg(const Fields& f, const typename Cx<R>::T* tw, int half, int xcd, hipStream_t s) {
-    if (half && YBS / 2 >= 2 * Shape<N>::T) launch_y_reg_b<N, INV, R, YBS / 2>(f, tw, xcd, s);
-    else launch_y_reg_b<N, INV, R, YBS>(f, tw, xcd, s);
+static void launch_y_reg(const Fields& f, const typename Cx<R>::T* tw, int half, int xcd, hipStream_t s, hipEvent_t e0,
+                         hipEvent_t e1) {
+    if (half && YBS / 2 >= 2 * Shape<N>::T) launch_y_reg_b<N, INV, R, YBS / 2>(f, tw, xcd, s, e0, e1);
+    else launch_y_reg_b<N, INV, R, YBS>(f, tw, xcd, s, e0, e1);
 }
 template <int N, typename R, bool KT1>
-static void launch_z_reg_b(const Fields& f, const typename Cx<R>::T* tw, const GreenDev& g, int xcd, hipStream_t s) {
+static void launch_z_reg_b(const Fields& f, const typename Cx<R>::T* tw, const GreenDev* g, int general, int xcd,
+                           hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     using Z = ZDims<N, KT1>;
     const int ntile = (f.h1 + Z::KT - 1) / Z::KT;
-    if (g.igamma == 0)
-        hipLaunchKernelGGL((k_z_green_reg<N, R, false, KT1>), dim3(f.n2 * ntile), dim3(Z::ZB), 0, s, f, tw, g, ntile, xcd);
+    if (!general)
+        hipExtLaunchKernelGGL((k_z_green_reg<N, R, false, KT1>), dim3(f.n2 * ntile), dim3(Z::ZB), 0, s, e0, e1, 0, f, tw, g,
+                              ntile, xcd);
     else
-        hipLaunchKernelGGL((k_z_green_reg<N, R, true, KT1>), dim3(f.n2 * ntile), dim3(Z::ZB), 0, s, f, tw, g, ntile, xcd);
+        hipExtLaunchKernelGGL((k_z_green_reg<N, R, true, KT1>), dim3(f.n2 * ntile), dim3(Z::ZB), 0, s, e0, e1, 0, f, tw, g,
+                              ntile, xcd);
 }
 template <int N, typename R>
-static void launch_z_reg(const Fields& f, const typename Cx<R>::T* tw, const GreenDev& g, int kt1, int xcd,
-                         hipStream_t s) {
+static void launch_z_reg(const Fields& f, const typename Cx<R>::T* tw, const GreenDev* g, int general, int kt1, int xcd,
+                         hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if constexpr (RegDims<N>::KT > 1) {
         if (kt1) {
-            launch_z_reg_b<N, R, true>(f, tw, g, xcd, s);
+            launch_z_reg_b<N, R, true>(f, tw, g, general, xcd, s, e0, e1);
             return;
         }
     }
-    launch_z_reg_b<N, R, false>(f, tw, g, xcd, s);
+    launch_z_reg_b<N, R, false>(f, tw, g, general, xcd, s, e0, e1);
 }
 
 #define CG_SWITCH_N(n, CALL)                                             \
@@ -1267,14 +1306,15 @@ static void launch_z_reg(const Fields& f, const typename Cx<R>::T* tw, const Gre
 // form with the block shape and placement of t; returns the x inverse's block count.
 template <typename R>
 static int reg_stage(int stage, const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
-                     const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, const double* sgsol,
-                     const FftTune& t, hipStream_t s) {
+                     const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev* g, int general,
+                     const double* sgsol, const FftTune& t, hipStream_t s, hipEvent_t e0 = nullptr,
+                     hipEvent_t e1 = nullptr) {
     int nb = -1;
-#define CG_XF(N) launch_x_fwd_reg<N, R>(f, tw1, sgsum, sgsol, t.xcd[0], s)
-#define CG_YF(N) launch_y_reg<N, false, R>(f, tw2, t.yhalf[0], t.xcd[1], s)
-#define CG_Z(N) launch_z_reg<N, R>(f, tw3, g, t.kt1, t.xcd[2], s)
-#define CG_YI(N) launch_y_reg<N, true, R>(f, tw2, t.yhalf[1], t.xcd[3], s)
-#define CG_XI(N) nb = launch_x_inv_reg<N, R>(f, tw1, sgsol, t.xcd[4], s)
+#define CG_XF(N) launch_x_fwd_reg<N, R>(f, tw1, sgsum, sgsol, t.xcd[0], s, e0, e1)
+#define CG_YF(N) launch_y_reg<N, false, R>(f, tw2, t.yhalf[0], t.xcd[1], s, e0, e1)
+#define CG_Z(N) launch_z_reg<N, R>(f, tw3, g, general, t.kt1, t.xcd[2], s, e0, e1)
+#define CG_YI(N) launch_y_reg<N, true, R>(f, tw2, t.yhalf[1], t.xcd[3], s, e0, e1)
+#define CG_XI(N) nb = launch_x_inv_reg<N, R>(f, tw1, sgsol, t.xcd[4], s, e0, e1)
     switch (stage) {
         case 0: CG_SWITCH_N(f.n1, CG_XF) break;
         case 1: CG_SWITCH_N(f.n2, CG_YF) break;
@@ -1342,10 +1382,13 @@ int fft_green_batch(const FftWork* w, int nrve, const Fields& f0, int mixed, int
 
 template <typename R>
 int run_green_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
-                  const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, const double* sgsol,
-                  const FftTune& t, hipStream_t s, int st0, int st1) {
+                  const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev* g, int general,
+                  const double* sgsol, const FftTune& t, hipStream_t s, int st0, int st1, hipEvent_t ev0,
+                  hipEvent_t ev1) {
     int nb = -1;
-    for (int st = st0; st < st1; st++) nb = reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, sgsol, t, s);
+    for (int st = st0; st < st1; st++)
+        nb = reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, general, sgsol, t, s, st == st0 ? ev0 : nullptr,
+                          st == st1 - 1 ? ev1 : nullptr);
     return nb;
 }
 
@@ -1355,8 +1398,8 @@ int run_green_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename 
 // candidate computes the same values in the same order, so the choice changes time only.
 template <typename R>
 int tune_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
-                    const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev& g, const double* sgsol, FftTune& t,
-                    hipStream_t s) {
+             const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev* g, int general, const double* sgsol,
+             FftTune& t, hipStream_t s) {
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return -1;
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -1375,9 +1418,9 @@ int tune_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>
                 if (st == 1) c.yhalf[0] = sh;
                 if (st == 3) c.yhalf[1] = sh;
                 if (st == 2) c.kt1 = sh;
-                for (int w = 0; w < 2; w++) reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, sgsol, c, s);
+                for (int w = 0; w < 2; w++) reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, general, sgsol, c, s);
                 hipEventRecord(e0, s);
-                for (int w = 0; w < 5; w++) reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, sgsol, c, s);
+                for (int w = 0; w < 5; w++) reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, general, sgsol, c, s);
                 hipEventRecord(e1, s);
                 float ms = 0.0f;
                 if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
@@ -1397,12 +1440,14 @@ int tune_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>
     return rc;
 }
 template int run_green_reg<double>(const Fields&, const double2*, const double2*, const double2*, const double*,
-                                   const GreenDev&, const double*, const FftTune&, hipStream_t, int, int);
+                                   const GreenDev*, int, const double*, const FftTune&, hipStream_t, int, int, hipEvent_t,
+                                   hipEvent_t);
 template int run_green_reg<float>(const Fields&, const float2*, const float2*, const float2*, const double*,
-                                  const GreenDev&, const double*, const FftTune&, hipStream_t, int, int);
+                                  const GreenDev*, int, const double*, const FftTune&, hipStream_t, int, int, hipEvent_t,
+                                   hipEvent_t);
 template int tune_reg<double>(const Fields&, const double2*, const double2*, const double2*, const double*,
-                              const GreenDev&, const double*, FftTune&, hipStream_t);
+                              const GreenDev*, int, const double*, FftTune&, hipStream_t);
 template int tune_reg<float>(const Fields&, const float2*, const float2*, const float2*, const double*,
-                             const GreenDev&, const double*, FftTune&, hipStream_t);
+                             const GreenDev*, int, const double*, FftTune&, hipStream_t);
 
 }  // namespace cg

@@ -200,12 +200,16 @@ void fft_destroy(FftCtx* c);
 // scale: the mixed form's stress scale (1 in FP64), used by the LDS kernels; the register
 // kernels take it from the previous pass' solid stress sums in the device array sgsol (9) as
 // max(von Mises, 1), so a captured stage stays valid when the scale changes.
+// ev0, ev1 (or null): events recorded at the start of the first and the end of the last register
+// pass (the stage's timers, no marker packets between the kernels)
 int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
-              int mixed, double scale, const double* sgsol, hipStream_t s);
+              int mixed, double scale, const double* sgsol, hipStream_t s, hipEvent_t ev0 = nullptr,
+              hipEvent_t ev1 = nullptr);
 // Passes st0 .. st1-1 of the register form (0 x forward, 1 y forward, 2 z + Gamma, 3 y inverse,
 // 4 x inverse); returns the last pass' block count.  Only for grids with fft_tunable().
 int fft_green_passes(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0,
-                     int igamma, int mixed, const double* sgsol, int st0, int st1, hipStream_t s);
+                     int igamma, int mixed, const double* sgsol, int st0, int st1, hipStream_t s,
+                     hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // Times the candidate shapes of each register pass on stream s (the passes overwrite the
 // spectrum, ΔL^ref and the block partials, which the next fft_green recomputes) and keeps the
 // fastest; 0 on success or when the lengths use the LDS kernels.
