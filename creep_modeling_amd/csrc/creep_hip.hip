@@ -172,9 +172,11 @@ struct cg_ctx {
     int idispini = 0;
     std::vector<double> dispini;
     // the mixed form's stress scale as the register FFT kernels read it: {1/scale, scale/V}
-    // the previous pass' solid stress sums (9), from which the register FFT kernels take the
-    // mixed form's stress scale (zero before the first pass: scale 1)
-    double* d_sgsol = nullptr;
+    // the mixed form's stress scale {1/s, s/V} as the register FFT kernels read it (the pass sums
+    // write it: s = max(svm, 1) of the pass' solid stress sums; 1 before the first pass) and the
+    // pass sums' last-block counter
+    double* d_scale = nullptr;
+    unsigned* d_cnt = nullptr;
     // HIP graphs of a pass' fixed device work: pre = FFT + Gamma stage and the sums of
     // w dLref F^-1; post = the solve's sums, <sigma> for the next pass and their copy to the host.
     // pre is captured again when C0 / S0, igamma or the FFT shapes change.  Off by default: the
@@ -367,7 +369,8 @@ int cg_destroy(cg_ctx* c) {
     if (c->d_fluct) hipFree(c->d_fluct);
     if (c->h_red) hipHostFree(c->h_red);
     if (c->h_pass) hipHostFree(c->h_pass);
-    if (c->d_sgsol) hipFree(c->d_sgsol);
+    if (c->d_scale) hipFree(c->d_scale);
+    if (c->d_cnt) hipFree(c->d_cnt);
     if (c->g_pre) hipGraphExecDestroy(c->g_pre);
     if (c->g_post) hipGraphExecDestroy(c->g_post);
     if (c->fft) fft_destroy(c->fft);
@@ -536,8 +539,13 @@ int cg_init(cg_ctx* c, const cg_process* p1) {
     HIPCHK(c, hipHostMalloc(&c->h_red, 256 * sizeof(double)));
     HIPCHK(c, hipHostMalloc(&c->h_pass, R_NCOPY * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
     HIPCHK(c, hipHostGetDevicePointer((void**)&c->h_pass_dev, c->h_pass, 0));
-    if (dalloc(c, &c->d_sgsum, 16) || dalloc(c, &c->d_fluct, 16) || dalloc(c, &c->d_sgsol, 16)) return -3;
-    HIPCHK(c, hipMemset(c->d_sgsol, 0, 16 * sizeof(double)));
+    if (dalloc(c, &c->d_sgsum, 16) || dalloc(c, &c->d_fluct, 16) || dalloc(c, &c->d_scale, 2)) return -3;
+    {
+        const double sc0[2] = {1.0, 1.0 / (double)V};
+        HIPCHK(c, hipMemcpy(c->d_scale, sc0, sizeof sc0, hipMemcpyHostToDevice));
+    }
+    HIPCHK(c, hipMalloc(&c->d_cnt, sizeof(unsigned)));
+    HIPCHK(c, hipMemset(c->d_cnt, 0, sizeof(unsigned)));
     c->fft = fft_create(f, c->err, sizeof c->err);
     if (!c->fft) return -5;
 
@@ -750,8 +758,8 @@ static void tune_write(const std::string& path, const std::string& key, const Ff
             t.yhalf[0], t.yhalf[1], t.kt1);
     fclose(fp);
 }
-// the mixed form's stress scale for the LDS FFT kernels (the register kernels derive the same
-// max(svm, 1) from d_sgsol on the device)
+// the mixed form's stress scale for the LDS FFT kernels (the pass sums write the same max(svm, 1)
+// into d_scale for the register kernels)
 static double pass_scale(const cg_ctx* c) { return c->mixed ? std::max(c->svmavg, 1.0) : 1.0; }
 // Captures what body() enqueues on the context's stream into an executable graph.
 template <class Body>
@@ -789,7 +797,7 @@ static int ensure_fft_tuned(cg_ctx* c, const cg_process* p) {
         launch_sigma_sum(c->f, c->s, &nb);
         launch_reduce_final(c->f.part, nb, 9, c->d_sgsum, c->s);
     }
-    if (fft_tune(c->f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, c->d_sgsol, c->s) != 0)
+    if (fft_tune(c->f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, c->d_scale, c->s) != 0)
         return fail(c, -2, "FFT shape tuning failed");
     c->tune_version++;
     if (!c->tune_path.empty()) tune_write(c->tune_path, key, fft_get_tune(c->fft));
@@ -834,13 +842,13 @@ static int pass_begin(cg_ctx* c, cg_process* p, cg_iter_state* it, SolveArgs& sa
         int n;
         hipEvent_t e0 = attach ? c->ev[0] : nullptr, e1 = attach ? c->ev[1] : nullptr;
         if (early)
-            n = fft_green_passes(f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, c->d_sgsol, 4, 5, c->s, e0,
+            n = fft_green_passes(f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, c->d_scale, 4, 5, c->s, e0,
                                  e1);
         else
-            n = fft_green(f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, scale, c->d_sgsol, c->s, e0, e1);
+            n = fft_green(f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, scale, c->d_scale, c->s, e0, e1);
         launch_reduce_final_cm(f.part, n, 9, c->d_fluct, c->s);
     };
-    // the register kernels take the scale from d_sgsol, so their stage can be replayed as a graph
+    // the register kernels take the scale from d_scale, so their stage can be replayed as a graph
     if (!run_stage) {
     } else if (c->use_graphs && fft_tunable(f, c->fft) && !early) {
         if (!c->g_pre || c->g_pre_version != c->green_version || c->g_pre_igamma != p->igamma ||
@@ -888,7 +896,7 @@ static int pass_reduce(cg_ctx* c) {
     // at the end of the last kernel
     auto sums = [&](hipEvent_t done) {
         launch_pass_sums(f.part, solve_blocks(c->V), R_NSOLVE, R_NCOPY, f.red, c->h_pass_dev, c->d_sgsum,
-                         c->d_sgsol, c->s, done);
+                         c->mixed ? c->d_scale : nullptr, c->d_cnt, c->V, c->s, done);
     };
     if (!c->use_graphs) {
         tstart(c, 2);
@@ -1018,7 +1026,7 @@ int cg_iterate(cg_ctx* c, cg_process* p, cg_iter_state* it, cg_iter_out* out) {
         const int slot = used_slot == 0 ? 1 : 0;
         const bool t1 = c->timing == 1;
         c->early_timed[slot] = t1 ? 1 : 0;
-        fft_green_passes(c->f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, c->d_sgsol, 0, 4, c->s,
+        fft_green_passes(c->f, c->fft, c->d_sgsum, c->c0, c->s0, p->igamma, c->mixed, c->d_scale, 0, 4, c->s,
                          t1 ? c->ev[8 + 2 * slot] : nullptr, t1 ? c->ev[9 + 2 * slot] : nullptr);
         c->early = 1;
         c->early_slot = slot;
@@ -1205,7 +1213,8 @@ int cg_batch_launch(cg_batch* b, cg_process* const* p, cg_iter_state* st, cg_ite
         w.f = c->f;
         fft_twiddles(c->fft, c->mixed, w.tw);
         w.sgsum = c->d_sgsum;
-        w.sgsol = c->d_sgsol;
+        w.dscale = c->mixed ? c->d_scale : nullptr;
+        w.counter = c->d_cnt;
         w.fluct = c->d_fluct;
         w.host = c->h_pass_dev;
         memcpy(w.g.c0, c->c0, sizeof w.g.c0);

@@ -573,15 +573,15 @@ bool fft_reg_supported(const Fields& f);
 template <typename R>
 int run_green_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
                   const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev* g, int general,
-                  const double* sgsol, const FftTune& t, hipStream_t s, int st0, int st1, hipEvent_t ev0,
+                  const double* dscale, const FftTune& t, hipStream_t s, int st0, int st1, hipEvent_t ev0,
                   hipEvent_t ev1);
 template <typename R>
 int tune_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
-             const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev* g, int general, const double* sgsol,
+             const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev* g, int general, const double* dscale,
              FftTune& t, hipStream_t s);
 
 int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
-              int mixed, double scale, const double* sgsol, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+              int mixed, double scale, const double* dscale, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     GreenDev g;
     for (int i = 0; i < 81; i++) { g.c0[i] = c0[i]; g.s0[i] = s0[i]; }
     g.igamma = igamma;
@@ -589,10 +589,10 @@ int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double*
         const Plan3& pl = c->plan;
         if (mixed)
             return run_green_reg<float>(f, c->tw32 + pl.ax[0].tw_off, c->tw32 + pl.ax[1].tw_off,
-                                        c->tw32 + pl.ax[2].tw_off, sgsum_dev, green_dev(c, g, s), igamma != 0, sgsol,
+                                        c->tw32 + pl.ax[2].tw_off, sgsum_dev, green_dev(c, g, s), igamma != 0, dscale,
                                         c->tune, s, 0, 5, ev0, ev1);
         return run_green_reg<double>(f, c->tw64 + pl.ax[0].tw_off, c->tw64 + pl.ax[1].tw_off,
-                                     c->tw64 + pl.ax[2].tw_off, sgsum_dev, green_dev(c, g, s), igamma != 0, sgsol,
+                                     c->tw64 + pl.ax[2].tw_off, sgsum_dev, green_dev(c, g, s), igamma != 0, dscale,
                                      c->tune, s, 0, 5, ev0, ev1);
     }
     if (mixed) return run_green<float>(f, c, sgsum_dev, g, scale, s);
@@ -600,7 +600,7 @@ int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double*
 }
 
 int fft_tune(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
-             int mixed, const double* sgsol, hipStream_t s) {
+             int mixed, const double* dscale, hipStream_t s) {
     if (!(c->use_reg && fft_reg_supported(f))) return 0;  // the LDS kernels have no shapes to pick
     GreenDev g;
     for (int i = 0; i < 81; i++) { g.c0[i] = c0[i]; g.s0[i] = s0[i]; }
@@ -608,12 +608,12 @@ int fft_tune(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* 
     const Plan3& pl = c->plan;
     if (mixed)
         return tune_reg<float>(f, c->tw32 + pl.ax[0].tw_off, c->tw32 + pl.ax[1].tw_off, c->tw32 + pl.ax[2].tw_off,
-                               sgsum_dev, green_dev(c, g, s), igamma != 0, sgsol, c->tune, s);
+                               sgsum_dev, green_dev(c, g, s), igamma != 0, dscale, c->tune, s);
     return tune_reg<double>(f, c->tw64 + pl.ax[0].tw_off, c->tw64 + pl.ax[1].tw_off, c->tw64 + pl.ax[2].tw_off,
-                            sgsum_dev, green_dev(c, g, s), igamma != 0, sgsol, c->tune, s);
+                            sgsum_dev, green_dev(c, g, s), igamma != 0, dscale, c->tune, s);
 }
 int fft_green_passes(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0,
-                     int igamma, int mixed, const double* sgsol, int st0, int st1, hipStream_t s, hipEvent_t ev0,
+                     int igamma, int mixed, const double* dscale, int st0, int st1, hipStream_t s, hipEvent_t ev0,
                      hipEvent_t ev1) {
     if (!(c->use_reg && fft_reg_supported(f))) return -1;
     GreenDev g;
@@ -622,10 +622,10 @@ int fft_green_passes(const Fields& f, FftCtx* c, const double* sgsum_dev, const 
     const Plan3& pl = c->plan;
     if (mixed)
         return run_green_reg<float>(f, c->tw32 + pl.ax[0].tw_off, c->tw32 + pl.ax[1].tw_off,
-                                    c->tw32 + pl.ax[2].tw_off, sgsum_dev, green_dev(c, g, s), igamma != 0, sgsol,
+                                    c->tw32 + pl.ax[2].tw_off, sgsum_dev, green_dev(c, g, s), igamma != 0, dscale,
                                     c->tune, s, st0, st1, ev0, ev1);
     return run_green_reg<double>(f, c->tw64 + pl.ax[0].tw_off, c->tw64 + pl.ax[1].tw_off,
-                                 c->tw64 + pl.ax[2].tw_off, sgsum_dev, green_dev(c, g, s), igamma != 0, sgsol,
+                                 c->tw64 + pl.ax[2].tw_off, sgsum_dev, green_dev(c, g, s), igamma != 0, dscale,
                                  c->tune, s, st0, st1, ev0, ev1);
 }
 int fft_green_batch(const FftWork* w, int nrve, const Fields& f0, int mixed, int general, hipStream_t s);
@@ -899,9 +899,9 @@ template <typename R> __device__ __forceinline__ typename Cx<R>::T* spec_of(cons
 // separated with X[N-k] afterwards.
 template <int N, typename R>
 __device__ __forceinline__ void x_fwd_blk(const Fields& f, const typename Cx<R>::T* __restrict__ tw,
-                                          const double* __restrict__ sgsum, const double* __restrict__ sgsol, int lb) {
+                                          const double* __restrict__ sgsum, const double* __restrict__ dscale, int lb) {
     // 1 / the mixed form's stress scale (1 in FP64)
-    const double inv_scale = sizeof(R) == 8 ? 1.0 : 1.0 / mixed_scale(sgsol);
+    const double inv_scale = sizeof(R) == 8 ? 1.0 : dscale[0];
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
     using X = XDims<N, R>;
@@ -988,8 +988,8 @@ __device__ __forceinline__ void x_fwd_blk(const Fields& f, const typename Cx<R>:
 template <int N, typename R>
 __global__ __launch_bounds__((XDims<N, R>::XB)) CG_XF_ATTR void k_x_fwd_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
                                                                           const double* __restrict__ sgsum,
-                                                                          const double* __restrict__ sgsol, int xcd) {
-    x_fwd_blk<N, R>(f, tw, sgsum, sgsol, fft_block(xcd));
+                                                                          const double* __restrict__ dscale, int xcd) {
+    x_fwd_blk<N, R>(f, tw, sgsum, dscale, fft_block(xcd));
 }
 template <int N, typename R>
 __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_bat(const FftWork* __restrict__ w, int nbs) {
@@ -997,7 +997,7 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_fwd_bat(const FftWork* 
     const int lb = entry_block(nbs, 0, &r);
     const FftWork& e = w[r];
     const Fields f = e.f;
-    x_fwd_blk<N, R>(f, (const typename Cx<R>::T*)e.tw[0], e.sgsum, e.sgsol, lb);
+    x_fwd_blk<N, R>(f, (const typename Cx<R>::T*)e.tw[0], e.sgsum, e.dscale, lb);
 }
 
 // y pass over every (kx, c) column of one z plane.  With two level-2 rounds per thread
@@ -1118,8 +1118,8 @@ __global__ __launch_bounds__((ZDims<N, KT1>::ZB)) CG_Z_ATTR void k_z_bat(const F
 // c2c inverse does); block partial sums of w dLref F^-1.
 template <int N, typename R>
 __device__ __forceinline__ void x_inv_blk(const Fields& f, const typename Cx<R>::T* __restrict__ tw,
-                                          const double* __restrict__ sgsol, int lb, int nbs) {
-    const double out_scale = (sizeof(R) == 8 ? 1.0 : mixed_scale(sgsol)) / (double)f.V;  // stress scale / V
+                                          const double* __restrict__ dscale, int lb, int nbs) {
+    const double out_scale = sizeof(R) == 8 ? 1.0 / (double)f.V : dscale[1];  // stress scale / V
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
     using X = XDims<N, R>;
@@ -1221,8 +1221,8 @@ __device__ __forceinline__ void x_inv_blk(const Fields& f, const typename Cx<R>:
 }
 template <int N, typename R>
 __global__ __launch_bounds__((XDims<N, R>::XB)) CG_XI_ATTR void k_x_inv_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
-                                                                          const double* __restrict__ sgsol, int xcd) {
-    x_inv_blk<N, R>(f, tw, sgsol, fft_block(xcd), gridDim.x);
+                                                                          const double* __restrict__ dscale, int xcd) {
+    x_inv_blk<N, R>(f, tw, dscale, fft_block(xcd), gridDim.x);
 }
 template <int N, typename R>
 __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_bat(const FftWork* __restrict__ w, int nbs) {
@@ -1230,7 +1230,7 @@ __global__ __launch_bounds__((XDims<N, R>::XB)) void k_x_inv_bat(const FftWork* 
     const int lb = entry_block(nbs, 0, &r);
     const FftWork& e = w[r];
     const Fields f = e.f;
-    x_inv_blk<N, R>(f, (const typename Cx<R>::T*)e.tw[0], e.sgsol, lb, nbs);
+    x_inv_blk<N, R>(f, (const typename Cx<R>::T*)e.tw[0], e.dscale, lb, nbs);
 }
 
 static bool reg_size(int n) { return n == 16 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128 || n == 256; }
@@ -1238,19 +1238,19 @@ bool fft_reg_supported(const Fields& f) { return reg_size(f.n1) && reg_size(f.n2
 
 // e0, e1 (or null): events recorded at the launch's start and end (the stage's timers)
 template <int N, typename R> static void launch_x_fwd_reg(const Fields& f, const typename Cx<R>::T* tw, const double* sgsum,
-                                                          const double* sgsol, int xcd, hipStream_t s, hipEvent_t e0,
+                                                          const double* dscale, int xcd, hipStream_t s, hipEvent_t e0,
                                                           hipEvent_t e1) {
     const long nrows = (long)f.n2 * f.n3;
     using X = XDims<N, R>;
     const int nb = (int)((nrows + X::RW - 1) / X::RW);
-    hipExtLaunchKernelGGL((k_x_fwd_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, e0, e1, 0, f, tw, sgsum, sgsol, xcd);
+    hipExtLaunchKernelGGL((k_x_fwd_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, e0, e1, 0, f, tw, sgsum, dscale, xcd);
 }
-template <int N, typename R> static int launch_x_inv_reg(const Fields& f, const typename Cx<R>::T* tw, const double* sgsol,
+template <int N, typename R> static int launch_x_inv_reg(const Fields& f, const typename Cx<R>::T* tw, const double* dscale,
                                                          int xcd, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     const long nrows = (long)f.n2 * f.n3;
     using X = XDims<N, R>;
     const int nb = (int)((nrows + X::RW - 1) / X::RW);
-    hipExtLaunchKernelGGL((k_x_inv_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, e0, e1, 0, f, tw, sgsol, xcd);
+    hipExtLaunchKernelGGL((k_x_inv_reg<N, R>), dim3(nb), dim3(X::XB), 0, s, e0, e1, 0, f, tw, dscale, xcd);
     return nb;
 }
 template <int N, bool INV, typename R, int YB>
@@ -1307,14 +1307,14 @@ static void launch_z_reg(const Fields& f, const typename Cx<R>::T* tw, const Gre
 template <typename R>
 static int reg_stage(int stage, const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
                      const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev* g, int general,
-                     const double* sgsol, const FftTune& t, hipStream_t s, hipEvent_t e0 = nullptr,
+                     const double* dscale, const FftTune& t, hipStream_t s, hipEvent_t e0 = nullptr,
                      hipEvent_t e1 = nullptr) {
     int nb = -1;
-#define CG_XF(N) launch_x_fwd_reg<N, R>(f, tw1, sgsum, sgsol, t.xcd[0], s, e0, e1)
+#define CG_XF(N) launch_x_fwd_reg<N, R>(f, tw1, sgsum, dscale, t.xcd[0], s, e0, e1)
 #define CG_YF(N) launch_y_reg<N, false, R>(f, tw2, t.yhalf[0], t.xcd[1], s, e0, e1)
 #define CG_Z(N) launch_z_reg<N, R>(f, tw3, g, general, t.kt1, t.xcd[2], s, e0, e1)
 #define CG_YI(N) launch_y_reg<N, true, R>(f, tw2, t.yhalf[1], t.xcd[3], s, e0, e1)
-#define CG_XI(N) nb = launch_x_inv_reg<N, R>(f, tw1, sgsol, t.xcd[4], s, e0, e1)
+#define CG_XI(N) nb = launch_x_inv_reg<N, R>(f, tw1, dscale, t.xcd[4], s, e0, e1)
     switch (stage) {
         case 0: CG_SWITCH_N(f.n1, CG_XF) break;
         case 1: CG_SWITCH_N(f.n2, CG_YF) break;
@@ -1383,11 +1383,11 @@ int fft_green_batch(const FftWork* w, int nrve, const Fields& f0, int mixed, int
 template <typename R>
 int run_green_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
                   const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev* g, int general,
-                  const double* sgsol, const FftTune& t, hipStream_t s, int st0, int st1, hipEvent_t ev0,
+                  const double* dscale, const FftTune& t, hipStream_t s, int st0, int st1, hipEvent_t ev0,
                   hipEvent_t ev1) {
     int nb = -1;
     for (int st = st0; st < st1; st++)
-        nb = reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, general, sgsol, t, s, st == st0 ? ev0 : nullptr,
+        nb = reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, general, dscale, t, s, st == st0 ? ev0 : nullptr,
                           st == st1 - 1 ? ev1 : nullptr);
     return nb;
 }
@@ -1398,7 +1398,7 @@ int run_green_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename 
 // candidate computes the same values in the same order, so the choice changes time only.
 template <typename R>
 int tune_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>::T* tw2,
-             const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev* g, int general, const double* sgsol,
+             const typename Cx<R>::T* tw3, const double* sgsum, const GreenDev* g, int general, const double* dscale,
              FftTune& t, hipStream_t s) {
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return -1;
@@ -1418,9 +1418,9 @@ int tune_reg(const Fields& f, const typename Cx<R>::T* tw1, const typename Cx<R>
                 if (st == 1) c.yhalf[0] = sh;
                 if (st == 3) c.yhalf[1] = sh;
                 if (st == 2) c.kt1 = sh;
-                for (int w = 0; w < 2; w++) reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, general, sgsol, c, s);
+                for (int w = 0; w < 2; w++) reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, general, dscale, c, s);
                 hipEventRecord(e0, s);
-                for (int w = 0; w < 5; w++) reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, general, sgsol, c, s);
+                for (int w = 0; w < 5; w++) reg_stage<R>(st, f, tw1, tw2, tw3, sgsum, g, general, dscale, c, s);
                 hipEventRecord(e1, s);
                 float ms = 0.0f;
                 if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {

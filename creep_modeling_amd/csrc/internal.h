@@ -109,10 +109,11 @@ void launch_correct_solve_full(const Fields& f, const PhaseTables& pt, const Pas
                                hipStream_t s, int* nb, int int_pow, int sym_tau, int small_ns,
                                hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // a pass' R_NSOLVE sums: red, the host's copy of red[0, ncopy) (host-mapped memory), <sigma> for
-// the next pass and the solid stress sums of the mixed-form scale (k_pass_final); done (or
-// null): recorded at the end of the last kernel
+// the next pass and, with dscale, the next pass' mixed-form stress scale {1/s, s/V}, s = max(von
+// Mises of the solid stress sum, 1), written by the block that finishes last (counter: zero
+// between calls); done (or null): recorded at the end of the last kernel
 void launch_pass_sums(double* part, int nblocks, int nvals, int ncopy, double* red, double* host, double* sgsum,
-                      double* sgsol, hipStream_t s, hipEvent_t done);
+                      double* dscale, unsigned* counter, long V, hipStream_t s, hipEvent_t done);
 // the correction + constitutive kernel over several RVEs: block b belongs to RVE blk_rve[b]
 void launch_correct_solve_batch(const SolveWork* work, const int* blk_rve, int nblocks, int modal, int int_pow,
                                 int sym_tau, int small_ns, hipStream_t s);
@@ -169,7 +170,8 @@ struct FftWork {
     Fields f;
     const void* tw[3];   // twiddles of the x, y, z transforms (the group's precision)
     const double* sgsum; // <sigma> (9)
-    const double* sgsol; // the previous pass' solid stress sums (9), for the mixed-form scale
+    double* dscale;      // the mixed form's stress scale {1/s, s/V} (written by the pass sums)
+    unsigned* counter;   // the pass sums' last-block counter (zero between passes)
     double* fluct;       // 9 sums of w dLref F^-1
     double* host;        // the pass' sums, host-mapped
     GreenDev g;
@@ -198,23 +200,23 @@ void fft_destroy(FftCtx* c);
 // PK1 -> Gamma convolution -> dLref; writes the 9 block partial sums of
 // w dLref F^-1 to f.part, column-major (part[i * nblocks + b]), and returns the number of blocks.
 // scale: the mixed form's stress scale (1 in FP64), used by the LDS kernels; the register
-// kernels take it from the previous pass' solid stress sums in the device array sgsol (9) as
-// max(von Mises, 1), so a captured stage stays valid when the scale changes.
+// kernels read {1/scale, scale/V} from the device array dscale (the pass sums write it), so a
+// captured stage stays valid when the scale changes.
 // ev0, ev1 (or null): events recorded at the start of the first and the end of the last register
 // pass (the stage's timers, no marker packets between the kernels)
 int fft_green(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
-              int mixed, double scale, const double* sgsol, hipStream_t s, hipEvent_t ev0 = nullptr,
+              int mixed, double scale, const double* dscale, hipStream_t s, hipEvent_t ev0 = nullptr,
               hipEvent_t ev1 = nullptr);
 // Passes st0 .. st1-1 of the register form (0 x forward, 1 y forward, 2 z + Gamma, 3 y inverse,
 // 4 x inverse); returns the last pass' block count.  Only for grids with fft_tunable().
 int fft_green_passes(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0,
-                     int igamma, int mixed, const double* sgsol, int st0, int st1, hipStream_t s,
+                     int igamma, int mixed, const double* dscale, int st0, int st1, hipStream_t s,
                      hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // Times the candidate shapes of each register pass on stream s (the passes overwrite the
 // spectrum, ΔL^ref and the block partials, which the next fft_green recomputes) and keeps the
 // fastest; 0 on success or when the lengths use the LDS kernels.
 int fft_tune(const Fields& f, FftCtx* c, const double* sgsum_dev, const double* c0, const double* s0, int igamma,
-             int mixed, const double* sgsol, hipStream_t s);
+             int mixed, const double* dscale, hipStream_t s);
 FftTune fft_get_tune(const FftCtx* c);
 // the register passes' twiddle tables (x, y, z) of a context, for its FftWork entry
 void fft_twiddles(const FftCtx* c, int mixed, const void** tw);

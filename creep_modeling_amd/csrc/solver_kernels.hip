@@ -178,8 +178,10 @@ struct PassFinal {
     int n, nvals, ncopy;
     double* red;
     double* host;   // host-mapped, fine-grained
-    double* sgsum;  // 9
-    double* sgsol;  // 9
+    double* sgsum;   // 9
+    double* dscale;  // 2, or null
+    unsigned* counter;
+    long V;
 };
 __device__ __forceinline__ void pass_final_blk(const PassFinal& a, int i) {
     double s = 0.0;
@@ -195,18 +197,34 @@ __device__ __forceinline__ void pass_final_blk(const PassFinal& a, int i) {
         a.red[i] = t;
         a.host[i] = t;
         if (i >= R_SGALL && i < R_SGALL + 9) a.sgsum[i - R_SGALL] = t;
-        if (i >= R_SGSOL && i < R_SGSOL + 9) a.sgsol[i - R_SGSOL] = t;
     }
     if (i == 0)
         for (int k = a.nvals + threadIdx.x; k < a.ncopy; k += BS) a.host[k] = a.red[k];
+    if (a.dscale) {  // the next pass' mixed-form scale, by the block that finishes last
+        __shared__ int last;
+        if (threadIdx.x == 0) {
+            __threadfence();
+            last = atomicAdd(a.counter, 1u) == (unsigned)(a.nvals - 1);
+        }
+        __syncthreads();
+        if (last && threadIdx.x == 0) {
+            __threadfence();
+            double sg[9];
+            for (int k = 0; k < 9; k++) sg[k] = __builtin_nontemporal_load(a.red + R_SGSOL + k);
+            const double sc = mixed_scale(sg);
+            a.dscale[0] = 1.0 / sc;
+            a.dscale[1] = sc / (double)a.V;
+            *a.counter = 0u;
+        }
+    }
     __threadfence_system();
 }
 __global__ __launch_bounds__(BS) void k_pass_final(PassFinal a) { pass_final_blk(a, blockIdx.x); }
 __global__ __launch_bounds__(BS) void k_pass_final_bat(const FftWork* __restrict__ w, int nblocks, int nvals,
                                                        int ncopy, int nch) {
     const FftWork& e = w[blockIdx.y];
-    PassFinal a{e.f.part, nblocks, nvals, ncopy, e.f.red, e.host, const_cast<double*>(e.sgsum),
-                const_cast<double*>(e.sgsol)};
+    PassFinal a{e.f.part, nblocks, nvals, ncopy, e.f.red, e.host, const_cast<double*>(e.sgsum), e.dscale, e.counter,
+                e.f.V};
     if (nch > 1) {
         a.part = e.f.part + (size_t)nvals * nblocks;
         a.n = nch;
@@ -222,9 +240,9 @@ void launch_pass_sums_batch(const FftWork* w, int nrve, int nblocks, int nvals, 
                           ncopy, nch);
 }
 void launch_pass_sums(double* part, int nblocks, int nvals, int ncopy, double* red, double* host, double* sgsum,
-                      double* sgsol, hipStream_t s, hipEvent_t done) {
+                      double* dscale, unsigned* counter, long V, hipStream_t s, hipEvent_t done) {
     const int nch = (nblocks + RCH - 1) / RCH;
-    PassFinal a{part, nblocks, nvals, ncopy, red, host, sgsum, sgsol};
+    PassFinal a{part, nblocks, nvals, ncopy, red, host, sgsum, dscale, counter, V};
     if (nch > 1) {
         double* tmp = part + (size_t)nvals * nblocks;
         hipLaunchKernelGGL(k_reduce_cols, dim3(nch, nvals), dim3(BS), 0, s, part, nblocks, tmp);

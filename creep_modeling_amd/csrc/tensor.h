@@ -180,7 +180,7 @@ CG_HD double vm_sq(const double* d) {
 CG_HD double vm_strain(const double* d) { return sqrt(vm_sq(d) * (2.0 / 3.0)); }
 CG_HD double vm_stress(const double* d) { return sqrt(vm_sq(d) * 1.5); }
 // the mixed form's stress scale: max(von Mises stress of the previous pass' solid stress sum, 1)
-CG_HD double mixed_scale(const double* sgsol) { return fmax(vm_stress(sgsol), 1.0); }
+CG_HD double mixed_scale(const double* sgsum_solid) { return fmax(vm_stress(sgsum_solid), 1.0); }
 // von Mises strain of a symmetric tensor given in the b-basis: the deviatoric
 // 5-vector norm (the b-basis is orthonormal): sqrt(2/3 |dev|^2)
 CG_HD double vm_strain6(const double* s) {
