@@ -252,10 +252,11 @@ static int check_singular(cg_ctx* c, const char* where) {
     return singular_from(c, c->h_red, where);
 }
 
-// reduce nvals partial columns into h_red[0..nvals) (synchronous)
+// reduce nvals partial columns into host_out[0..nvals) (synchronous); h_red then holds red[0,
+// R_NCOPY), the singular-system flags included
 static int reduce_to_host(cg_ctx* c, int nblocks, int nvals, double* host_out) {
     launch_reduce_final(c->f.part, nblocks, nvals, c->f.red, c->s);
-    HIPCHK(c, hipMemcpyAsync(c->h_red, c->f.red, nvals * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->f.red, R_NCOPY * sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
     memcpy(host_out, c->h_red, nvals * sizeof(double));
     return 0;
@@ -1323,17 +1324,10 @@ int cg_end_increment(cg_ctx* c, const cg_process* p, int scan_diagnostics, cg_in
     }
     int nb;
     double r[128];
-    // accumulation (LS-EVPFFT.f90:431-446)
-    {
-        launch_accumulate(f, tdot, c->s, &nb);
-        // max reduction through the partials: read all partials is avoided by a final op kernel
-        std::vector<double> hp(nb);
-        HIPCHK(c, hipMemcpyAsync(hp.data(), f.part, nb * sizeof(double), hipMemcpyDeviceToHost, c->s));
-        HIPCHK(c, hipStreamSynchronize(c->s));
-        double mx = c->edotvmmx;
-        for (int b = 0; b < nb; b++) mx = std::max(mx, hp[b]);
-        c->edotvmmx = mx;
-    }
+    // accumulation (LS-EVPFFT.f90:431-446); the largest strain rate stays on the device until the
+    // averages' copy below
+    constexpr int R_EDOTMX = 100, R_LAST = 101;  // free slots of red (13 values from R_LAST)
+    launch_accumulate(f, tdot, f.red + R_EDOTMX, c->s, &nb);
     if (p->iupdate == 1) {
         launch_plastic_defgrad(f, pt, tdot, c->s);
         // update_grid_velgrad_node: dv in the (now free) dLref buffer
@@ -1371,16 +1365,16 @@ int cg_end_increment(cg_ctx* c, const cg_process* p, int scan_diagnostics, cg_in
     c->dvmavg = vm_strain(vs);
     c->evmavg += c->dvmavg * tdot;
     launch_averages(f, pt, c->s, &nb);
-    if (reduce_to_host(c, nb, 50, r)) return -2;
-    // eelavg = eavg + (disgradsym - ept) w of the last solid voxel in loop order (:482)
+    // eelavg = eavg + (disgradsym - ept) w of the last solid voxel in loop order (:482): its 13
+    // values travel with the averages and the largest strain rate, in one copy
+    launch_reduce_final(f.part, nb, 50, f.red, c->s);
+    if (c->last_solid >= 0) launch_pick_last(f, c->last_solid, f.red + R_LAST, c->s);
+    HIPCHK(c, hipMemcpyAsync(c->h_red, f.red, (R_LAST + 13) * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    memcpy(r, c->h_red, 50 * sizeof(double));
+    c->edotvmmx = std::max(c->edotvmmx, c->h_red[R_EDOTMX]);
     double last[13] = {0};
-    if (c->last_solid >= 0) {
-        for (int k = 0; k < 6; k++) {
-            HIPCHK(c, hipMemcpy(&last[k], f.dgs + k * f.ld + c->last_solid, sizeof(double), hipMemcpyDeviceToHost));
-            HIPCHK(c, hipMemcpy(&last[6 + k], f.ept + k * f.ld + c->last_solid, sizeof(double), hipMemcpyDeviceToHost));
-        }
-        HIPCHK(c, hipMemcpy(&last[12], f.w + c->last_solid, sizeof(double), hipMemcpyDeviceToHost));
-    }
+    if (c->last_solid >= 0) memcpy(last, c->h_red + R_LAST, sizeof last);
     double dl6[6], dl3[9];
     for (int k = 0; k < 6; k++) dl6[k] = last[k] - last[6 + k];
     from6(dl6, dl3);
@@ -1411,7 +1405,8 @@ int cg_end_increment(cg_ctx* c, const cg_process* p, int scan_diagnostics, cg_in
         const double lfloor = std::max(1.0e-12 * Lref, 2.2250738585072014e-308);
         if (ensure_part(c, (size_t)((V + 255) / 256) * 24 + 64)) return -3;
         launch_diagnostics(f, lfloor, scan_diagnostics, c->s, &nb);
-        HIPCHK(c, hipMemcpyAsync(c->h_red, f.red, 23 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        // with the singular-system flags of the updates above (the diagnostics raise none)
+        HIPCHK(c, hipMemcpyAsync(c->h_red, f.red, R_NCOPY * sizeof(double), hipMemcpyDeviceToHost, c->s));
         HIPCHK(c, hipStreamSynchronize(c->s));
         const double* d = c->h_red;
         const double HUGEV = 1.7976931348623157e308;
@@ -1446,7 +1441,8 @@ int cg_end_increment(cg_ctx* c, const cg_process* p, int scan_diagnostics, cg_in
     HIPCHK(c, hipStreamSynchronize(c->s));
     HIPCHK(c, hipGetLastError());
     tcollect(c, 3);
-    return check_singular(c, "update_defgrad / update_elstic_defgrad / update_el_stiff / update_tensor_config");
+    const char* where = "update_defgrad / update_elstic_defgrad / update_el_stiff / update_tensor_config";
+    return out ? singular_from(c, c->h_red, where) : check_singular(c, where);
 }
 
 int cg_calc_c0(cg_ctx* c, const cg_process* p) {
@@ -1460,7 +1456,7 @@ int cg_calc_c0(cg_ctx* c, const cg_process* p) {
     if (reduce_to_host(c, nb, 72, r)) return -2;
     tstop(c, 3);
     tcollect(c, 3);
-    if (int e = check_singular(c, "calc_c0")) return e;
+    if (int e = singular_from(c, c->h_red, "calc_c0")) return e;
     double dsde[36], dP[36], a66[36];
     for (int i = 0; i < 36; i++) {
         dsde[i] = r[i] / c->wphcsol;

@@ -482,7 +482,10 @@ struct FftCtx {
 // once per increment; the stream has passed the previous upload by then, as every pass waits
 // for its sums).
 static const GreenDev* green_dev(FftCtx* c, const GreenDev& g, hipStream_t s) {
-    if (!c->g_valid || memcmp(c->h_g, &g, sizeof g) != 0) {
+    // compared member by member: the struct's tail padding is not part of the value
+    const bool same = c->g_valid && c->h_g->igamma == g.igamma && memcmp(c->h_g->c0, g.c0, sizeof g.c0) == 0 &&
+                      memcmp(c->h_g->s0, g.s0, sizeof g.s0) == 0;
+    if (!same) {
         *c->h_g = g;
         hipMemcpyAsync(c->d_g, c->h_g, sizeof g, hipMemcpyHostToDevice, s);
         c->g_valid = 1;

@@ -125,7 +125,10 @@ void launch_sch_from_A(const Fields& f, const PhaseTables& pt, hipStream_t s);
 void launch_c066mod(const Fields& f, const double* c066, const double* goperr0, int modal, hipStream_t s);
 void launch_gas_compl(const Fields& f, double tdot, double complgas, int modal, hipStream_t s);
 void launch_goperr0(const Fields& f, const double* c0, const double* s0, int igamma, hipStream_t s, int* nb);
-void launch_accumulate(const Fields& f, double tdot, hipStream_t s, int* nb);
+// max_out: the largest von Mises strain rate of a solid voxel (device)
+void launch_accumulate(const Fields& f, double tdot, double* max_out, hipStream_t s, int* nb);
+// out[0..13): disgradsym (6), ept (6) and w of voxel v
+void launch_pick_last(const Fields& f, long v, double* out, hipStream_t s);
 void launch_plastic_defgrad(const Fields& f, const PhaseTables& pt, double tdot, hipStream_t s);
 void launch_update_defgrad(const Fields& f, double tdot, hipStream_t s, int* nb);
 void launch_div_w(const Fields& f, double detfavg, hipStream_t s);

@@ -1544,9 +1544,20 @@ __global__ __launch_bounds__(BS) void k_accumulate(Fields f, double tdot) {
     }
     block_reduce_store<1, BS, OpMax>(acc, f.part);
 }
-void launch_accumulate(const Fields& f, double tdot, hipStream_t s, int* nb) {
+void launch_accumulate(const Fields& f, double tdot, double* max_out, hipStream_t s, int* nb) {
     *nb = nblk(f.V);
     hipLaunchKernelGGL(k_accumulate, dim3(*nb), dim3(BS), 0, s, f, tdot);
+    hipLaunchKernelGGL(k_reduce_final_op<OpMax>, dim3(1), dim3(BS), 0, s, f.part, *nb, 1, max_out);
+}
+// the 13 values of voxel v that eelavg takes from the last solid voxel: disgradsym (6), ept (6), w
+__global__ void k_pick_last(Fields f, long v, double* __restrict__ out) {
+    const int k = threadIdx.x;
+    if (k < 6) out[k] = f.dgs[k * f.ld + v];
+    else if (k < 12) out[k] = f.ept[(k - 6) * f.ld + v];
+    else if (k == 12) out[k] = f.w[v];
+}
+void launch_pick_last(const Fields& f, long v, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_pick_last, dim3(1), dim3(64), 0, s, f, v, out);
 }
 
 // update_plastic_defgrad (various_functions.f90:1748-1827)
