@@ -55,6 +55,11 @@ __device__ __forceinline__ void flag_singular(const Fields& f, long v) {
 }
 
 // |x|^(n-1) for an integer n (exponentiation by squaring) or through xpow (exp(y ln x)).
+// byte v of a per-voxel byte array of V entries through a buffer resource that ends at V
+// (an index past V reads 0)
+__device__ __forceinline__ int ld_byte(const uint8_t* a, long V, long v) {
+    return __builtin_amdgcn_raw_buffer_load_b8(Vox::rs(a, (uint32_t)V), (int)v, 0, 0);
+}
 template <bool INTPOW>
 __device__ __forceinline__ double powm1(double x, double n) {
     if (INTPOW) {
@@ -738,16 +743,20 @@ template <bool INTPOW, bool SYMTAU, bool SMALLNS>
 // factors of cm^-1, read once per evaluation for the residual norm |cm R'| (keeping them out
 // of the registers keeps the Newton state within two waves per SIMD).
 // sal: this thread's column of an LDS array [18][SBS] that receives A, A^-1 (schA).
+// pA, plam, psgt, pF, pS: A = Fe ag, lambda, sgt and the 42 operators of k_solve_ops, loaded by
+// the caller before phase A (their loads overlap phase A's).
 __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTables& pt, const PassConst& pc,
                                                 const Vox& x, int p0, double* cml, double* sal, double* sgf,
-                                                double* acc, double w, const double* Lmix) {
+                                                double* acc, double w, const double* Lmix, const double* pA,
+                                                const double* plam, const double* psgt, const double* pF,
+                                                const double* pS) {
     const cphase& P = ((cphase*)pt.ph)[p0];
     cdbl* tab = (cdbl*)pt.data;
     const double tdot = pc.tdot;
     {  // A = Fe ag from memory; A^-1 by the same inv3 as update_schmid's
         double Am[9], Ai[9];
 #pragma unroll
-        for (int k = 0; k < 9; k++) Am[k] = x.ld(f.schA, k);
+        for (int k = 0; k < 9; k++) Am[k] = pA[k];
         inv3(Am, Ai);
 #pragma unroll
         for (int k = 0; k < 9; k++) {
@@ -759,8 +768,8 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
     {
         double lam[6], sgt[6], D[6];
         for (int c = 0; c < 6; c++) {
-            lam[c] = x.ld(f.sg, c);
-            sgt[c] = x.ld(f.sgt, c);
+            lam[c] = plam[c];
+            sgt[c] = psgt[c];
             sg[c] = lam[c];  // the solve starts from lambda
         }
         to6(Lmix, D);
@@ -774,7 +783,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
         {
             double F[21], dk[6];
 #pragma unroll
-            for (int i = 0; i < 21; i++) F[i] = x.ld(f.sop, i);
+            for (int i = 0; i < 21; i++) F[i] = pF[i];
 #pragma unroll
             for (int i = 0; i < 21; i++) cml[i * SBS] = F[i];
 #pragma unroll
@@ -796,7 +805,7 @@ __device__ __forceinline__ void solve_solid_sym(const Fields& f, const PhaseTabl
         for (int i = 0; i < 6; i++)
 #pragma unroll
             for (int j = i; j < 6; j++) {
-                const double sij = x.ld(f.sop, 21 + sp6(i, j));
+                const double sij = pS[sp6(i, j)];
                 A[sp6(i, j)] += sij;
                 b[i] += sij * sgt[j];
                 if (j != i) b[j] += sij * sgt[i];
@@ -954,10 +963,36 @@ __device__ __forceinline__ void correct_solve_block(const Fields& f, const Phase
     const Vox x(v, f.ld);
     double* const cm_lds = pool;
     double* const sa_lds = pool + 21 * SBS;
-    // ---- phase A: correction_to_current + dL_iter/mixing; its 31 sums are reduced
-    // before the constitutive solve so their accumulators are not live across it
+    // Every load that depends on nothing is issued first, by every lane and in one basic block,
+    // so that the loads of phase A and of the solve's inputs are in flight together: the
+    // weight, the gas and phase bytes, L, F^-1 and dLref and, for the symmetric solve,
+    // A = Fe ag, lambda and sgt; the 42 per-increment operators follow at the end of phase A's
+    // arithmetic, ahead of its block reduction.  Lanes past V read column padding (every column
+    // has at least 256 doubles past V) and bytes past V read 0 (the byte resources end at V);
+    // they compute and store nothing.
+    const double w0 = x.ld(f.w, 0);
+    const int gasb = ld_byte(f.gas, V, v), ph = ld_byte(f.phase, V, v);
+    double pA[9], plam[6], psgt[6], pF[21], pS[21];
+    if (!MODAL) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) pA[k] = x.ld(f.schA, k);
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            plam[c] = x.ld(f.sg, c);
+            psgt[c] = x.ld(f.sgt, c);
+        }
+    }
+    double L[9], Fi[9], t[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+        L[c] = x.ld(f.L, c);
+        Fi[c] = x.ld(f.Fi, c);
+        t[c] = x.ld(f.dLref, c);
+    }
     double w = 0.0, Lmix[9];  // Lmix: this voxel's velocity gradient after the mixing (phase A)
     bool gas = false;
+    // ---- phase A: correction_to_current + dL_iter/mixing; its 31 sums are reduced
+    // before the constitutive solve so their accumulators are not live across it
     {
         double accA[R_NSOLVE - R_NPHASEB];
 #pragma unroll
@@ -970,12 +1005,10 @@ __device__ __forceinline__ void correct_solve_block(const Fields& f, const Phase
                 for (int i = 0; i < 9; i++) dm[i] = a.dvelgradavg[i] - a.fluct[i];
                 mm3(dm, a.finv_inv, dud);
             }
-            double L[9], Lo[9], Fi[9], t[9];
-            for (int c = 0; c < 9; c++) {
-                L[c] = x.ld(f.L, c);
-                Fi[c] = x.ld(f.Fi, c);
-                t[c] = x.ld(f.dLref, c) + dud[c];
-            }
+            w = w0;
+            gas = gasb != 0;
+            double Lo[9];
+            for (int c = 0; c < 9; c++) t[c] += dud[c];
             if (pc.lold_same) {
                 for (int c = 0; c < 9; c++) Lo[c] = L[c];
             } else {
@@ -984,8 +1017,6 @@ __device__ __forceinline__ void correct_solve_block(const Fields& f, const Phase
             double q[9];
             mm3(t, Fi, q);
             for (int c = 0; c < 9; c++) L[c] += q[c];
-            w = x.ld(f.w, 0);
-            gas = f.gas[v] != 0;
             for (int c = 0; c < 9; c++) {
                 accA[R_LALL - R_NPHASEB + c] = L[c] * w;
                 accA[(gas ? R_LGAS : R_LSOL) - R_NPHASEB + c] = L[c] * w;
@@ -1008,6 +1039,13 @@ __device__ __forceinline__ void correct_solve_block(const Fields& f, const Phase
                 Lmix[c] = L[c];
             }
         }
+        if (!MODAL) {
+#pragma unroll
+            for (int i = 0; i < 21; i++) {
+                pF[i] = x.ld(f.sop, i);
+                pS[i] = x.ld(f.sop, 21 + i);
+            }
+        }
         if (MODAL) {
             block_reduce_store_cm<R_NSOLVE - R_NPHASEB, SBS>(accA, f.part, R_NPHASEB, blk, nblk);
         } else {  // scratch aliased onto the solve's LDS columns, free until phase B
@@ -1025,12 +1063,11 @@ __device__ __forceinline__ void correct_solve_block(const Fields& f, const Phase
         if (!gas && !MODAL) {
             // one pass per distinct phase among the wave's solid lanes (one for a
             // single-phase RVE); the phase id is then uniform inside the solve
-            const int ph = f.phase[v];
             for (;;) {
                 const int p0 = __builtin_amdgcn_readfirstlane(ph);
                 if (ph == p0) {
                     solve_solid_sym<INTPOW, SYMTAU, SMALLNS>(f, pt, pc, x, p0, cm_lds + threadIdx.x, sa_lds + threadIdx.x, sgf,
-                                                    acc, w, Lmix);
+                                                    acc, w, Lmix, pA, plam, psgt, pF, pS);
                     break;
                 }
             }
