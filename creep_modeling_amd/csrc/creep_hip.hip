@@ -148,6 +148,7 @@ struct cg_ctx {
     int sop_valid = 0;     // f.sop holds the operators for sop_tdot and the current c066mod, sg66
     int taun_valid = 0;    // f.taun = |ttau| (false after ttau was written other than by a harden)
     double sop_tdot = 0.0;
+    int snap_valid = 0;   // the increment's snapshot was taken (it can roll back)
     int sgsum_valid = 0, lold_same = 0, int_pow = 0, sym_tau = 0, small_ns = 0, ngas = 0, n3node = 1;
     long last_solid = -1;
     double nsys_solid = 0.0;
@@ -690,7 +691,13 @@ int cg_begin_increment(cg_ctx* c, const cg_process* p, int do_schmid) {
     tstart(c, 3);
     memcpy(c->dsim_snap, p->dsim, sizeof c->dsim_snap);
     memcpy(c->scauchy_snap, p->scauchy, sizeof c->scauchy_snap);
-    launch_snapshot(c->f, 0, c->s);
+    // take_snapshot copies 66 columns per voxel (1.1 GB at 128^3).  Only an increment under
+    // Delta-t control with tdot > tdotmin can roll back (LS-EVPFFT.f90:361-368: the cut-back
+    // needs tdot > tdotmin, and tdot only shrinks within an increment), so only those copy
+    // every column; the others copy L alone, which the diagnostics compare with the increment's
+    // end.  cg_rollback refuses an increment without the full snapshot.
+    c->snap_valid = (p->ictrl == -1 && p->tdot > p->tdotmin) ? 1 : 0;
+    launch_snapshot(c->f, 0, c->s, c->snap_valid);
     PhaseTables pt{c->d_ph, c->d_tab};
     if (do_schmid) launch_update_schmid(c->f, pt, c->modal, c->s);
     if (c->modal) {  // form_G: Goperr0 = (1/V) sum Gamma(xi)
@@ -1285,6 +1292,8 @@ int cg_batch_iterate(cg_batch* b, cg_process* const* p, cg_iter_state* st, cg_it
 int cg_rollback(cg_ctx* c, cg_process* p) {
     drop_early(c);
     if (!c || !c->initialised || !p) return -1;
+    if (!c->snap_valid)
+        return fail(c, -1, "cg_rollback: this increment took no snapshot (only ictrl = -1 with tdot > tdotmin can roll back)");
     memcpy(p->dsim, c->dsim_snap, sizeof c->dsim_snap);
     memcpy(p->scauchy, c->scauchy_snap, sizeof c->scauchy_snap);
     launch_snapshot(c->f, 1, c->s);

@@ -148,7 +148,8 @@ void launch_kinematic(const Fields& f, double tdot, hipStream_t s, int* nb);
 void launch_kinematic_raw(const double* L, long ld, const double* J, long V, double tdot, double* part, double* red,
                           hipStream_t s);
 long kinematic_scratch_doubles(long V);
-void launch_snapshot(const Fields& f, int restore, hipStream_t s);
+// take_snapshot's copies (restore: back); full = 0: L only (the diagnostics compare L with it)
+void launch_snapshot(const Fields& f, int restore, hipStream_t s, int full = 1);
 void launch_init_voxels(const Fields& f, const PhaseTables& pt, const double* euler, const int* grain,
                         const double* hp_voxel, const double* udot, double tdot, int keep_defgrad, hipStream_t s,
                         int* nb);

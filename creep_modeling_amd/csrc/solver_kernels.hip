@@ -2181,12 +2181,13 @@ void launch_diagnostics(const Fields& f, double lfloor, int scan_cond, hipStream
 }
 
 // take_snapshot / rollback (:2839-2895)
-void launch_snapshot(const Fields& f, int restore, hipStream_t s) {
+void launch_snapshot(const Fields& f, int restore, hipStream_t s, int full) {
     const size_t V = f.ld, ns = f.ns;  // whole columns, padding included
     struct { double* live; double* snap; size_t n; } T[] = {
         {f.L, f.s_L, 9 * V}, {f.Lold, f.s_Lold, 9 * V}, {f.sg, f.s_sg, 6 * V},
         {f.edotp, f.s_edotp, 6 * V}, {f.gdot, f.s_gdot, ns * V}, {f.ttau, f.s_ttau, 2 * ns * V}};
     for (auto& t : T) {
+        if (!full && t.live != f.L) continue;
         if (restore) hipMemcpyAsync(t.live, t.snap, t.n * sizeof(double), hipMemcpyDeviceToDevice, s);
         else hipMemcpyAsync(t.snap, t.live, t.n * sizeof(double), hipMemcpyDeviceToDevice, s);
     }

@@ -114,7 +114,8 @@ int cg_set_displacement(cg_ctx *ctx, int idispini, const double *disp);
  * (LS-EVPFFT.f90:156-173) for the first process p1. */
 int cg_init(cg_ctx *ctx, const cg_process *p1);
 
-/* take_snapshot (various_functions.f90:2839), update_schmid (:472) if
+/* take_snapshot (various_functions.f90:2839; every field only when the increment can roll
+ * back, ictrl = -1 and tdot > tdotmin, else L alone), update_schmid (:472) if
  * do_schmid, form_G (:609; Gamma is evaluated on the fly in the FFT stage and
  * Goperr0 by a reduction) and calc_c066mod_Goperr066mod (:812). */
 int cg_begin_increment(cg_ctx *ctx, const cg_process *p, int do_schmid);
@@ -144,7 +145,8 @@ int cg_batch_iterate(cg_batch *b, cg_process *const *p, cg_iter_state *st, cg_it
  * other one's results. */
 int cg_batch_launch(cg_batch *b, cg_process *const *p, cg_iter_state *st, cg_iter_out *out, const int *active);
 int cg_batch_wait(cg_batch *b, int *rc);
-/* rollback (various_functions.f90:2868-2895), restoring p->dsim / p->scauchy. */
+/* rollback (various_functions.f90:2868-2895), restoring p->dsim / p->scauchy; -1 when the
+ * increment took no snapshot (it began with ictrl != -1 or tdot <= tdotmin). */
 int cg_rollback(cg_ctx *ctx, cg_process *p);
 /* kinematic_increments + kinematic_substep (various_functions.f90:2745-2837):
  * out[4] = max volinc, max rotang (deg), max stretchinc, min detF_next; returns nsub in *nsub. */

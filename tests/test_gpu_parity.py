@@ -135,6 +135,29 @@ def test_substep_rollback_path_matches_oracle():
     check_rows(rows)
 
 
+def test_rollback_refused_without_a_snapshot():
+    # with tdot == tdotmin no cut-back can happen, so the increment copies only L for its
+    # snapshot; cg_rollback must then refuse instead of restoring stale fields
+    import ctypes as C
+
+    from creep_modeling_amd import driver
+
+    s = driver.Solver(problem(0, n=8, ngrains=4, nsteps=2))
+    try:
+        assert s.begin()
+        assert s.P.tdot <= s.P.tdotmin
+        assert s.L.cg_rollback(s.h, C.byref(s.P)) == -1
+        assert b"no snapshot" in s.L.cg_last_error(s.h)
+        # the increment still runs to the end after the refused call
+        while True:
+            s._chk(s.L.cg_iterate(s.h, C.byref(s.P), C.byref(s.st), C.byref(s.out)), "cg_iterate")
+            if s.pass_done():
+                break
+        assert s.finish()["passes"] > 0
+    finally:
+        s.close()
+
+
 @pytest.mark.parametrize("n", [8, 16])
 def test_gas_phase_matches_oracle(n):
     from creep_modeling_amd import parse
