@@ -255,8 +255,9 @@ static int check_singular(cg_ctx* c, const char* where) {
 
 // reduce nvals partial columns into host_out[0..nvals) (synchronous); h_red then holds red[0,
 // R_NCOPY), the singular-system flags included
-static int reduce_to_host(cg_ctx* c, int nblocks, int nvals, double* host_out) {
-    launch_reduce_final(c->f.part, nblocks, nvals, c->f.red, c->s);
+// done: red already holds the sums (a column-major reduction ran)
+static int reduce_to_host(cg_ctx* c, int nblocks, int nvals, double* host_out, int done = 0) {
+    if (!done) launch_reduce_final(c->f.part, nblocks, nvals, c->f.red, c->s);
     HIPCHK(c, hipMemcpyAsync(c->h_red, c->f.red, R_NCOPY * sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
     memcpy(host_out, c->h_red, nvals * sizeof(double));
@@ -1340,8 +1341,10 @@ int cg_end_increment(cg_ctx* c, const cg_process* p, int scan_diagnostics, cg_in
     if (p->iupdate == 1) {
         launch_plastic_defgrad(f, pt, tdot, c->s);
         // update_grid_velgrad_node: dv in the (now free) dLref buffer
-        nb = fft_integrate(f, c->fft, p->igamma, f.dLref, c->s);
-        if (reduce_to_host(c, nb, 9, r)) return -2;
+        int cm = 0;
+        nb = fft_integrate(f, c->fft, p->igamma, f.dLref, c->s, &cm);
+        if (cm) launch_reduce_final_cm(f.part, nb, 9, f.red, c->s);
+        if (reduce_to_host(c, cm ? 0 : nb, 9, r, cm)) return -2;
         launch_xnode(f, f.dLref, r, tdot, p->igamma, c->n3node, c->s);
         launch_update_defgrad(f, tdot, c->s, &nb);
         if (reduce_to_host(c, nb, 39, r)) return -2;

@@ -792,7 +792,17 @@ __global__ __launch_bounds__(FBS) void k_x_inv3(Fields f, Plan3 p, int rows, dou
     }
 }
 
-int fft_integrate(const Fields& f, FftCtx* c, int ideriv, double* dv, hipStream_t s) {
+static int integrate_reg(const Fields& f, const double2* const* tw, const FftTune& t, int ideriv, double* dv,
+                         hipStream_t s);
+int fft_integrate(const Fields& f, FftCtx* c, int ideriv, double* dv, hipStream_t s, int* cm) {
+    if (fft_tunable(f, c)) {  // the register passes, partial sums column-major
+        const void* tw[3];
+        fft_twiddles(c, 0, tw);
+        const double2* t2[3] = {(const double2*)tw[0], (const double2*)tw[1], (const double2*)tw[2]};
+        *cm = 1;
+        return integrate_reg(f, t2, c->tune, ideriv, dv, s);
+    }
+    *cm = 0;
     const int rows = c->rows;
     const int nrow = f.n2 * f.n3 / rows;
     const size_t shx = 2 * (size_t)rows * 9 * (f.n1 + 1) * sizeof(double2);
@@ -900,9 +910,12 @@ template <typename R> __device__ __forceinline__ typename Cx<R>::T* spec_of(cons
 // x forward: PK1 = J (sigma - <sigma>) F^-T (LS-EVPFFT.f90:256-269), r2c along x.
 // Components (2p, 2p+1) of a row form one complex signal; the two half spectra are
 // separated with X[N-k] afterwards.
-template <int N, typename R>
+// LF (the nodal integration, update_grid_velgrad_node :1970-2056): the transformed field is
+// L F instead, and the block's sums of L F / V go to f.part column-major (nbs blocks).
+template <int N, typename R, bool LF = false>
 __device__ __forceinline__ void x_fwd_blk(const Fields& f, const typename Cx<R>::T* __restrict__ tw,
-                                          const double* __restrict__ sgsum, const double* __restrict__ dscale, int lb) {
+                                          const double* __restrict__ sgsum, const double* __restrict__ dscale, int lb,
+                                          int nbs = 0) {
     // 1 / the mixed form's stress scale (1 in FP64)
     const double inv_scale = sizeof(R) == 8 ? 1.0 : dscale[0];
     using T2 = typename Cx<R>::T;
@@ -918,8 +931,11 @@ __device__ __forceinline__ void x_fwd_blk(const Fields& f, const typename Cx<R>:
     const long V = f.V;
     const long nrows = (long)f.n2 * f.n3;
     const long row0 = (long)lb * RW;  // the block's rows
-    double sa[9];
-    for (int c = 0; c < 9; c++) sa[c] = sgsum[c];
+    double sa[9], lfs[9];
+    for (int c = 0; c < 9; c++) {
+        sa[c] = LF ? 0.0 : sgsum[c];
+        lfs[c] = 0.0;
+    }
     // each thread's voxels: every load is issued before the products (rows past the grid
     // read the last row and are zeroed)
     constexpr int NE = (RW * N + XBS - 1) / XBS;
@@ -930,6 +946,20 @@ __device__ __forceinline__ void x_fwd_blk(const Fields& f, const typename Cx<R>:
         const int r = e / N, i = e - r * N;
         const long row = row0 + r;
         const long v = (row < nrows ? row : nrows - 1) * N + i;
+        if constexpr (LF) {
+            double Lv[9], Fv[9], lf[9];
+            for (int c = 0; c < 9; c++) {
+                Lv[c] = f.L[c * f.ld + v];
+                Fv[c] = f.F[c * f.ld + v];
+            }
+            mm3(Lv, Fv, lf);
+            const bool in = row < nrows;
+            for (int c = 0; c < 9; c++) {
+                stage[(r * 9 + c) * SP + i] = in ? (R)lf[c] : (R)0;
+                if (in) lfs[c] += lf[c] * (1.0 / (double)V);
+            }
+            continue;
+        }
         double s6[6], s[9], Fi[9], pk[9];
         for (int c = 0; c < 6; c++) s6[c] = f.sg[c * f.ld + v];
         for (int c = 0; c < 9; c++) Fi[c] = f.Fi[c * f.ld + v];
@@ -987,6 +1017,7 @@ __device__ __forceinline__ void x_fwd_blk(const Fields& f, const typename Cx<R>:
         else o = T2{(R)0.5 * (Z.y + Zm.y), (R)-0.5 * (Z.x - Zm.x)};           // (Z - conj Zm)/(2i)
         spec[row * w + rem] = o;
     }
+    if constexpr (LF) block_reduce_store_cm<9, XBS>(lfs, f.part, 0, lb, nbs);
 }
 template <int N, typename R>
 __global__ __launch_bounds__((XDims<N, R>::XB)) CG_XF_ATTR void k_x_fwd_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
@@ -1053,10 +1084,47 @@ __global__ __launch_bounds__(YB) CG_Y_ATTR void k_y_bat(const FftWork* __restric
     y_blk<N, INV, R, YB>(f, (const typename Cx<R>::T*)e.tw[1], ntile, lb);
 }
 
-// z pass with the Green operator: KT whole kx columns (9 components each) of one y row
-template <int N, typename R, bool GENERAL, bool KT1>
-__device__ __forceinline__ void z_blk(const Fields& f, const typename Cx<R>::T* __restrict__ tw, const GreenDev& g,
-                                      int ntile, int bid) {
+// The integration operator of the nodal update at one frequency (integrate_tensor2
+// :359-429 with the derivative operator of update_grid_velgrad_node): u_l = sum_m K_m T_lm
+// with K = conj(k)/|k|^2 averaged with the mirror frequency's k(-xi)/|k(-xi)|^2 (on the
+// Nyquist planes k(-xi) is not -k(xi)), k = i xi (ideriv 0), i sin(xi) (1) or the rotated
+// operator (2, zero at the all-Nyquist point); u_l replaces component l < 3.  P: the nine
+// components T_lm (3l + m) of the point, st apart.
+template <typename T2>
+__device__ __forceinline__ void integ_apply(int ideriv, int kx, int ky, int kz, int n1, int n2, int n3, T2* P, int st) {
+    double Kr[3] = {0, 0, 0}, Ki[3] = {0, 0, 0};
+    for (int side = 0; side < 2; side++) {
+        double kr[3], ki[3];
+        if (side == 0) kmod_at(ideriv, kx, ky, kz, n1, n2, n3, kr, ki);
+        else kmod_at(ideriv, (n1 - kx) % n1, (n2 - ky) % n2, (n3 - kz) % n3, n1, n2, n3, kr, ki);
+        const double k2 = kr[0] * kr[0] + ki[0] * ki[0] + kr[1] * kr[1] + ki[1] * ki[1] + kr[2] * kr[2] + ki[2] * ki[2];
+        if (k2 > 0.0)
+            for (int m = 0; m < 3; m++) {
+                Kr[m] += 0.5 * kr[m] / k2;
+                Ki[m] += 0.5 * (side == 0 ? -ki[m] : ki[m]) / k2;
+            }
+    }
+    const bool allnyq = (n1 % 2 == 0 && kx == n1 / 2) && (n2 % 2 == 0 && ky == n2 / 2) && (n3 % 2 == 0 && kz == n3 / 2) &&
+                        n3 > 1;
+    double ur[3], ui[3];
+    for (int l = 0; l < 3; l++) {
+        double sr = 0.0, si = 0.0;
+        for (int m = 0; m < 3; m++) {
+            const T2 z = P[(3 * l + m) * st];
+            sr += Kr[m] * z.x - Ki[m] * z.y;
+            si += Kr[m] * z.y + Ki[m] * z.x;
+        }
+        ur[l] = (ideriv == 2 && allnyq) ? 0.0 : sr;
+        ui[l] = (ideriv == 2 && allnyq) ? 0.0 : si;
+    }
+    for (int l = 0; l < 3; l++) P[l * st] = T2{ur[l], ui[l]};
+}
+
+// z pass with the Green operator: KT whole kx columns (9 components each) of one y row.
+// INTEG: the nodal integration's operator (integ_apply) in place of Gamma.
+template <int N, typename R, bool GENERAL, bool KT1, bool INTEG = false>
+__device__ __forceinline__ void z_blk(const Fields& f, const typename Cx<R>::T* __restrict__ tw, const GreenDev* g,
+                                      int ntile, int bid, int ideriv = 0) {
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
     using Z = ZDims<N, KT1>;
@@ -1085,7 +1153,8 @@ __device__ __forceinline__ void z_blk(const Fields& f, const typename Cx<R>::T* 
     __syncthreads();
     for (int e = threadIdx.x; e < KTe * N; e += Z::ZB) {
         const int kz = e / KTe, kxl = e - kz * KTe;
-        green_apply<R, GENERAL>(g, kx0 + kxl, j, kz, f.n1, f.n2, N, ex + kz * NS + kxl * 9, 1);
+        if constexpr (INTEG) integ_apply(ideriv, kx0 + kxl, j, kz, f.n1, f.n2, N, ex + kz * NS + kxl * 9, 1);
+        else green_apply<R, GENERAL>(*g, kx0 + kxl, j, kz, f.n1, f.n2, N, ex + kz * NS + kxl * 9, 1);
     }
     __syncthreads();
 #pragma unroll
@@ -1105,7 +1174,7 @@ template <int N, typename R, bool GENERAL, bool KT1>
 __global__ __launch_bounds__((ZDims<N, KT1>::ZB)) CG_Z_ATTR void k_z_green_reg(Fields f, const typename Cx<R>::T* __restrict__ tw,
                                                                           const GreenDev* __restrict__ g, int ntile,
                                                                           int xcd) {
-    z_blk<N, R, GENERAL, KT1>(f, tw, *g, ntile, fft_block(xcd));
+    z_blk<N, R, GENERAL, KT1>(f, tw, g, ntile, fft_block(xcd));
 }
 template <int N, typename R, bool GENERAL, bool KT1>
 __global__ __launch_bounds__((ZDims<N, KT1>::ZB)) CG_Z_ATTR void k_z_bat(const FftWork* __restrict__ w, int ntile, int nbs) {
@@ -1113,15 +1182,17 @@ __global__ __launch_bounds__((ZDims<N, KT1>::ZB)) CG_Z_ATTR void k_z_bat(const F
     const int lb = entry_block(nbs, 0, &r);
     const FftWork& e = w[r];
     const Fields f = e.f;
-    z_blk<N, R, GENERAL, KT1>(f, (const typename Cx<R>::T*)e.tw[2], e.g, ntile, lb);
+    z_blk<N, R, GENERAL, KT1>(f, (const typename Cx<R>::T*)e.tw[2], &e.g, ntile, lb);
 }
 
 // x inverse (c2r): dLref = Re(IFFT)/V with the Hermitian continuation of the half
 // spectrum (imaginary parts at kx = 0 and n1/2 dropped, as Re() of the reference's
 // c2c inverse does); block partial sums of w dLref F^-1.
-template <int N, typename R>
+// LF (the nodal integration): components 0..2 of the result, /V, go to dv[c V + v]; no sums.
+template <int N, typename R, bool LF = false>
 __device__ __forceinline__ void x_inv_blk(const Fields& f, const typename Cx<R>::T* __restrict__ tw,
-                                          const double* __restrict__ dscale, int lb, int nbs) {
+                                          const double* __restrict__ dscale, int lb, int nbs,
+                                          double* __restrict__ dv = nullptr) {
     const double out_scale = sizeof(R) == 8 ? 1.0 / (double)f.V : dscale[1];  // stress scale / V
     using T2 = typename Cx<R>::T;
     using D = RegDims<N>;
@@ -1155,6 +1226,7 @@ __device__ __forceinline__ void x_inv_blk(const Fields& f, const typename Cx<R>:
     for (int q = 0; q < NE; q++) {
         const int e = threadIdx.x + q * XBS;
         if (RW * N % XBS != 0 && e >= RW * N) break;
+        if (LF) break;
         const int rr = e / N, i = e - rr * N;
         const long row = row0 + rr;
         const long v = (row < nrows ? row : nrows - 1) * N + i;
@@ -1202,6 +1274,18 @@ __device__ __forceinline__ void x_inv_blk(const Fields& f, const typename Cx<R>:
                 }
         }
     __syncthreads();
+    if constexpr (LF) {
+#pragma unroll
+        for (int q = 0; q < NE; q++) {
+            const int e = threadIdx.x + q * XBS;
+            if (RW * N % XBS != 0 && e >= RW * N) break;
+            const int rr = e / N, i = e - rr * N;
+            const long row = row0 + rr;
+            if (row < nrows)
+                for (int c = 0; c < 3; c++) dv[c * V + row * N + i] = (double)out[(rr * 9 + c) * SP + i] * out_scale;
+        }
+        return;
+    }
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int q = 0; q < NE; q++) {
@@ -1330,6 +1414,59 @@ static int reg_stage(int stage, const Fields& f, const typename Cx<R>::T* tw1, c
 #undef CG_Z
 #undef CG_YI
 #undef CG_XI
+    return nb;
+}
+
+// The nodal integration on the register passes (FP64 spectrum): x forward of L F with its
+// sums, the y passes of the stage, the z pass with integ_apply, x inverse to dv.
+template <int N>
+__global__ __launch_bounds__((XDims<N, double>::XB)) void k_x_fwd_lf_reg(Fields f, const double2* __restrict__ tw, int xcd) {
+    x_fwd_blk<N, double, true>(f, tw, nullptr, nullptr, fft_block(xcd), gridDim.x);
+}
+template <int N>
+__global__ __launch_bounds__((ZDims<N, false>::ZB)) void k_z_integ_reg(Fields f, const double2* __restrict__ tw, int ntile,
+                                                                      int ideriv, int xcd) {
+    z_blk<N, double, false, false, true>(f, tw, nullptr, ntile, fft_block(xcd), ideriv);
+}
+template <int N>
+__global__ __launch_bounds__((XDims<N, double>::XB)) void k_x_inv_lf_reg(Fields f, const double2* __restrict__ tw,
+                                                                       double* __restrict__ dv, int xcd) {
+    x_inv_blk<N, double, true>(f, tw, nullptr, fft_block(xcd), gridDim.x, dv);
+}
+static int integrate_reg(const Fields& f, const double2* const* tw, const FftTune& t, int ideriv, double* dv,
+                         hipStream_t s) {
+    int nb = 0;
+#define CG_IXF(N)                                                                                      \
+    {                                                                                                  \
+        using X = XDims<N, double>;                                                                    \
+        nb = (int)(((long)f.n2 * f.n3 + X::RW - 1) / X::RW);                                          \
+        hipLaunchKernelGGL((k_x_fwd_lf_reg<N>), dim3(nb), dim3(X::XB), 0, s, f, tw[0], t.xcd[0]);      \
+    }
+#define CG_IYF(N) launch_y_reg<N, false, double>(f, tw[1], t.yhalf[0], t.xcd[1], s, nullptr, nullptr)
+#define CG_IZ(N)                                                                                                 \
+    {                                                                                                            \
+        using Z = ZDims<N, false>;                                                                               \
+        const int ntile = (f.h1 + Z::KT - 1) / Z::KT;                                                            \
+        hipLaunchKernelGGL((k_z_integ_reg<N>), dim3(f.n2 * ntile), dim3(Z::ZB), 0, s, f, tw[2], ntile, ideriv,   \
+                           t.xcd[2]);                                                                            \
+    }
+#define CG_IYI(N) launch_y_reg<N, true, double>(f, tw[1], t.yhalf[1], t.xcd[3], s, nullptr, nullptr)
+#define CG_IXI(N)                                                                                                \
+    {                                                                                                            \
+        using X = XDims<N, double>;                                                                              \
+        const int nbi = (int)(((long)f.n2 * f.n3 + X::RW - 1) / X::RW);                                         \
+        hipLaunchKernelGGL((k_x_inv_lf_reg<N>), dim3(nbi), dim3(X::XB), 0, s, f, tw[0], dv, t.xcd[4]);           \
+    }
+    CG_SWITCH_N(f.n1, CG_IXF)
+    CG_SWITCH_N(f.n2, CG_IYF)
+    CG_SWITCH_N(f.n3, CG_IZ)
+    CG_SWITCH_N(f.n2, CG_IYI)
+    CG_SWITCH_N(f.n1, CG_IXI)
+#undef CG_IXF
+#undef CG_IYF
+#undef CG_IZ
+#undef CG_IYI
+#undef CG_IXI
     return nb;
 }
 

@@ -227,7 +227,8 @@ void fft_twiddles(const FftCtx* c, int mixed, const void** tw);
 void fft_set_tune(FftCtx* c, const FftTune& t);
 bool fft_tunable(const Fields& f, const FftCtx* c);
 // spectral integration of L F for the nodal update: dv (3 x V) and 9 block partial
-// sums of L F / V into f.part; returns the number of blocks.
-int fft_integrate(const Fields& f, FftCtx* c, int ideriv, double* dv, hipStream_t s);
+// sums of L F / V into f.part; returns the number of blocks, *cm = 1 when the partials are
+// column-major (part[i * nblocks + b]: the register passes), 0 when row-major.
+int fft_integrate(const Fields& f, FftCtx* c, int ideriv, double* dv, hipStream_t s, int* cm);
 
 }  // namespace cg
