@@ -16,6 +16,7 @@ bounded sample on this host.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -105,6 +106,45 @@ def run_gpu(problem, precision, steps, warmup, graphs=False):
     st_stage = s.ctx.stats()
     s.close()
     return rows, st, t1 - t0, st_stage
+
+
+def pass_costs(problem, steps, warmup):
+    """Per-pass cost of the constitutive kernel against its work: after `warmup` increments, every
+    pass of `steps` increments is timed with the kernel's own launch events and paired with the
+    device counters of that pass (Newton steps and outer hardening iterations per voxel); prints the
+    mean time per work class and the least-squares split t = a + b * NR + c * outer."""
+    import numpy as np
+    import torch
+
+    from creep_modeling_amd import driver
+
+    torch.cuda.init()
+    s = driver.Solver(problem, precision=0, scan_cond=True)
+    for i in range(warmup):
+        s.step()
+    s.L.cg_set_timing(s.h, 2)
+    rows = []
+    for i in range(steps):
+        assert s.begin()
+        s.ctx.stats(reset=True)
+        while True:
+            s._chk(s.L.cg_iterate(s.h, C.byref(s.P), C.byref(s.st), C.byref(s.out)), "cg_iterate")
+            st = s.ctx.stats(reset=True)
+            rows.append((st.t_ms[1], st.nr_steps / st.voxel_iters, st.outer / st.voxel_iters))
+            if s.pass_done():
+                break
+        s.finish()
+        print("increment %d: %d passes" % (warmup + i + 1, s.passes), file=sys.stderr, flush=True)
+    s.close()
+    R = np.array(rows)
+    classes = {}
+    for t, nr, ou in R:
+        classes.setdefault((round(nr, 2), round(ou, 2)), []).append(t)
+    fit = np.linalg.lstsq(np.c_[np.ones(len(R)), R[:, 1], R[:, 2]], R[:, 0], rcond=None)[0]
+    out = {"passes": len(R), "fit_ms": {"a": fit[0], "per_newton_step": fit[1], "per_outer_iteration": fit[2]},
+           "classes": [{"nr_per_voxel": k[0], "outer_per_voxel": k[1], "passes": len(v), "solve_ms": float(np.mean(v))}
+                       for k, v in sorted(classes.items(), key=lambda kv: -len(kv[1]))[:12]]}
+    print(json.dumps(out))
 
 
 def host_cpu():
@@ -298,6 +338,8 @@ def main():
     ap.add_argument("--cpu-config0", action="store_true",
                     help="run only the CPU oracle on the 32^3 / 64-grain RVE, all 50 increments, and exit")
     ap.add_argument("--checksums", action="store_true", help="print the sha256 of every workload's microstructure")
+    ap.add_argument("--pass-costs", action="store_true",
+                    help="time every pass of the constitutive kernel against its Newton and hardening work and exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -329,6 +371,9 @@ def main():
     V = problem.nvox
     ns = problem.phases[0].nsyst
 
+    if args.pass_costs:
+        pass_costs(problem, args.steps, args.warmup)
+        return
     rows, st, T, sst = run_gpu(problem, 0, args.steps, args.warmup, args.graphs)
     passes = sum(r["passes"] for r in rows)
     value = passes * V / T
@@ -340,6 +385,7 @@ def main():
     flops = flops_constitutive(st, ns)
     solve_tflops = flops / (st.t_ms[1] * 1e-3) / 1e12 if st.t_ms[1] > 0 else 0.0
     dominant_solve = t_solve >= t_fft
+    tr = measured("k_correct_solve", "bytes_per_launch", args)
     if dominant_solve:
         # the constitutive kernel runs on the FP64 vector ALU (no matrix-core instruction): its peak is
         # the FP64 vector rate; achieved = the flop model (counted Newton steps, evaluations and
@@ -356,7 +402,7 @@ def main():
                     frac=fft_gbs / HBM_PEAK_GBS, traffic=measured("fft_green", "bytes_per_launch", args),
                     avg_ms=t_fft)
     out = {
-        "metric": "voxel-iters/s at 128^3 creep RVE (mixed vs FP64 GPU) + % HBM roofline",
+        "metric": "voxel-iters/s at %d^3 creep RVE (mixed vs FP64 GPU) + %% HBM roofline" % args.n,
         "value": value,
         "unit": "voxel-iters/s",
         "n_gpus": 1,
@@ -386,6 +432,10 @@ def main():
                                "frac": fft_gbs / HBM_PEAK_GBS, "alg_bytes_per_voxel": fft_alg_bytes(args.n),
                                "traffic": measured("fft_green", "bytes_per_launch", args)},
         "constitutive": {"tflops": solve_tflops, "frac_fp64_peak": solve_tflops / FP64_PEAK_TFLOPS,
+                         # the same kernel seen from HBM: the counted traffic of its timed launches
+                         # (FETCH_SIZE / WRITE_SIZE, scaled) over the live average duration
+                         "hbm_gbs_counted": (tr / (t_solve * 1e-3) / 1e9) if tr else None,
+                         "hbm_frac_counted": (tr / (t_solve * 1e-3) / 1e9 / HBM_PEAK_GBS) if tr else None,
                          "nr_steps_per_voxel_iter": st.nr_steps / max(st.voxel_iters, 1),
                          "evals_per_voxel_iter": st.evals / max(st.voxel_iters, 1),
                          "pow_per_voxel_iter": st.evals * ns / max(st.voxel_iters, 1)},
