@@ -21,10 +21,17 @@ def test_fcc_increments_match_oracle():
         assert relerr(fo[k], fg[k]) < FIELD_TOL, k
 
 
-def test_xnode_matches_oracle():
+@pytest.mark.parametrize("n,igamma", [(8, 0), (16, 0)])
+def test_xnode_matches_oracle(n, igamma):
     # nodal coordinates after two increments (update_grid_velgrad_node, various_functions.f90:1970-2056):
-    # the displacement from the initial grid agrees to the field tolerance
-    p = problem(0, n=8, ngrains=4, nsteps=2)
+    # the displacement from the initial grid agrees to the field tolerance.  n = 8 runs the LDS
+    # transforms, n = 16 the register passes.  igamma = 1 and 2 are left out: there the reference's
+    # operator amplifies the rounding of its FFT at frequencies where k vanishes only in exact
+    # arithmetic, so the oracle's nodal coordinates carry its own rounding noise (measured 1e-3 to
+    # 6e-3 of the displacement against the kernels' Hermitian operator);
+    # tests/test_nodal_conditioning.py pins that on the CPU
+    p = problem(0, n=n, ngrains=4, nsteps=2)
+    p.processes[0].igamma = igamma
     rows, fo, fg = run_pair(p, 2, fields=("xnode",))
     check_rows(rows)
     n1, n2, n3 = p.n
